@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Throughput of the ZOLA-3D localization hot path on one MI355X.
+
+Workload: isolated-emitter MLE localization, 21x21-pixel ROIs, 64x64 pupil,
+astigmatic PSF from 28 Zernike modes (Noll 5 = 1 rad, Noll 4..28 ~ N(0, 0.05)),
+NA 1.45, lambda 680 nm, n 1.518, 110 nm pixels; emitters x, y ~ U(+-1 px),
+z ~ U[-1, 1] um, photons log-uniform 500..5000, background U[5, 50], Poisson
+noise, float32 pixels.  Synthetic data are generated on the GPU with the
+library's own PSF synthesis.
+
+One step = per-ROI initial values (template correlation + getAandB) and the
+batched Levenberg-Marquardt fit with analytic tangent Jacobians, for BATCH ROIs
+resident in HBM.  The plain per-ROI GPU form (the reference's coordinate
+descent) is timed on the same ROIs, and the reference CPU algorithm (the FP64
+restatement in oracle/) on a bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "zola-3d_amd"))
+
+F64 = 5 * 64 * 64 * np.log2(64 * 64)     # flops of one 64x64 complex FFT (5 N log2 N)
+PEAK_FP32 = 157.3e12                      # MI355X FP32 vector peak (AMD spec)
+DX = 0.11
+R = 21
+ZMIN_T, ZSTEP_T, NZ = -1.0, 0.1, 21       # initial-z templates at 100 nm
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--batch", type=int, default=65536)
+    p.add_argument("--plain-batch", type=int, default=16384)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--cpu-sample", type=int, default=0, help="ROIs for the CPU baseline (0: auto)")
+    return p.parse_args()
+
+
+def synth(ctx, torch, B, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    u = lambda: torch.rand(B, device="cuda", generator=g, dtype=torch.float64)
+    x = (2 * u() - 1) * DX
+    y = (2 * u() - 1) * DX
+    z = 2 * u() - 1
+    A = torch.exp(np.log(500) + (np.log(5000) - np.log(500)) * u())
+    bg = 5 + 45 * u()
+    par = torch.stack([x, y, z, torch.zeros_like(x)], 1).float().contiguous()
+    psf = torch.empty(B, R, R, device="cuda")
+    ctx.psf_forward_dev(par, psf)
+    mu = A.float()[:, None, None] * psf + bg.float()[:, None, None]
+    px = torch.poisson(mu, generator=g).float().contiguous()
+    truth = torch.stack([A, bg, x, y, z], 1)
+    return px, truth
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path shards ROIs across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import zola_hip as Z
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    coeffs = Z.astigmatic_coeffs(28)
+    ctx = Z.Context(Z.optics(), R, coeffs)
+    ctx.set_stream(stream.cuda_stream)
+    B = args.batch
+    px, truth = synth(ctx, torch, B, seed=0x20A1D3)
+    init = torch.empty(B, 5, dtype=torch.float64, device="cuda")
+    theta = torch.empty_like(init)
+    nll = torch.empty(B, dtype=torch.float64, device="cuda")
+    crlb = torch.empty(B, 3, dtype=torch.float64, device="cuda")
+    iters = torch.empty(B, dtype=torch.int32, device="cuda")
+    status = torch.empty(B, dtype=torch.int32, device="cuda")
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fit_ms = []
+
+    def step(timed):
+        if timed:
+            ev[0].record(stream)
+        ctx.fit_init_dev(px, NZ, ZMIN_T, ZSTEP_T, 0.0, init)
+        if timed:
+            ev[1].record(stream)
+        ctx.fit_dev(cfg, px, init, theta, nll, crlb, iters, status)
+        if timed:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    init_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()
+        init_ms.append(ev[0].elapsed_time(ev[1]))
+        fit_ms.append(ev[1].elapsed_time(ev[2]))
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = B / (ms_per_step / 1e3)
+
+    st = status.cpu().numpy()
+    nfield = (st >> 8).astype(np.int64)
+    it_np = iters.cpu().numpy()
+    th = theta.cpu().numpy()
+    tr = truth.cpu().numpy()
+    fit_kernel_ms = float(np.mean(fit_ms))
+    flops = float(nfield.sum()) * F64
+    achieved = flops / (fit_kernel_ms / 1e3)
+    conv = float(np.mean((st & Z.ST_CONVERGED) != 0))
+    err_nm = np.abs(th[:, 2:] - tr[:, 2:]) * 1000.0
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_fit_lm64_r21.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        if t.get("batch") == B:
+            traffic = t.get("bytes_per_launch")
+
+    out = {
+        "metric": "MLE localizations/s (21^2 ROI, 64^2 pupil), batched LM with analytic "
+                  "tangents vs plain GPU; %roofline",
+        "value": round(value, 1),
+        "unit": "localizations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 PSF synthesis, f64 likelihood/normal equations",
+        "data": "synthetic (GPU-generated Poisson ROIs from the library's PSF model, seed 0x20A1D3)",
+        "config": {
+            "workload": f"isolated-emitter localization: {B} ROIs of 21x21 px per step, 64x64 "
+                        "pupil, 28 Zernike modes (astigmatic), init + LM fit (max 30 it, "
+                        "tol 1e-4 um)",
+            "batch_rois": B, "roi": R, "pupil": 64, "zernike_modes": 28,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_fit_lm64<21,4>",
+            "achieved": round(achieved / 1e12, 4),
+            "peak": PEAK_FP32 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32, 5),
+            "traffic": traffic,
+            "kernel_ms": round(fit_kernel_ms, 4),
+            "algorithmic_flops_per_launch": flops,
+            "fields_per_roi": round(float(nfield.mean()), 3),
+        },
+        "fit": {
+            "init_kernel_ms": round(float(np.mean(init_ms)), 4),
+            "iters_mean": round(float(it_np.mean()), 3),
+            "converged_frac": round(conv, 5),
+            "median_abs_err_vs_truth_nm": [round(float(v), 3) for v in np.median(err_nm, axis=0)],
+        },
+    }
+
+    if not args.no_plain:
+        Bp = min(args.plain_batch, B)
+        cfgp = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=-1.3, zmax=1.3,
+                              photon_threshold=0.0)
+        thp = torch.empty(Bp, 5, dtype=torch.float64, device="cuda")
+        nlp = torch.empty(Bp, dtype=torch.float64, device="cuda")
+        crp = torch.empty(Bp, 3, dtype=torch.float64, device="cuda")
+        itp = torch.empty(Bp, dtype=torch.int32, device="cuda")
+        stp = torch.empty(Bp, dtype=torch.int32, device="cuda")
+        pxp, inp = px[:Bp].contiguous(), init[:Bp].contiguous()
+        ctx.fit_dev(cfgp, pxp, inp, thp, nlp, crp, itp, stp)   # warm-up
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        ctx.fit_dev(cfgp, pxp, inp, thp, nlp, crp, itp, stp)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        pms = ev[0].elapsed_time(ev[1])
+        nfp = (stp.cpu().numpy() >> 8).astype(np.int64)
+        pflops = float(nfp.sum()) * F64
+        plain_rate = Bp / (pms / 1e3)
+        out["plain_gpu"] = {
+            "form": "reference coordinate descent (k_fit_ref64<21,4>), FD steps of "
+                    "Localization.java, one PSF synthesis per likelihood",
+            "rois": Bp, "kernel_ms": round(pms, 3),
+            "value": round(plain_rate, 1),
+            "achieved_tflops": round(pflops / (pms / 1e3) / 1e12, 4),
+            "frac": round(pflops / (pms / 1e3) / PEAK_FP32, 5),
+            "fields_per_roi": round(float(nfp.mean()), 2),
+            "iters_mean": round(float(itp.float().mean().item()), 3),
+        }
+        # both forms fit the same ROIs from the same start: compare on common ROIs
+        lm_rate_kernel = B / ((np.mean(fit_ms) + np.mean(init_ms)) / 1e3)
+        out["batched_vs_plain"] = round(lm_rate_kernel / (Bp / ((pms + np.mean(init_ms) * Bp / B) / 1e3)), 3)
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        S = args.cpu_sample or threads * 6
+        S = min(S, B)
+        rois = px[:S].double().cpu().numpy()
+        ini = init[:S].cpu().numpy()
+        t0 = time.perf_counter()
+        O.localize_batch(O.optics(), R, coeffs, rois, ini, minZ=-1.3, maxZ=1.3, iterMax=20,
+                         thr=0.0, nthreads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(S / dt, 3),
+            "unit": "localizations/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {S} ROIs of the same batch, same initial values; FP64 C "
+                      f"restatement of cpu/Localization.localize + PSFPhaseMany (no PSF cache), "
+                      f"{dt:.1f} s on {threads} threads",
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

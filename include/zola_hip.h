@@ -1,0 +1,142 @@
+/*
+ * zola_hip.h -- C ABI of the MI355X (gfx950) localization and calibration
+ * hot path of ZOLA-3D.
+ *
+ * This is the drop-in boundary that replaces the reference's jcuda layer
+ * (src/main/java/org/pasteur/imagej/cuda/MyCudaStream.java:36-132,
+ * cuda/MyDefaultVecKernels.java:120-141,483-545, cuda/MyVecDouble.java and the
+ * direct JCufft/JCublas2/JCusparse calls inside process/gpu/PSFmany_float_.java,
+ * process/gpu/Modelmany_double_.java, process/gpu/Predetection_cross_.java and
+ * process/gpu/ZernikePhaseRetrieval_.java).  One call carries a whole batch,
+ * so the Java side no longer needs the per-evaluation thread barrier of
+ * Modelmany_double_.getLikelihood (Modelmany_double_.java:865-940).
+ *
+ * Conventions
+ *  - Plain C types only.  Every entry point returns ZOLA_OK (0) or a negative
+ *    ZOLA_E* code; zola_last_error() gives a message for the context.
+ *  - Entry points named *_dev take device pointers (memory already resident
+ *    in HBM) and run asynchronously on the context's stream; the others take
+ *    host pointers, copy, run and synchronise.
+ *  - Units: um for positions, photons for intensities.  Array layouts are
+ *    row-major [i][ii] with the first index the ImageJ x column, as in the
+ *    reference (process/cpu/LocalizationPipeline.java:592-596).
+ *  - A context is used by one host thread at a time.
+ */
+#ifndef ZOLA_HIP_H
+#define ZOLA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZOLA_OK 0
+#define ZOLA_EINVAL (-1)      /* bad argument                             */
+#define ZOLA_EUNSUPPORTED (-2)/* optics/ROI outside what the kernels cover */
+#define ZOLA_EHIP (-3)        /* HIP runtime error                        */
+#define ZOLA_ENOMEM (-4)
+
+/* Optics of the calibration file (process/cpu/DataPhase.java:587-760 keys
+ * sizeFFT, sizePix, NA, index, wavelength, sigmaGaussianKernel, withApoFactor;
+ * nwat as set by DataPhase.setNwat). */
+typedef struct zola_optics {
+    int    size_fft;     /* sizeFFT; rounded down to even, then up to a power of two */
+    double xystep;       /* camera pixel, um                                  */
+    double zstep;        /* calibration z step, um                            */
+    double wavelength;   /* um                                                */
+    double noil;         /* immersion index                                   */
+    double nwat;         /* sample index                                      */
+    double na;
+    double sigma;        /* Gaussian blur sigma, camera pixels                */
+    int    with_apo;     /* 1: apodisation factor (PSFPhaseMany.java:189-200) */
+    int    reserved;     /* must be 0                                         */
+} zola_optics;
+
+typedef struct zola_ctx zola_ctx;
+
+/* Creates a context for ROI side R (sizeoutput) with J Zernike coefficients
+ * in the reference's order (utils/Zernike.java:26-79).  Replaces
+ * DataPhase_(sizeFFT, json) + setSizeoutput + setMany
+ * (process/gpu/DataPhase_.java:275-297) and MyCudaStream.init. */
+int  zola_ctx_create(const zola_optics* optics, int R, const double* coeffs, int J,
+                     zola_ctx** out);
+int  zola_ctx_destroy(zola_ctx* ctx);
+const char* zola_last_error(const zola_ctx* ctx);
+/* Geometry: info[0]=N (pupil grid) info[1]=D (disk pixels) info[2]=s2 (blur
+ * grid) info[3]=R info[4]=J. */
+int  zola_ctx_info(const zola_ctx* ctx, int* info);
+/* Replaces ZernikePhase_.setA + computeCombination + PSFmany_float_.updatePhase
+ * (process/gpu/ZernikePhase_.java:602-611, PSFmany_float_.java:1338). */
+int  zola_set_coefficients(zola_ctx* ctx, const double* coeffs, int J);
+/* Replaces PSFmany_float_.updateSigmaGaussianKernel. */
+int  zola_set_sigma(zola_ctx* ctx, double sigma);
+/* Runs subsequent work on this HIP stream (hipStream_t passed as void*). */
+int  zola_set_stream(zola_ctx* ctx, void* hip_stream);
+
+/* Batched PSF synthesis.  params: B x 4 floats [x, y, z, zoil] (um; z is the
+ * emitter depth z_wat, zoil the focus term), out: B x R x R floats, each PSF
+ * normalised to sum 1 over the ROI.  Replaces PSFmany_float_.computePSF
+ * (process/gpu/PSFmany_float_.java:1626-1712) and the kernels it launches
+ * (vec_computePSF_phaseNManywithOil_f, vec_computePSF_signalN2Many_f,
+ * vec_mulMany_f, vec_computePSF_signalNsqrtMany_fcrop, vec_divScalarMany_f,
+ * three cuFFT executions and a cublasSgemv). */
+int  zola_psf_forward(zola_ctx* ctx, int B, const float* params, float* out);
+int  zola_psf_forward_dev(zola_ctx* ctx, int B, const float* params, float* out);
+
+/* Adjoint (vector-Jacobian product) of zola_psf_forward.  Given w = dL/dpsf
+ * (B x R x R), writes dparams (B x 4: dL/dx, dL/dy, dL/dz, dL/dzoil) and, when
+ * da is not NULL, accumulates sum over the batch of dL/da_j into da (J
+ * doubles, device memory for the _dev form; zeroed by the host form).  No
+ * reference counterpart: it replaces finite-difference PSF re-evaluation in
+ * process/gpu/ZernikePhaseRetrieval_.java:1108-1187 and Localization_.java. */
+int  zola_psf_backward(zola_ctx* ctx, int B, const float* params, const float* w,
+                       float* dparams, double* da);
+int  zola_psf_backward_dev(zola_ctx* ctx, int B, const float* params, const float* w,
+                           float* dparams, double* da);
+
+/* Fit modes */
+#define ZOLA_FIT_REFERENCE 0  /* reference coordinate descent (cpu/Localization.java:491-653) */
+#define ZOLA_FIT_LM        1  /* batched Poisson-MLE Levenberg-Marquardt with analytic tangents */
+
+/* Status bits (out_status) */
+#define ZOLA_ST_CONVERGED 1
+#define ZOLA_ST_MAXITER   2
+#define ZOLA_ST_NAN       4
+#define ZOLA_ST_LOWPHOTON 8   /* reference photon-count break (Localization.java:597-611) */
+/* Bits 8..31 of out_status hold the number of pupil fields (64x64 inverse
+ * transforms) the fit computed for that ROI: status >> 8. */
+#define ZOLA_ST_NFIELD_SHIFT 8
+
+typedef struct zola_fit_config {
+    int    mode;             /* ZOLA_FIT_*                                    */
+    int    max_iter;         /* iterMax (reference default 20)                */
+    double zmin, zmax;       /* clamp of z (reference: minZ-0.3, maxZ+0.3)    */
+    double zfocus;           /* z_oil of every ROI (param.Zfocus)             */
+    double photon_threshold; /* reference photonThreshold for the break       */
+    double tol;              /* LM: stop when |dx|,|dy|,|dz| < tol (um)        */
+} zola_fit_config;
+
+/* Fits B ROIs of side R.  pixels: B x R x R floats (photons).  init: B x 5
+ * doubles [A, B, x, y, z].  Outputs: theta B x 5 doubles, nll B doubles,
+ * crlb B x 3 doubles (x, y, z um), iters B ints, status B ints.  Replaces
+ * Localization_.localize / localizeWithoutFinishing + finish1
+ * (process/gpu/Localization_.java:633-1011) driven through
+ * Modelmany_double_ (process/gpu/Modelmany_double_.java:545-791). */
+int  zola_fit_batch(zola_ctx* ctx, const zola_fit_config* cfg, int B, const float* pixels,
+                    const double* init, double* theta, double* nll, double* crlb,
+                    int* iters, int* status);
+int  zola_fit_batch_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
+                        const float* pixels, const double* init, double* theta,
+                        double* nll, double* crlb, int* iters, int* status);
+
+/* Per-ROI initial values as the reference pipeline forms them for an isolated
+ * ROI: z from the best zero-shift normalised correlation against nZ templates
+ * PSF(0,0,z_r) (z_r = zmin_t + r*zstep_t), then getAandB with that template
+ * (process/cpu/LocalizationPipeline.java:1119-1162, threshold 0.1).  Writes
+ * init B x 5 doubles [A, B, 0, 0, z_r]. */
+int  zola_fit_init_dev(zola_ctx* ctx, int B, const float* pixels, int nZ, double zmin_t,
+                       double zstep_t, double zfocus, double* init);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

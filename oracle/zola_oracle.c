@@ -869,6 +869,11 @@ static int solve_spd(int n, double* A, double* b) {
 }
 
 /* theta order everywhere in this section: [A, B, x, y, z]. */
+
+/* Relative noise floor of the likelihood comparison (times A + npx*B);
+ * 1e-6 matches FP32 synthesis, 1e-12 is used for the FP64 optimum. */
+static double lm_noise_floor = 1e-12;
+void zo_set_lm_noise_floor(double v) { lm_noise_floor = v; }
 static double lm_eval(zo_model* m, const double* roi, const double* th, double zfocus,
                       double* psf, double* dpsf, double* g, double* H) {
     int n = m->R * m->R;
@@ -937,8 +942,16 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             tn[0] = fmax(tn[0], 1.0);
             tn[1] = fmax(tn[1], 0.01);
             tn[4] = fmin(fmax(tn[4], zmin), zmax);
+            /* predicted decrease of the quadratic model: below the likelihood's
+             * noise floor the step is taken without the comparison */
+            double pred = 0.0;
+            for (int p = 0; p < 5; p++) {
+                double hd = 0.0;
+                for (int q = 0; q < 5; q++) hd += H[p * 5 + q] * d[q];
+                pred -= g[p] * d[p] + 0.5 * d[p] * hd;
+            }
             double nn = lm_nll(m, roi, tn, zfocus, psf);
-            if (nn <= nll) {
+            if (pred < lm_noise_floor * (th[0] + (double)n * th[1]) || nn <= nll) {
                 double step = fmax(fabs(tn[2] - th[2]), fmax(fabs(tn[3] - th[3]), fabs(tn[4] - th[4])));
                 memcpy(th, tn, sizeof(th));
                 lambda = fmax(lambda * 0.1, 1e-9);

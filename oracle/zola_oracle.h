@@ -110,6 +110,8 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
               double zmax, int maxit, double tol, double* out_nll, int* out_iter,
               double* out_crlb);
 
+void zo_set_lm_noise_floor(double v);
+
 /* Fisher matrix sum(J J^T / mu) by analytic tangents, 5x5 in [A,B,x,y,z] order. */
 void zo_fisher_analytic(zo_model* m, const double* theta, double zfocus, double* fisher);
 

@@ -1,0 +1,86 @@
+"""GPU fits (zola_fit_batch) against the FP64 oracle.
+
+* LM form: converges to the Poisson MLE; compared with the oracle's FP64
+  Levenberg-Marquardt optimum (tolerances 0.5 nm x/y, 1 nm z, 1e-5 relative NLL).
+* Reference form: the coordinate descent of process/cpu/Localization.java,
+  compared with its FP64 restatement (zo_localize) from the same start.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import simdata
+import zola_hip as Z
+
+pytestmark = pytest.mark.gpu
+
+R = 21
+B = 24
+ZMIN_T, ZSTEP_T, NZ = -1.0, 0.1, 21
+
+
+@pytest.fixture(scope="module")
+def setup():
+    a = Z.astigmatic_coeffs(28)
+    m = O.Model(O.optics(), R, a)
+    rois, truth = simdata.isolated_rois(m, B, seed=11)
+    tmpl = simdata.templates(m, ZMIN_T, ZSTEP_T, NZ)
+    init = simdata.init_restated(rois, tmpl, ZMIN_T, ZSTEP_T)
+    ctx = Z.Context(Z.optics(), R, a)
+    return m, ctx, rois, truth, init
+
+
+def test_fit_init_matches_restatement(setup, torch_cuda):
+    torch = torch_cuda
+    m, ctx, rois, truth, init = setup
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    px = torch.from_numpy(rois).cuda()
+    out = torch.zeros(B, 5, dtype=torch.float64, device="cuda")
+    ctx.fit_init_dev(px, NZ, ZMIN_T, ZSTEP_T, 0.0, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    print(np.c_[got[:, 4], init[:, 4]])
+    assert np.abs(got[:, 4] - init[:, 4]).max() < 2e-3
+    np.testing.assert_allclose(got[:, 0], init[:, 0], rtol=1e-3)
+    np.testing.assert_allclose(got[:, 1], init[:, 1], rtol=1e-3)
+
+
+def test_lm_fit_matches_fp64_optimum(setup):
+    m, ctx, rois, truth, init = setup
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-5)
+    res = ctx.fit(cfg, rois, init)
+    ref = [m.lm_fit(rois[b].astype(np.float64), init[b], zmin=-1.3, zmax=1.3) for b in range(B)]
+    th_ref = np.array([r["theta"] for r in ref])
+    nll_ref = np.array([r["nll"] for r in ref])
+    d = (res["theta"][:, 2:] - th_ref[:, 2:]) * 1000.0
+    print("LM |dx|,|dy|,|dz| max nm:", np.abs(d).max(axis=0), "iters", res["iters"],
+          "fields", res["status"] >> 8)
+    print("NLL rel", np.abs(res["nll"] - nll_ref).max() / np.abs(nll_ref).min())
+    assert np.all(res["status"] & Z.ST_CONVERGED)
+    assert np.abs(d[:, :2]).max() <= 0.5
+    assert np.abs(d[:, 2]).max() <= 1.0
+    assert np.all(np.abs(res["nll"] - nll_ref) <= 1e-5 * np.abs(nll_ref))
+    crlb_ref = np.array([r["crlb"] for r in ref])
+    np.testing.assert_allclose(res["crlb"], crlb_ref, rtol=1e-3)
+
+
+def test_reference_fit_matches_restatement(setup):
+    m, ctx, rois, truth, init = setup
+    cfg = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=-1.3, zmax=1.3,
+                         photon_threshold=0.0)
+    res = ctx.fit(cfg, rois, init)
+    O.lib().zo_set_psf_cache(1)
+    try:
+        ref = [m.localize(rois[b].astype(np.float64), init[b], minZ=-1.3, maxZ=1.3, iterMax=20)
+               for b in range(B)]
+    finally:
+        O.lib().zo_set_psf_cache(0)
+    th_ref = np.array([r["theta"] for r in ref])
+    it_ref = np.array([r["iters"] for r in ref])
+    d = (res["theta"][:, 2:] - th_ref[:, 2:]) * 1000.0
+    both = (it_ref < 20) & (res["iters"] < 20)
+    print("ref-form |d| nm max (converged in both):", np.abs(d[both]).max(axis=0),
+          "iters gpu", res["iters"], "cpu", it_ref)
+    assert both.sum() >= B // 2
+    assert np.abs(d[both][:, :2]).max() <= 0.5
+    assert np.abs(d[both][:, 2]).max() <= 1.0

@@ -1,0 +1,137 @@
+// zola_fft.hpp -- register-resident radix-2 FFT building blocks for gfx950.
+//
+// One lane owns a whole short transform in VGPRs (float re[L], im[L]); the
+// transform is fully unrolled at compile time with constexpr twiddles, so
+// trivial twiddles (1, +-i, (+-1 +- i)/sqrt2) cost no multiplies.  Cross-lane
+// data movement (the 2-D transpose) goes through LDS in the callers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <utility>
+
+namespace zola {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// constexpr sin/cos (Taylor series after reduction to [-pi, pi]) for the
+// compile-time twiddle tables.
+constexpr double cx_reduce(double x) {
+    while (x > kPi) x -= 2 * kPi;
+    while (x < -kPi) x += 2 * kPi;
+    return x;
+}
+constexpr double cx_sin(double x) {
+    x = cx_reduce(x);
+    double term = x, sum = x;
+    for (int n = 1; n < 30; n++) {
+        term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double cx_cos(double x) {
+    x = cx_reduce(x);
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 30; n++) {
+        term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+constexpr int bitrev(int v, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) r |= ((v >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+template <int LEN, int K, int SIGN>
+struct Twiddle {
+    static constexpr double ang = SIGN * 2.0 * kPi * K / LEN;
+    static constexpr float c = (float)cx_cos(ang);
+    static constexpr float s = (float)cx_sin(ang);
+};
+
+// (dr + i di) *= exp(SIGN * 2 pi i K / LEN)
+template <int LEN, int K, int SIGN>
+__device__ __forceinline__ void twiddle_mul(float& dr, float& di) {
+    constexpr float r2 = 0.70710678118654752440f;
+    if constexpr (K == 0) {
+    } else if constexpr (4 * K == LEN) {
+        float t = dr;
+        dr = -SIGN * di;
+        di = SIGN * t;
+    } else if constexpr (8 * K == LEN) {
+        float t = dr;
+        dr = (dr - SIGN * di) * r2;
+        di = (di + SIGN * t) * r2;
+    } else if constexpr (8 * K == 3 * LEN) {
+        float t = dr;
+        dr = (-dr - SIGN * di) * r2;
+        di = (-di + SIGN * t) * r2;
+    } else {
+        constexpr float c = Twiddle<LEN, K, SIGN>::c;
+        constexpr float s = Twiddle<LEN, K, SIGN>::s;
+        float t = dr;
+        dr = dr * c - di * s;
+        di = t * s + di * c;
+    }
+}
+
+// In-place unnormalised DFT X[m] = sum_j x[j] exp(SIGN 2 pi i j m / L),
+// decimation in frequency; on return X[m] sits at index bitrev(m).
+template <int L, int SIGN>
+__device__ __forceinline__ void fft_dif(float (&re)[L], float (&im)[L]) {
+    static_for<0, ilog2(L)>([&](auto st) {
+        constexpr int len = L >> decltype(st)::value;
+        constexpr int half = len / 2;
+        static_for<0, L / len>([&](auto blk) {
+            static_for<0, half>([&](auto kk) {
+                constexpr int k = decltype(kk)::value;
+                constexpr int a = decltype(blk)::value * len + k;
+                constexpr int b = a + half;
+                float ar = re[a], ai = im[a], br = re[b], bi = im[b];
+                re[a] = ar + br;
+                im[a] = ai + bi;
+                float dr = ar - br, di = ai - bi;
+                twiddle_mul<len, k, SIGN>(dr, di);
+                re[b] = dr;
+                im[b] = di;
+            });
+        });
+    });
+}
+
+// Natural-order index of output m after fft_dif.
+template <int L>
+__device__ __forceinline__ constexpr int fidx(int m) { return bitrev(m, ilog2(L)); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Orders LDS traffic of one wave around a cross-lane exchange: all lanes'
+// LDS writes before the fence are complete before any read after it.  Only
+// the issuing wave is involved (waves of a block run independent ROIs).
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+}  // namespace zola

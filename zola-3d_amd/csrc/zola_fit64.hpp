@@ -1,0 +1,714 @@
+// zola_fit64.hpp -- per-ROI fitting kernels on the 64x64 pupil grid (gfx950).
+//
+// One wave owns one ROI for the whole fit (persistent over iterations); the
+// waves of a block share the pupil tables in LDS and never synchronise with
+// each other after the table load, so ROIs with different iteration counts
+// retire independently (per-ROI convergence masks are the waves themselves).
+//
+// Two fit forms:
+//  * k_fit_lm64  -- Poisson-MLE Levenberg-Marquardt.  The Jacobian of the
+//    model comes from analytic tangent fields dE/dx, dE/dy, dE/dz (one extra
+//    pupil transform each, sharing the retained E), no finite differences.
+//    FP32 PSF synthesis; FP64 likelihood, gradient, normal matrix and solve.
+//  * k_fit_ref64 -- the reference's coordinate descent with central finite
+//    differences (process/cpu/Localization.java:29-31,115,317-653,672-1143),
+//    every likelihood a full PSF synthesis (cached when only A/B change).
+#pragma once
+#include "zola_psf64.hpp"
+
+namespace zola {
+
+struct FitArgs {
+    int B;
+    int max_iter;
+    float xystep;
+    double zmin, zmax, zfocus, thr, tol;
+    const float* pixels;      // B*R*R
+    const double* init;       // B*5 [A, B, x, y, z]
+    double* theta;            // B*5
+    double* nll;              // B
+    double* crlb;             // B*3
+    int* iters;               // B
+    int* status;              // B
+};
+
+enum : int { kStConverged = 1, kStMaxIter = 2, kStNaN = 4, kStLowPhoton = 8 };
+
+template <int R>
+struct Px {
+    static constexpr int G = RoiLayout<R>::G;
+    static constexpr int RPL = RoiLayout<R>::RPL;
+    __device__ static bool valid(int lane, int t) {
+        const int g = lane / R;
+        return g < G && g * RPL + t < R;
+    }
+};
+
+// One pupil field at (x, y, z, zoil): field 0 = E (kept in Er/Ei when KEEP),
+// 1..3 = tangent fields d/dx, d/dy, d/dz whose image is 2 Re(conj(E) dE).
+// Returns the raw blurred values v (signed) of this lane's ROI pixels.  Every
+// caller runs it from a single site inside its iteration loop.
+template <int R, bool KEEP>
+__device__ __forceinline__ void eval_field64(const Model64& M, const BlockTables64& T,
+                                             WaveScratch64& S, int lane, int field, float x,
+                                             float y, float z, float zoil, float (&Er)[2][32],
+                                             float (&Ei)[2][32],
+                                             float (&v)[RoiLayout<R>::RPL]) {
+    stage1_64(T, S, lane, field, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
+    wave_lds_fence();
+    float acc[R];
+    if (!KEEP || field == 0) stage2_E64<R, KEEP>(M, S, lane, acc, Er, Ei);
+    else stage2_T64<R>(M, S, lane, acc, Er, Ei);
+    column_blur64<R>(T, S, lane, acc, v);
+}
+
+// |v| normalised to sum 1 over the ROI; returns the sum.
+template <int R>
+__device__ __forceinline__ float normalise(int lane, const float (&v)[RoiLayout<R>::RPL],
+                                           float (&psf)[RoiLayout<R>::RPL]) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        psf[t] = fabsf(v[t]);
+        if (Px<R>::valid(lane, t)) s += psf[t];
+    }
+    s = wave_sum(s);
+    const float inv = 1.f / s;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) psf[t] *= inv;
+    return s;
+}
+
+// Poisson NLL without ln n! (Localization.computeLikelihood, 1125-1143), FP64.
+template <int R>
+__device__ __forceinline__ double nll64(int lane, const float (&psf)[RoiLayout<R>::RPL],
+                                        const float (&n)[RoiLayout<R>::RPL], double A, double Bg) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        if (Px<R>::valid(lane, t)) {
+            const double mu = (double)psf[t] * A + Bg;
+            acc += mu - (double)n[t] * log(mu);
+        }
+    }
+    return wave_sum(acc);
+}
+
+// Gaussian elimination with partial pivoting on a 5x5 system (A destroyed).
+__device__ __forceinline__ bool solve5(double (&A)[25], double (&b)[5]) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+        int p = c;
+#pragma unroll
+        for (int r = c + 1; r < 5; r++)
+            if (fabs(A[r * 5 + c]) > fabs(A[p * 5 + c])) p = r;
+        if (A[p * 5 + c] == 0.0) return false;
+        if (p != c) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                // swap rows c and p (p is lane-uniform; select keeps it branch free)
+                double tc = A[c * 5 + k];
+                double tp = 0.0;
+#pragma unroll
+                for (int r = c + 1; r < 5; r++)
+                    if (r == p) tp = A[r * 5 + k];
+                A[c * 5 + k] = tp;
+#pragma unroll
+                for (int r = c + 1; r < 5; r++)
+                    if (r == p) A[r * 5 + k] = tc;
+            }
+            double tc = b[c], tp = 0.0;
+#pragma unroll
+            for (int r = c + 1; r < 5; r++)
+                if (r == p) tp = b[r];
+            b[c] = tp;
+#pragma unroll
+            for (int r = c + 1; r < 5; r++)
+                if (r == p) b[r] = tc;
+        }
+#pragma unroll
+        for (int r = c + 1; r < 5; r++) {
+            const double f = A[r * 5 + c] / A[c * 5 + c];
+#pragma unroll
+            for (int k = c; k < 5; k++) A[r * 5 + k] -= f * A[c * 5 + k];
+            b[r] -= f * b[c];
+        }
+    }
+#pragma unroll
+    for (int c = 4; c >= 0; c--) {
+        double s = b[c];
+#pragma unroll
+        for (int k = c + 1; k < 5; k++) s -= A[c * 5 + k] * b[k];
+        b[c] = s / A[c * 5 + c];
+    }
+    return true;
+}
+
+struct LmState {
+    double H[15];   // normal matrix, packed upper triangle
+    double gr[5];   // gradient
+};
+
+__device__ __forceinline__ double hget(const double (&H)[15], int p, int q) {
+    // packed upper triangle, row-major
+    if (p > q) { int t = p; p = q; q = t; }
+    const int off = p * 5 - (p * (p - 1)) / 2;
+    return H[off + (q - p)];
+}
+
+// ---------------------------------------------------------------------------
+// Levenberg-Marquardt (theta order [A, B, x, y, z])
+// ---------------------------------------------------------------------------
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
+    __shared__ BlockTables64 T;
+    __shared__ WaveScratch64 Sall[WPB];
+    __shared__ LmState Lst[WPB];
+    load_tables64(T, M);
+    __syncthreads();
+    constexpr int RPL = RoiLayout<R>::RPL;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * WPB + w;
+    if (b >= A.B) return;
+    WaveScratch64& S = Sall[w];
+    const int j = lane % R, g = lane / R;
+    float n[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; t++)
+        n[t] = Px<R>::valid(lane, t) ? A.pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+    const float zoil = (float)A.zfocus;
+    double th[5], tn[5];
+#pragma unroll
+    for (int p = 0; p < 5; p++) th[p] = A.init[(size_t)b * 5 + p];
+    th[0] = fmax(th[0], 1.0);
+    th[1] = fmax(th[1], 0.01);
+#pragma unroll
+    for (int p = 0; p < 5; p++) tn[p] = th[p];
+
+    float Er[2][32], Ei[2][32];
+    float psf[RPL], sgn[RPL], dp[3][RPL];
+    float Ssum = 1.f;
+    // wave-uniform FP64 state lives in LDS between uses (keeps VGPRs for E)
+    LmState& L = Lst[w];
+    double nll = 0.0, lambda = 1e-3;
+    const double dxmax = (double)A.xystep;
+    int it = 0, tries = 0, conv = 0, nan = 0, nfield = 0;
+    bool first = true;
+    while (true) {
+        bool eval_now = true;
+        bool noise_floor = false;
+        if (!first) {
+            double Am[25], d[5];
+#pragma unroll
+            for (int p = 0; p < 5; p++) {
+#pragma unroll
+                for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
+                Am[p * 5 + p] *= (1.0 + lambda);
+                d[p] = -L.gr[p];
+            }
+            if (!solve5(Am, d)) {
+                lambda *= 10;
+                eval_now = false;
+            } else {
+                // Predicted decrease of the quadratic model; below the FP32
+                // synthesis noise of the likelihood the step is taken without
+                // the likelihood comparison (the gradient, not the noisy NLL,
+                // then drives convergence).
+                double pred = 0.0;
+#pragma unroll
+                for (int p = 0; p < 5; p++) {
+                    double hd = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 5; q++) hd += hget(L.H, p, q) * d[q];
+                    pred -= L.gr[p] * d[p] + 0.5 * d[p] * hd;
+                }
+                noise_floor = pred < 1e-6 * (th[0] + (double)(R * R) * th[1]);
+#pragma unroll
+                for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
+#pragma unroll
+                for (int p = 2; p < 4; p++) {
+                    if (d[p] > dxmax) tn[p] = th[p] + dxmax;
+                    if (d[p] < -dxmax) tn[p] = th[p] - dxmax;
+                }
+                if (d[4] > 0.2) tn[4] = th[4] + 0.2;
+                if (d[4] < -0.2) tn[4] = th[4] - 0.2;
+                tn[0] = fmax(tn[0], 1.0);
+                tn[1] = fmax(tn[1], 0.01);
+                tn[4] = fmin(fmax(tn[4], A.zmin), A.zmax);
+            }
+        }
+        bool accepted = false;
+        if (eval_now) {
+            // field 0 at the trial point, then (if accepted) the 3 tangent fields
+            for (int f = 0; f < 4; f++) {
+                float v[RPL];
+                eval_field64<R, true>(M, T, S, lane, f, (float)tn[2], (float)tn[3], (float)tn[4],
+                                      zoil, Er, Ei, v);
+                nfield++;
+                if (f == 0) {
+                    float psfn[RPL];
+                    const float Sn = normalise<R>(lane, v, psfn);
+                    const double nn = nll64<R>(lane, psfn, n, tn[0], tn[1]);
+                    if (nn != nn) nan = 1;
+                    accepted = first || noise_floor || nn <= nll;
+                    if (!accepted) {
+                        lambda *= 10;
+                        break;
+                    }
+                    if (!first) {
+                        const double step = fmax(fabs(tn[2] - th[2]),
+                                                 fmax(fabs(tn[3] - th[3]), fabs(tn[4] - th[4])));
+                        lambda = fmax(lambda * 0.1, 1e-9);
+                        if (step < A.tol) conv = 1;
+                    }
+#pragma unroll
+                    for (int p = 0; p < 5; p++) th[p] = tn[p];
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) {
+                        psf[t] = psfn[t];
+                        sgn[t] = v[t] < 0.f ? -1.f : 1.f;
+                    }
+                    Ssum = Sn;
+                    nll = nn;
+                } else {
+                    // d psf = (sgn dv - psf * sum(sgn dv)) / S
+                    float sv = 0.f;
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) {
+                        v[t] *= sgn[t];
+                        if (Px<R>::valid(lane, t)) sv += v[t];
+                    }
+                    sv = wave_sum(sv);
+                    const float inv = 1.f / Ssum;
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) {
+                        const float d = (v[t] - psf[t] * sv) * inv;
+                        dp[0][t] = f == 1 ? d : dp[0][t];
+                        dp[1][t] = f == 2 ? d : dp[1][t];
+                        dp[2][t] = f == 3 ? d : dp[2][t];
+                    }
+                }
+            }
+            if (accepted) {
+                // gradient and normal matrix sum J J^T / mu (Fisher form)
+                double acc[20];
+#pragma unroll
+                for (int k = 0; k < 20; k++) acc[k] = 0.0;
+                const double Av = th[0], Bg = th[1];
+#pragma unroll
+                for (int t = 0; t < RPL; t++) {
+                    if (!Px<R>::valid(lane, t)) continue;
+                    const double mu = (double)psf[t] * Av + Bg;
+                    const double J[5] = {(double)psf[t], 1.0, Av * dp[0][t], Av * dp[1][t],
+                                         Av * dp[2][t]};
+                    const double wg = 1.0 - (double)n[t] / mu, wh = 1.0 / mu;
+                    int k = 5;
+#pragma unroll
+                    for (int p = 0; p < 5; p++) {
+                        acc[p] += wg * J[p];
+#pragma unroll
+                        for (int q = p; q < 5; q++) acc[k++] += wh * J[p] * J[q];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 20; k++) acc[k] = wave_sum(acc[k]);
+#pragma unroll
+                for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
+#pragma unroll
+                for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
+                wave_lds_fence();
+            }
+        }
+        if (first) {
+            first = false;
+            if (A.max_iter <= 0) break;
+            continue;
+        }
+        tries++;
+        if (accepted || tries >= 12) {
+            if (!accepted) conv = 1;
+            it++;
+            tries = 0;
+            if (conv || it >= A.max_iter) break;
+        }
+    }
+    // CRLB of x, y, z from the Fisher matrix at the optimum
+    double cr[3];
+#pragma unroll
+    for (int c = 2; c < 5; c++) {
+        double Am[25], e[5];
+#pragma unroll
+        for (int p = 0; p < 5; p++) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
+            e[p] = (p == c) ? 1.0 : 0.0;
+        }
+        cr[c - 2] = solve5(Am, e) ? sqrt(e[c]) : (double)NAN;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int p = 0; p < 5; p++) A.theta[(size_t)b * 5 + p] = th[p];
+        A.nll[b] = nll;
+#pragma unroll
+        for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = cr[c];
+        A.iters[b] = it;
+        A.status[b] = (conv ? kStConverged : kStMaxIter) | (nan ? kStNaN : 0) | (nfield << 8);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reference coordinate descent (cpu/Localization.localize), as a state
+// machine around one PSF-synthesis site.
+// ---------------------------------------------------------------------------
+
+// Laplace-expansion determinant and cofactor inverse (utils/Matrixe.java:140-246).
+template <int NN>
+__device__ __forceinline__ double det_lap(const double* a) {
+    if constexpr (NN == 1) {
+        return a[0];
+    } else if constexpr (NN == 2) {
+        return a[0] * a[3] - a[1] * a[2];
+    } else {
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+            double sub[(NN - 1) * (NN - 1)];
+            int r = 0;
+#pragma unroll
+            for (int rr = 1; rr < NN; rr++, r++) {
+                int c = 0;
+#pragma unroll
+                for (int cc = 0; cc < NN; cc++) {
+                    if (cc == i) continue;
+                    sub[r * (NN - 1) + c++] = a[rr * NN + cc];
+                }
+            }
+            sum += ((i & 1) ? -1.0 : 1.0) * a[i] * det_lap<NN - 1>(sub);
+        }
+        return sum;
+    }
+}
+
+__device__ __forceinline__ void inverse_cofactor5(const double (&a)[25], double (&inv)[25]) {
+    const double det = 1.0 / det_lap<5>(a);
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            double sub[16];
+            int r = 0;
+#pragma unroll
+            for (int rr = 0; rr < 5; rr++) {
+                if (rr == i) continue;
+                int c = 0;
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++) {
+                    if (cc == j) continue;
+                    sub[r * 4 + c++] = a[rr * 5 + cc];
+                }
+                r++;
+            }
+            inv[j * 5 + i] = (((i + j) & 1) ? -1.0 : 1.0) * det_lap<4>(sub) * det;
+        }
+}
+
+enum : int { kRsInit, kRsAmpA, kRsAmpB, kRsPos, kRsCheck, kRsFisher, kRsDone };
+
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
+    __shared__ BlockTables64 T;
+    __shared__ WaveScratch64 Sall[WPB];
+    load_tables64(T, M);
+    __syncthreads();
+    constexpr int RPL = RoiLayout<R>::RPL;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * WPB + w;
+    if (b >= A.B) return;
+    WaveScratch64& S = Sall[w];
+    const int j = lane % R, g = lane / R;
+    float n[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; t++)
+        n[t] = Px<R>::valid(lane, t) ? A.pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+    const float zoil = (float)A.zfocus;
+    double tA = A.init[(size_t)b * 5 + 0], tB = A.init[(size_t)b * 5 + 1];
+    double th[3] = {A.init[(size_t)b * 5 + 2], A.init[(size_t)b * 5 + 3], A.init[(size_t)b * 5 + 4]};
+    const double h = .0005, eps = .0001, maxShift = (double)A.xystep / 2.;  // Localization.java:29-31,115
+
+    // psf at the current position (cur) and the last synthesised one (tmp)
+    float cur[RPL], tmp[RPL], hi[RPL], dd[3][RPL];
+    double pc[3] = {0, 0, 0};
+    bool hc = false;
+    double req[3] = {0, 0, 0};
+    bool want = false;
+
+    int st = kRsInit, which = 0, k = 0, it = 0, nbIter = A.max_iter, nfield = 0;
+    int brk = 0, ok = 1, nan = 0, conv = 0, initial = 1;
+    double likelihood = 0, save = 0, l1 = 0, l3 = 0, grad = 0, prevLik = 0;
+    double oldX = 1.7976931348623157e308, oldY = oldX, oldZ = oldX;
+    auto at_theta = [&]() { return hc && pc[0] == th[0] && pc[1] == th[1] && pc[2] == th[2]; };
+    auto L = [&](const float (&ps)[RPL], double a, double bb) { return nll64<R>(lane, ps, n, a, bb); };
+    // min_amp (929-977 for A with h*1000, 997-1058 for B with h*10), on the cached psf
+    auto min_amp = [&](bool isB) {
+        const double hh = isB ? h * 10 : h * 1000;
+        double& t = isB ? tB : tA;
+        double sv = t;
+        double a1, a2, a3;
+        if (!isB) { a1 = L(cur, tA - hh, tB); a2 = L(cur, tA, tB); a3 = L(cur, tA + hh, tB); }
+        else { a1 = L(cur, tA, tB - hh); a2 = L(cur, tA, tB); a3 = L(cur, tA, tB + hh); t = sv; }
+        double gd;
+        if (fabs((a3 + a1 - 2 * a2) / (hh * hh)) == 0) gd = (a3 - a1) * (2 * hh);
+        else gd = ((a3 - a1) / (2 * hh)) / fabs((a3 + a1 - 2 * a2) / (hh * hh));
+        for (double gamma = 1; gamma > .02; gamma /= 10) {
+            t -= gamma * gd;
+            const double lk = L(cur, tA, tB);
+            if (lk < a2) { a2 = lk; sv = t; }
+            else t = sv;
+        }
+        return L(cur, tA, tB);
+    };
+
+    while (st != kRsDone) {
+        if (want) {
+            float Er[2][32], Ei[2][32], v[RPL];
+            eval_field64<R, false>(M, T, S, lane, 0, (float)req[0], (float)req[1], (float)req[2],
+                                   zoil, Er, Ei, v);
+            normalise<R>(lane, v, tmp);
+            want = false;
+            nfield++;
+        }
+        if (st == kRsInit || st == kRsAmpA || st == kRsAmpB) {
+            if (!at_theta()) {
+                if (k == 1) {  // tmp holds psf(theta)
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) pc[c] = th[c];
+                    hc = true;
+                    k = 0;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) req[c] = th[c];
+                    want = true;
+                    k = 1;
+                    continue;
+                }
+            }
+            if (st == kRsInit) {
+                tA = fmax(1.0, tA);
+                tB = fmax(.01, tB);
+                likelihood = L(cur, tA, tB);
+                st = kRsAmpA;
+            } else if (st == kRsAmpA) {
+                likelihood = min_amp(false);
+                tA = fmax(1.0, tA);
+                tB = fmax(.01, tB);
+                st = kRsAmpB;
+            } else {
+                likelihood = min_amp(true);
+                tA = fmax(1.0, tA);
+                tB = fmax(.01, tB);
+                if (initial) {
+                    initial = 0;
+                    it = 0;
+                    if (A.max_iter <= 0) { st = kRsFisher; k = 0; continue; }
+                    st = kRsPos; which = 0; k = 0;
+                } else {
+                    st = kRsCheck;
+                }
+            }
+            continue;
+        }
+        if (st == kRsPos) {
+            // minimizeZ / minimizeX / minimizeY (672-906); which: 0 = z, 1 = x, 2 = y
+            const int ci = which == 0 ? 2 : which - 1;
+            if (k == 0) {
+                save = th[ci];
+#pragma unroll
+                for (int c = 0; c < 3; c++) req[c] = th[c];
+                req[ci] = th[ci] - h;
+                want = true;
+                k = 1;
+            } else if (k == 1) {
+                l1 = L(tmp, tA, tB);
+#pragma unroll
+                for (int c = 0; c < 3; c++) req[c] = th[c];
+                req[ci] = th[ci] + h;
+                want = true;
+                k = 2;
+            } else if (k == 2) {
+                l3 = L(tmp, tA, tB);
+                const double l2 = likelihood;
+                if (fabs((l3 + l1 - 2 * l2) / (h * h)) == 0) grad = (l3 - l1) * (2 * h);
+                else grad = ((l3 - l1) / (2 * h)) / fabs((l3 + l1 - 2 * l2) / (h * h));
+                if (grad > maxShift) grad = maxShift;
+                else if (grad < -maxShift) grad = -maxShift;
+                prevLik = likelihood;
+                th[ci] -= 1.0 * grad;
+#pragma unroll
+                for (int c = 0; c < 3; c++) req[c] = th[c];
+                want = true;
+                k = 3;
+            } else {
+                const double lik = L(tmp, tA, tB);
+                bool done = false;
+                if (lik < likelihood) {
+                    prevLik = lik;
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) pc[c] = th[c];
+                    hc = true;
+                    done = true;
+                } else {
+                    th[ci] = save;
+                    if (k == 3) {
+                        th[ci] -= (1.0 / 10) * grad;
+#pragma unroll
+                        for (int c = 0; c < 3; c++) req[c] = th[c];
+                        want = true;
+                        k = 4;
+                    } else {
+                        done = true;
+                    }
+                }
+                if (done) {
+                    likelihood = prevLik;
+                    if (which == 0) {
+                        if (th[2] < A.zmin) th[2] = A.zmin;
+                        if (th[2] > A.zmax) th[2] = A.zmax;
+                    }
+                    which++;
+                    k = 0;
+                    if (which == 3) st = kRsAmpA;
+                }
+            }
+            continue;
+        }
+        if (st == kRsCheck) {
+            const double distance = sqrt((th[0] - oldX) * (th[0] - oldX) + (th[1] - oldY) * (th[1] - oldY) +
+                                         (th[2] - oldZ) * (th[2] - oldZ));
+            int breaking = 0;
+            if (it > 4) breaking = !(tA > A.thr);
+            if (breaking) brk = 1;
+            if (distance < eps || breaking) {
+                nbIter = it + 1;
+                conv = !breaking;
+                st = kRsFisher;
+                k = 0;
+                continue;
+            }
+            if (likelihood != likelihood) {
+                ok = 0;
+                nan = 1;
+                st = kRsDone;
+                continue;
+            }
+            oldX = th[0]; oldY = th[1]; oldZ = th[2];
+            it++;
+            if (it >= A.max_iter) {
+                nbIter = A.max_iter;
+                st = kRsFisher;
+                k = 0;
+            } else {
+                st = kRsPos; which = 0; k = 0;
+            }
+            continue;
+        }
+        if (st == kRsFisher) {
+            // computeFisher (317-403) with the reference's steps: probes k = 0..5
+            // at x+h, x-h, y+h, y-h, z+h, z-h; then the centre (cached psf).
+            if (k >= 1 && k <= 6) {
+                const int q = (k - 1) / 2;
+                if ((k - 1) % 2 == 0) {
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) hi[t] = tmp[t];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) {
+                        const double f2 = (double)hi[t] * tA + tB, f0 = (double)tmp[t] * tA + tB;
+                        const float d = (float)((f2 - f0) / (2 * fabs(h)));
+                        dd[0][t] = q == 0 ? d : dd[0][t];
+                        dd[1][t] = q == 1 ? d : dd[1][t];
+                        dd[2][t] = q == 2 ? d : dd[2][t];
+                    }
+                }
+            }
+            if (k < 6) {
+                const int q = k / 2;
+                const double sgnh = (k % 2 == 0) ? h : -h;
+                req[0] = th[0] + (q == 0 ? sgnh : 0.0);
+                req[1] = th[1] + (q == 1 ? sgnh : 0.0);
+                req[2] = th[2] + (q == 2 ? sgnh : 0.0);
+                want = true;
+                k++;
+                continue;
+            }
+            if (!at_theta()) {
+                if (k == 7) {
+#pragma unroll
+                    for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) pc[c] = th[c];
+                    hc = true;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 3; c++) req[c] = th[c];
+                    want = true;
+                    k = 7;
+                    continue;
+                }
+            }
+            double acc[15];
+#pragma unroll
+            for (int kk = 0; kk < 15; kk++) acc[kk] = 0.0;
+#pragma unroll
+            for (int t = 0; t < RPL; t++) {
+                if (!Px<R>::valid(lane, t)) continue;
+                const double f1 = (double)cur[t] * tA + tB;
+                // x, y, z, then A (the reference differentiates the model f1) and B
+                const double J[5] = {(double)dd[0][t], (double)dd[1][t], (double)dd[2][t], f1, 1.0};
+                int kk = 0;
+#pragma unroll
+                for (int p = 0; p < 5; p++)
+#pragma unroll
+                    for (int q = p; q < 5; q++) acc[kk++] += (1 / f1) * J[p] * J[q];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 15; kk++) acc[kk] = wave_sum(acc[kk]);
+            double F[25], inv[25];
+            int kk = 0;
+#pragma unroll
+            for (int p = 0; p < 5; p++)
+#pragma unroll
+                for (int q = p; q < 5; q++) { F[p * 5 + q] = acc[kk]; F[q * 5 + p] = acc[kk]; kk++; }
+            likelihood = L(cur, tA, tB);
+            inverse_cofactor5(F, inv);
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = sqrt(inv[c * 5 + c]);
+            }
+            st = kRsDone;
+            continue;
+        }
+    }
+    if (lane == 0) {
+        if (!ok) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = (double)NAN;
+        }
+        A.theta[(size_t)b * 5 + 0] = tA;
+        A.theta[(size_t)b * 5 + 1] = tB;
+        A.theta[(size_t)b * 5 + 2] = th[0];
+        A.theta[(size_t)b * 5 + 3] = th[1];
+        A.theta[(size_t)b * 5 + 4] = th[2];
+        A.nll[b] = likelihood;
+        A.iters[b] = nbIter;
+        A.status[b] = (conv ? kStConverged : 0) | (nbIter == A.max_iter && !conv ? kStMaxIter : 0) |
+                      (nan ? kStNaN : 0) | (brk ? kStLowPhoton : 0) | (nfield << 8);
+    }
+}
+
+}  // namespace zola

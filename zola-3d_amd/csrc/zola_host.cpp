@@ -1,0 +1,410 @@
+// zola_host.cpp -- context, pupil tables and the C ABI (include/zola_hip.h).
+//
+// The optics tables are built on the host in FP64 exactly as the reference
+// builds them (process/PhaseParameters.java:265-366 for the grid and disk,
+// utils/Zernike.java:26-79,545-605 for the basis, process/cpu/
+// PSFPhaseMany.java:145-210 for kx, ky, kz, kz_oil and the amplitude,
+// PSFPhaseMany.java:387-404 for the Gaussian), then reduced to the compact
+// device form used by the kernels: a 32x32 window of the fftshifted pupil
+// and the separable spatial blur kernel h1.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/zola_hip.h"
+#include "zola_kernels.h"
+#include "zola_fit64.hpp"
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+int ref_pow2(int x) { return (int)std::pow(2.0, (int)std::ceil(std::log((double)x) / std::log(2.0))); }
+
+double ref_fact(int k) {
+    double r = 1.0;
+    long long f = 1;
+    for (int i = 1; i <= k && i < 20; i++) f *= i;
+    r = (double)f;
+    for (int i = 20; i <= k; i++) r *= i;
+    return r;
+}
+
+double zernike_value(int n, int m, double theta, double rho) {
+    int am = m < 0 ? -m : m;
+    double som = 0.0;
+    for (int k = 0; k <= (n - am) / 2; k++) {
+        double truc = std::pow(-1.0, k) * ref_fact(n - k);
+        double trac = ref_fact(k) * ref_fact((n + am) / 2 - k) * ref_fact((n - am) / 2 - k);
+        som += (truc / trac) * std::pow(rho, n - 2 * k);
+    }
+    return m >= 0 ? som * std::cos(theta * m) : som * std::sin(theta * am);
+}
+
+}  // namespace
+
+struct zola_ctx {
+    zola_optics o{};
+    int N = 0, R = 0, s2 = 0, D = 0, J = 0;
+    double center = 0, ringsize = 0, radius_px = 0, kappa = 0, kz0 = 0, kzo0 = 0;
+    std::vector<int> di, dj, wu, wv;
+    std::vector<double> kz, kzo, amp, Z, coef, phase;
+    zola::Model64 model{};
+    float* d_tab = nullptr;  // 4 x 1024 floats: phi, amp, kz', kzo'
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    // grow-only staging for the host-pointer entry points
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    float* d_tmpl = nullptr;      // template stack for zola_fit_init_dev
+    size_t tmpl_bytes = 0;
+};
+
+static int set_err(zola_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_err(ctx, ZOLA_EHIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+static void compute_phase(zola_ctx* c) {
+    c->phase.assign(c->D, 0.0);
+    for (int j = 0; j < c->J; j++)
+        for (int p = 0; p < c->D; p++) c->phase[p] += c->Z[(size_t)j * c->D + p] * c->coef[j];
+}
+
+static void compute_h1(zola_ctx* c) {
+    // G1(k) = exp(-(k' - s2/2)^2 / (2 sigF^2)), k' = (k + s2/2) % s2, sigF = s2/(2 pi sigma);
+    // h1(d) = (1/s2) sum_k G1(k) cos(2 pi k d / s2)  (G1 is even, so h1 is real).
+    const int s2 = c->s2;
+    const double sigF = (1.0 / c->o.sigma) * s2 / (PI * 2);
+    const double sigpow = sigF * sigF;
+    std::vector<double> G(s2);
+    for (int k = 0; k < s2; k++) {
+        double kk = (double)((k + s2 / 2) % s2) - (double)(s2 / 2);
+        G[k] = std::exp(-.5 * kk * kk / sigpow);
+    }
+    for (int d = 0; d < s2; d++) {
+        double acc = 0;
+        for (int k = 0; k < s2; k++) acc += G[k] * std::cos(2 * PI * (double)k * d / s2);
+        c->model.h1[d] = (float)(acc / s2);
+    }
+}
+
+static int upload_tables(zola_ctx* c) {
+    std::vector<float> tab(4 * 1024, 0.f);
+    for (int p = 0; p < c->D; p++) {
+        int w = c->wu[p] * 32 + c->wv[p];
+        tab[w] = (float)c->phase[p];
+        tab[1024 + w] = (float)c->amp[p];
+        tab[2048 + w] = (float)(c->kz[p] - c->kz0);
+        tab[3072 + w] = (float)(c->kzo[p] - c->kzo0);
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->d_tab, tab.data(), tab.size() * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
+}
+
+static int stage(zola_ctx* c, size_t bytes) {
+    if (bytes <= c->stage_bytes) return ZOLA_OK;
+    if (c->d_stage) HIP_TRY(c, hipFree(c->d_stage));
+    c->d_stage = nullptr;
+    c->stage_bytes = 0;
+    HIP_TRY(c, hipMalloc(&c->d_stage, bytes));
+    c->stage_bytes = bytes;
+    return ZOLA_OK;
+}
+
+extern "C" {
+
+int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zola_ctx** out) {
+    if (!o || !out || R <= 0 || J < 0 || (J > 0 && !coeffs)) return ZOLA_EINVAL;
+    *out = nullptr;
+    zola_ctx* c = new zola_ctx();
+    c->o = *o;
+    int size = o->size_fft;
+    if (size % 2 != 0) size--;
+    if (size <= 0 || o->xystep <= 0 || o->wavelength <= 0 || o->sigma <= 0) {
+        *out = c;
+        return set_err(c, ZOLA_EINVAL, "invalid optics");
+    }
+    c->N = ref_pow2(size);
+    c->R = R;
+    c->J = J;
+    int s2 = 2 * R;
+    if (s2 > c->N) s2 = c->N;
+    c->s2 = ref_pow2(s2);
+    const int N = c->N;
+    c->center = (N / 2) - .5;
+    c->ringsize = (o->na / o->wavelength) * (o->na / o->wavelength);
+    c->radius_px = std::sqrt(c->ringsize) * (N * o->xystep);
+    c->kappa = 2 * PI / (o->xystep * N);
+    *out = c;
+    if (N != zola::kN)
+        return set_err(c, ZOLA_EUNSUPPORTED, "pupil grid N=%d: this build covers N=64", N);
+    if (c->s2 != N)
+        return set_err(c, ZOLA_EUNSUPPORTED, "blur grid s2=%d must equal N=%d (R >= N/4)", c->s2, N);
+    if (!zola_kernels_support_roi(R))
+        return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not instantiated", R);
+
+    const double left = (o->nwat / o->wavelength) * (o->nwat / o->wavelength);
+    const double left_oil = (o->noil / o->wavelength) * (o->noil / o->wavelength);
+    const double cc = c->center;
+    int D = 0;
+    for (int i = 0; i < N; i++)
+        for (int ii = 0; ii < N; ii++) {
+            double a = (i - cc) / (N * o->xystep), b = (ii - cc) / (N * o->xystep);
+            if (a * a + b * b <= c->ringsize) D++;
+        }
+    c->D = D;
+    c->di.resize(D); c->dj.resize(D); c->wu.resize(D); c->wv.resize(D);
+    c->kz.resize(D); c->kzo.resize(D); c->amp.resize(D);
+    int id = 0;
+    double kzmin = 1e300, kzmax = -1e300, kzomin = 1e300, kzomax = -1e300;
+    for (int i = 0; i < N; i++)
+        for (int ii = 0; ii < N; ii++) {
+            double a = (i - cc) / (N * o->xystep), b = (ii - cc) / (N * o->xystep);
+            if (!(a * a + b * b <= c->ringsize)) continue;
+            c->di[id] = i;
+            c->dj[id] = ii;
+            double kxx = ((i - cc) / (o->xystep * N)) * ((i - cc) / (o->xystep * N));
+            double kyy = ((ii - cc) / (o->xystep * N)) * ((ii - cc) / (o->xystep * N));
+            double right = kxx + kyy;
+            c->kz[id] = left > right ? 2 * PI * std::sqrt(left - right) : 0.0;
+            c->kzo[id] = left_oil > right ? 2 * PI * std::sqrt(left_oil - right) : 0.0;
+            if (o->with_apo) c->amp[id] = left_oil > right ? 1.0 / std::pow(1 - (right / left_oil), .25) : 0.0;
+            else c->amp[id] = 1.0 / std::sqrt((double)D);
+            int sx = (i + N / 2) % N, sy = (ii + N / 2) % N;
+            int nu = sx < N / 2 ? sx : sx - N, nv = sy < N / 2 ? sy : sy - N;
+            c->wu[id] = nu + zola::kW / 2;
+            c->wv[id] = nv + zola::kW / 2;
+            if (c->wu[id] < 0 || c->wu[id] >= zola::kW || c->wv[id] < 0 || c->wv[id] >= zola::kW)
+                return set_err(c, ZOLA_EUNSUPPORTED,
+                               "pupil radius %.3f px exceeds N/4 = %d px", c->radius_px, N / 4);
+            kzmin = std::fmin(kzmin, c->kz[id]); kzmax = std::fmax(kzmax, c->kz[id]);
+            kzomin = std::fmin(kzomin, c->kzo[id]); kzomax = std::fmax(kzomax, c->kzo[id]);
+            id++;
+        }
+    c->kz0 = 0.5 * (kzmin + kzmax);
+    c->kzo0 = 0.5 * (kzomin + kzomax);
+
+    // Zernike basis on the disk (order of utils/Zernike.java:26-79)
+    c->Z.assign((size_t)(J > 0 ? J : 1) * D, 0.0);
+    c->coef.assign(J > 0 ? J : 1, 0.0);
+    {
+        std::vector<int> zn(J), zm(J);
+        int idx = 0;
+        for (int rang = 0; idx < J; rang++)
+            for (int u = -rang; u <= rang && idx < J; u += 2) { zn[idx] = rang; zm[idx] = u; idx++; }
+        double cz = ((double)N) / 2. - .5;
+        for (int j = 0; j < J; j++)
+            for (int p = 0; p < D; p++) {
+                double x = ((double)(c->di[p] - cz) / c->radius_px);
+                double y = ((double)(c->dj[p] - cz) / c->radius_px);
+                double theta = std::atan2(y, x), rho = std::sqrt(x * x + y * y);
+                c->Z[(size_t)j * D + p] = rho < 1 ? zernike_value(zn[j], zm[j], theta, rho) : 0.0;
+            }
+        for (int j = 0; j < J; j++) c->coef[j] = coeffs[j];
+    }
+    compute_phase(c);
+    compute_h1(c);
+
+    HIP_TRY(c, hipStreamCreate(&c->stream));
+    c->own_stream = true;
+    HIP_TRY(c, hipMalloc((void**)&c->d_tab, 4 * 1024 * sizeof(float)));
+    c->model.phi = c->d_tab;
+    c->model.amp = c->d_tab + 1024;
+    c->model.kz = c->d_tab + 2048;
+    c->model.kzo = c->d_tab + 3072;
+    c->model.kappa = (float)c->kappa;
+    int rc = upload_tables(c);
+    if (rc) return rc;
+    c->err.clear();
+    return ZOLA_OK;
+}
+
+int zola_ctx_destroy(zola_ctx* c) {
+    if (!c) return ZOLA_OK;
+    if (c->d_tab) (void)hipFree(c->d_tab);
+    if (c->d_stage) (void)hipFree(c->d_stage);
+    if (c->d_tmpl) (void)hipFree(c->d_tmpl);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ZOLA_OK;
+}
+
+const char* zola_last_error(const zola_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int zola_ctx_info(const zola_ctx* c, int* info) {
+    if (!c || !info) return ZOLA_EINVAL;
+    info[0] = c->N; info[1] = c->D; info[2] = c->s2; info[3] = c->R; info[4] = c->J;
+    return ZOLA_OK;
+}
+
+int zola_set_coefficients(zola_ctx* c, const double* coeffs, int J) {
+    if (!c || !coeffs || J != c->J) return set_err(c, ZOLA_EINVAL, "coefficient count must be %d", c ? c->J : 0);
+    for (int j = 0; j < J; j++) c->coef[j] = coeffs[j];
+    compute_phase(c);
+    return upload_tables(c);
+}
+
+int zola_set_sigma(zola_ctx* c, double sigma) {
+    if (!c || !(sigma > 0)) return set_err(c, ZOLA_EINVAL, "sigma must be > 0");
+    c->o.sigma = sigma;
+    compute_h1(c);
+    return ZOLA_OK;
+}
+
+int zola_set_stream(zola_ctx* c, void* s) {
+    if (!c) return ZOLA_EINVAL;
+    if (c->own_stream && c->stream) HIP_TRY(c, hipStreamDestroy(c->stream));
+    c->own_stream = false;
+    c->stream = (hipStream_t)s;
+    return ZOLA_OK;
+}
+
+int zola_psf_forward_dev(zola_ctx* c, int B, const float* params, float* out) {
+    if (!c) return ZOLA_EINVAL;
+    if (B < 0 || (B > 0 && (!params || !out))) return set_err(c, ZOLA_EINVAL, "bad batch");
+    if (B == 0) return ZOLA_OK;
+    HIP_TRY(c, zola_launch_psf_forward64(c->model, c->R, B, params, out, c->stream));
+    return ZOLA_OK;
+}
+
+int zola_psf_forward(zola_ctx* c, int B, const float* params, float* out) {
+    if (!c) return ZOLA_EINVAL;
+    if (B < 0 || (B > 0 && (!params || !out))) return set_err(c, ZOLA_EINVAL, "bad batch");
+    if (B == 0) return ZOLA_OK;
+    size_t pb = (size_t)B * 4 * sizeof(float), ob = (size_t)B * c->R * c->R * sizeof(float);
+    int rc = stage(c, pb + ob);
+    if (rc) return rc;
+    float* dp = (float*)c->d_stage;
+    float* dout = (float*)((char*)c->d_stage + pb);
+    HIP_TRY(c, hipMemcpyAsync(dp, params, pb, hipMemcpyHostToDevice, c->stream));
+    rc = zola_psf_forward_dev(c, B, dp, dout);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
+}
+
+int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
+                       const double* init, double* theta, double* nll, double* crlb, int* iters,
+                       int* status) {
+    if (!c) return ZOLA_EINVAL;
+    if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
+    if (B == 0) return ZOLA_OK;
+    if (!pixels || !init || !theta || !nll || !crlb || !iters || !status)
+        return set_err(c, ZOLA_EINVAL, "null buffer");
+    if (cfg->mode != ZOLA_FIT_REFERENCE && cfg->mode != ZOLA_FIT_LM)
+        return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", cfg->mode);
+    if (cfg->max_iter < 0 || !(cfg->zmax >= cfg->zmin))
+        return set_err(c, ZOLA_EINVAL, "bad iteration count or z range");
+    zola::FitArgs A{};
+    A.B = B;
+    A.max_iter = cfg->max_iter;
+    A.xystep = (float)c->o.xystep;
+    A.zmin = cfg->zmin;
+    A.zmax = cfg->zmax;
+    A.zfocus = cfg->zfocus;
+    A.thr = cfg->photon_threshold;
+    A.tol = cfg->tol;
+    A.pixels = pixels;
+    A.init = init;
+    A.theta = theta;
+    A.nll = nll;
+    A.crlb = crlb;
+    A.iters = iters;
+    A.status = status;
+    HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, A, c->stream));
+    return ZOLA_OK;
+}
+
+int zola_fit_batch(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
+                   const double* init, double* theta, double* nll, double* crlb, int* iters,
+                   int* status) {
+    if (!c) return ZOLA_EINVAL;
+    if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
+    if (B == 0) return ZOLA_OK;
+    const size_t R2 = (size_t)c->R * c->R;
+    const size_t bp = B * R2 * sizeof(float), bi = (size_t)B * 5 * sizeof(double);
+    const size_t bt = bi, bn = (size_t)B * sizeof(double), bc = (size_t)B * 3 * sizeof(double);
+    const size_t bk = (size_t)B * sizeof(int);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = al(bp) + al(bi) + al(bt) + al(bn) + al(bc) + 2 * al(bk);
+    int rc = stage(c, total);
+    if (rc) return rc;
+    char* base = (char*)c->d_stage;
+    float* dpx = (float*)base; base += al(bp);
+    double* dini = (double*)base; base += al(bi);
+    double* dth = (double*)base; base += al(bt);
+    double* dnll = (double*)base; base += al(bn);
+    double* dcr = (double*)base; base += al(bc);
+    int* dit = (int*)base; base += al(bk);
+    int* dst = (int*)base;
+    HIP_TRY(c, hipMemcpyAsync(dpx, pixels, bp, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dini, init, bi, hipMemcpyHostToDevice, c->stream));
+    rc = zola_fit_batch_dev(c, cfg, B, dpx, dini, dth, dnll, dcr, dit, dst);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(theta, dth, bt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(nll, dnll, bn, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(crlb, dcr, bc, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(iters, dit, bk, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(status, dst, bk, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
+}
+
+int zola_fit_init_dev(zola_ctx* c, int B, const float* pixels, int nZ, double zmin_t,
+                      double zstep_t, double zfocus, double* init) {
+    if (!c) return ZOLA_EINVAL;
+    if (B < 0 || nZ <= 0 || !(zstep_t > 0)) return set_err(c, ZOLA_EINVAL, "bad init arguments");
+    if (B == 0) return ZOLA_OK;
+    if (!pixels || !init) return set_err(c, ZOLA_EINVAL, "null buffer");
+    const size_t R2 = (size_t)c->R * c->R;
+    const size_t need = (size_t)nZ * (R2 + 4) * sizeof(float);
+    if (need > c->tmpl_bytes) {
+        if (c->d_tmpl) HIP_TRY(c, hipFree(c->d_tmpl));
+        c->d_tmpl = nullptr;
+        c->tmpl_bytes = 0;
+        HIP_TRY(c, hipMalloc((void**)&c->d_tmpl, need));
+        c->tmpl_bytes = need;
+    }
+    // templates PSF(0, 0, z_r) with z_r = zmin_t + r * zstep_t (Predetection.java:118-125,190)
+    std::vector<float> par((size_t)nZ * 4, 0.f);
+    for (int r = 0; r < nZ; r++) {
+        par[(size_t)r * 4 + 2] = (float)(zmin_t + r * zstep_t);
+        par[(size_t)r * 4 + 3] = (float)zfocus;
+    }
+    float* dpar = c->d_tmpl + (size_t)nZ * R2;
+    HIP_TRY(c, hipMemcpyAsync(dpar, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
+    int rc = zola_psf_forward_dev(c, nZ, dpar, c->d_tmpl);
+    if (rc) return rc;
+    HIP_TRY(c, zola_launch_fit_init64(c->R, B, pixels, nZ, c->d_tmpl, zmin_t, zstep_t, init,
+                                      c->stream));
+    // the parameter upload must not be overwritten before the kernels read it
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,19 @@
+// zola_kernels.h -- host-side declarations of the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace zola {
+struct Model64;
+struct FitArgs;
+}
+
+bool zola_kernels_support_roi(int R);
+
+hipError_t zola_launch_psf_forward64(const zola::Model64& M, int R, int B, const float* params,
+                                     float* out, hipStream_t st);
+
+hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, const zola::FitArgs& A,
+                             hipStream_t st);
+
+hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,
+                                  double zmin_t, double zstep_t, double* init, hipStream_t st);

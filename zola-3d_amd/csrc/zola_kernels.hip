@@ -1,0 +1,214 @@
+// zola_kernels.hip -- gfx950 kernels of the ZOLA-3D localization hot path and
+// their host launchers (called from zola_host.cpp).
+#include <hip/hip_runtime.h>
+
+#include "zola_kernels.h"
+#include "zola_psf64.hpp"
+
+namespace zola {
+
+// ---------------------------------------------------------------------------
+// Batched PSF synthesis: one wave per PSF, WPB waves per block sharing the
+// pupil tables in LDS.
+// ---------------------------------------------------------------------------
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_psf_forward64(Model64 M, int B,
+                                                            const float* __restrict__ params,
+                                                            float* __restrict__ out) {
+    __shared__ BlockTables64 T;
+    __shared__ WaveScratch64 S[WPB];
+    load_tables64(T, M);
+    __syncthreads();
+    constexpr int G = RoiLayout<R>::G, RPL = RoiLayout<R>::RPL;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * WPB + w;
+    if (b >= B) return;
+    const float x = params[b * 4 + 0], y = params[b * 4 + 1];
+    const float z = params[b * 4 + 2], zoil = params[b * 4 + 3];
+    stage1_64(T, S[w], lane, 0, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
+    wave_lds_fence();
+    float acc[R];
+    float Er[2][32], Ei[2][32];
+    stage2_E64<R, false>(M, S[w], lane, acc, Er, Ei);
+    float o[RPL];
+    column_blur64<R>(T, S[w], lane, acc, o);
+    const int j = lane % R, g = lane / R;
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        o[t] = fabsf(o[t]);
+        const int i = g * RPL + t;
+        if (g < G && i < R) s += o[t];
+    }
+    s = wave_sum(s);
+    const float inv = 1.f / s;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        const int i = g * RPL + t;
+        if (g < G && i < R) out[(size_t)b * R * R + i * R + j] = o[t] * inv;
+    }
+}
+
+constexpr int kWPB = 8;
+
+template <int R>
+static hipError_t launch_psf_forward_r(const Model64& M, int B, const float* params, float* out,
+                                       hipStream_t st) {
+    dim3 grid((B + kWPB - 1) / kWPB), block(64 * kWPB);
+    hipLaunchKernelGGL((k_psf_forward64<R, kWPB>), grid, block, 0, st, M, B, params, out);
+    return hipGetLastError();
+}
+
+}  // namespace zola
+
+using namespace zola;
+
+#define ZOLA_R_DISPATCH(R, CALL) \
+    switch (R) {                 \
+        case 17: return CALL(17); \
+        case 21: return CALL(21); \
+        case 31: return CALL(31); \
+        case 41: return CALL(41); \
+        default: return hipErrorInvalidValue; \
+    }
+
+bool zola_kernels_support_roi(int R) {
+    return R == 17 || R == 21 || R == 31 || R == 41;
+}
+
+hipError_t zola_launch_psf_forward64(const Model64& M, int R, int B, const float* params,
+                                     float* out, hipStream_t st) {
+#define CALL(RR) launch_psf_forward_r<RR>(M, B, params, out, st)
+    ZOLA_R_DISPATCH(R, CALL)
+#undef CALL
+}
+
+// ---------------------------------------------------------------------------
+// Fits and their initial values
+// ---------------------------------------------------------------------------
+#include "zola_fit64.hpp"
+
+namespace zola {
+
+// Initial values for an isolated ROI: best zero-shift normalised correlation
+// against the template stack (z refined by the reference's parabola,
+// process/cpu/Predetection.java:529-531,589-593), then getAandB with that
+// template (process/cpu/LocalizationPipeline.java:1119-1162).
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __restrict__ pixels,
+                                                         int nZ, const float* __restrict__ tmpl,
+                                                         double zmin_t, double zstep_t,
+                                                         double* __restrict__ init) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * WPB + w;
+    if (b >= B) return;
+    const int j = lane % R, g = lane / R;
+    float n[RPL];
+    double sn = 0, snn = 0;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        n[t] = Px<R>::valid(lane, t) ? pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+        sn += n[t];
+        snn += (double)n[t] * n[t];
+    }
+    sn = wave_sum(sn);
+    snn = wave_sum(snn);
+    const double P = (double)R * R;
+    const double vn = snn - sn * sn / P;
+    auto corr = [&](int r) -> double {
+        double st = 0, stt = 0, stn = 0;
+#pragma unroll
+        for (int t = 0; t < RPL; t++) {
+            if (!Px<R>::valid(lane, t)) continue;
+            const double tv = tmpl[(size_t)r * R * R + (g * RPL + t) * R + j];
+            st += tv;
+            stt += tv * tv;
+            stn += tv * n[t];
+        }
+        st = wave_sum(st);
+        stt = wave_sum(stt);
+        stn = wave_sum(stn);
+        const double vt = stt - st * st / P;
+        return (stn - st * sn / P) / sqrt(vt * vn);
+    };
+    int best = 0;
+    double cbest = -1e300;
+    for (int r = 0; r < nZ; r++) {
+        const double c = corr(r);
+        if (c > cbest) { cbest = c; best = r; }
+    }
+    double z0 = zmin_t + best * zstep_t;
+    if (best > 1 && best < nZ - 1) {
+        const double ya = corr(best - 1), yb = cbest, yc = corr(best + 1);
+        const double xa = -zstep_t, xb = 0.0, xc = zstep_t;
+        const double a = (yc - ya) / ((xc - xa) * (xc - xb)) - (yb - ya) / ((xb - xa) * (xc - xb));
+        const double bq = ((yb - ya) / (xb - xa)) - a * (xb + xa);
+        z0 += -bq / (2 * a);
+    }
+    // getAandB(patch, template, 0.1)
+    float tmax = -3.0e38f;
+#pragma unroll
+    for (int t = 0; t < RPL; t++)
+        if (Px<R>::valid(lane, t)) tmax = fmaxf(tmax, tmpl[(size_t)best * R * R + (g * RPL + t) * R + j]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
+    const float thr = tmax * 0.1f;
+    double sb = 0, cb = 0, sa = 0, ca = 0;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        if (!Px<R>::valid(lane, t)) continue;
+        const float tv = tmpl[(size_t)best * R * R + (g * RPL + t) * R + j];
+        if (tv < thr) { sb += n[t]; cb += 1; } else { sa += n[t]; ca += 1; }
+    }
+    sb = wave_sum(sb); cb = wave_sum(cb); sa = wave_sum(sa); ca = wave_sum(ca);
+    const double Bv = cb != 0 ? sb / cb : 0.0;
+    const double Av = sa - ca * Bv;
+    if (lane == 0) {
+        init[(size_t)b * 5 + 0] = Av;
+        init[(size_t)b * 5 + 1] = Bv;
+        init[(size_t)b * 5 + 2] = 0.0;
+        init[(size_t)b * 5 + 3] = 0.0;
+        init[(size_t)b * 5 + 4] = z0;
+    }
+}
+
+constexpr int kWPB_LM = 4;
+constexpr int kWPB_REF = 4;
+
+template <int R>
+static hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st) {
+    if (mode == 1) {
+        dim3 grid((A.B + kWPB_LM - 1) / kWPB_LM), block(64 * kWPB_LM);
+        hipLaunchKernelGGL((k_fit_lm64<R, kWPB_LM>), grid, block, 0, st, M, A);
+    } else {
+        dim3 grid((A.B + kWPB_REF - 1) / kWPB_REF), block(64 * kWPB_REF);
+        hipLaunchKernelGGL((k_fit_ref64<R, kWPB_REF>), grid, block, 0, st, M, A);
+    }
+    return hipGetLastError();
+}
+
+template <int R>
+static hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
+                                double zstep_t, double* init, hipStream_t st) {
+    constexpr int W = 4;
+    dim3 grid((B + W - 1) / W), block(64 * W);
+    hipLaunchKernelGGL((k_fit_init64<R, W>), grid, block, 0, st, B, pixels, nZ, tmpl, zmin_t,
+                       zstep_t, init);
+    return hipGetLastError();
+}
+
+}  // namespace zola
+
+hipError_t zola_launch_fit64(const Model64& M, int R, int mode, const FitArgs& A, hipStream_t st) {
+#define CALL(RR) launch_fit_r<RR>(M, mode, A, st)
+    ZOLA_R_DISPATCH(R, CALL)
+#undef CALL
+}
+
+hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,
+                                  double zmin_t, double zstep_t, double* init, hipStream_t st) {
+#define CALL(RR) launch_init_r<RR>(B, pixels, nZ, tmpl, zmin_t, zstep_t, init, st)
+    ZOLA_R_DISPATCH(R, CALL)
+#undef CALL
+}

@@ -1,0 +1,216 @@
+// zola_psf64.hpp -- wave-per-PSF synthesis of the ZOLA PSF model on a 64x64
+// pupil grid, gfx950.
+//
+// Model (process/cpu/PSFPhaseMany.java:553-622):
+//   P(u,v) = A(u,v) exp(i q),  q = phi + kx x + ky y + kz z - kzoil zoil
+//   E = IDFT_64x64(P)                 (pupil scattered fftshifted)
+//   I = |E|^2 on the s2 x s2 grid     (s2 = 64 here)
+//   Bl = IDFT(G . DFT(I))             (Gaussian blur, G separable)
+//   psf = |Bl| cropped to R x R, divided by its sum
+//
+// Layout choices (all per wave = one PSF/ROI):
+//  * The pupil has radius < N/4, so its fftshifted support lies in a 32x32
+//    window of signed frequencies n in [-16, 15].  A 64-point IDFT of a
+//    32-point support splits into two 32-point IDFTs (even / odd outputs)
+//    after a twiddle exp(2 pi i n h / 64).  Each lane runs 32-point
+//    transforms entirely in VGPRs.
+//  * Stage 1 (along the first axis): lane = (column v, half h).  The stage-1
+//    twiddle is folded into the phase q before sincos, so it is free.
+//  * LDS transpose (split re/im arrays, row stride 33: conflict free).
+//  * Stage 2 (along the second axis): lane = grid row a, both halves.
+//  * The Gaussian blur is applied as a separable circular convolution with the
+//    exact s2-periodic kernel h1 = IDFT(G1) (real, even), so no further FFTs:
+//    row pass in the lane that owns the row (coefficients are wave-uniform,
+//    read from SGPRs), column pass after a second LDS transpose, lane =
+//    (ROI column j, group of RPL ROI rows).
+// Tangent fields dE/dtheta = IDFT(i f P) (f = kx, ky or kz) reuse the same
+// stages; the tangent image is 2 Re(conj(E) dE).
+#pragma once
+#include "zola_fft.hpp"
+
+namespace zola {
+
+constexpr int kN = 64;     // pupil / E grid
+constexpr int kW = 32;     // support window
+constexpr int kS1 = 33;    // LDS row stride of the stage-1 transpose
+constexpr int kTS = 68;    // LDS row stride of the column-blur transpose
+
+struct alignas(16) BlockTables64 {
+    float phi[kW * kW];    // aberration phase on the window [wu][wv]
+    float amp[kW * kW];    // pupil amplitude (0 outside the disk)
+    float kz[kW * kW];     // kz - kz0 (rad/um)
+    float kzo[kW * kW];    // kzoil - kzoil0 (rad/um)
+    float h3[192];         // blur kernel h1, repeated three times
+};
+
+struct alignas(16) WaveScratch64 {
+    float re[kN * kS1];
+    float im[kN * kS1];
+};
+
+// Model constants shared by all kernels (passed by value in kernel args).
+struct Model64 {
+    const float* phi;  // global copies of the window tables (1024 each)
+    const float* amp;
+    const float* kz;
+    const float* kzo;
+    float kappa;       // 2 pi / (dx N): kx = kappa (n_u + 1/2)
+    float h1[64];      // blur kernel over signed offset d (index d & 63)
+};
+
+template <int R>
+struct RoiLayout {
+    static constexpr int G = 64 / R;              // lane groups over ROI rows
+    static constexpr int RPL = (R + G - 1) / G;   // ROI rows per lane
+};
+
+__device__ __forceinline__ void load_tables64(BlockTables64& T, const Model64& M) {
+    for (int t = threadIdx.x; t < kW * kW; t += blockDim.x) {
+        T.phi[t] = M.phi[t];
+        T.amp[t] = M.amp[t];
+        T.kz[t] = M.kz[t];
+        T.kzo[t] = M.kzo[t];
+    }
+    for (int t = threadIdx.x; t < 192; t += blockDim.x) T.h3[t] = M.h1[t & 63];
+}
+
+// Signed frequency of FFT input slot j (j < 16: j, else j - 32).
+__device__ __forceinline__ constexpr int slot_n(int j) { return j < 16 ? j : j - 32; }
+
+// Stage 1 for one field (runtime, wave-uniform selector):
+//   field 0 = E, 1 = dE/dx, 2 = dE/dy, 3 = dE/dz  (dE = IDFT(i f P)).
+// Writes S1(a, v) = sum_u P(u,v) w^(n_u a) for all 64 rows a.
+__device__ __forceinline__ void stage1_64(const BlockTables64& T, WaveScratch64& S, int lane,
+                                          int field, float X, float Y, float z, float zoil,
+                                          float kappa) {
+    const int v = lane & 31;
+    const int h = lane >> 5;
+    const float nv = (float)(v - 16);
+    const float Xh = X + (float)h * (float)(2.0 * kPi / 64.0);
+    const float cY = Y * (nv + 0.5f) + 0.5f * X;
+    // i f P = (-f Pi, f Pr); for field 0 the factor is (c0, f) = (1, 0).
+    const float c0 = field == 0 ? 1.f : 0.f;
+    const float fx = field == 1 ? kappa : 0.f;
+    const float fy = field == 2 ? kappa * (nv + 0.5f) : 0.f;
+    const float fz = field == 3 ? 1.f : 0.f;
+    float re[32], im[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+        const int nu = slot_n(j);
+        const int wi = (nu + 16) * kW + v;
+        const float kzv = T.kz[wi];
+        const float q = T.phi[wi] + Xh * (float)nu + cY + kzv * z - T.kzo[wi] * zoil;
+        float s, c;
+        sincosf(q, &s, &c);
+        const float a = T.amp[wi];
+        const float pr = a * c, pi = a * s;
+        const float f = fx * ((float)nu + 0.5f) + fy + fz * kzv;
+        re[j] = c0 * pr - f * pi;
+        im[j] = c0 * pi + f * pr;
+    }
+    fft_dif<32, +1>(re, im);
+#pragma unroll
+    for (int m = 0; m < 32; m++) {
+        const int a = 2 * m + h;
+        S.re[a * kS1 + v] = re[fidx<32>(m)];
+        S.im[a * kS1 + v] = im[fidx<32>(m)];
+    }
+}
+
+// Stage 2 half hp for row a = lane: returns E(a, 2m'+hp) in natural order.
+template <int HP>
+__device__ __forceinline__ void stage2_half64(const WaveScratch64& S, int lane, float (&re)[32],
+                                              float (&im)[32]) {
+    // Re-read the row for each half instead of keeping 64 loaded values live
+    // across the first half's transform (the compiler would otherwise CSE the
+    // two identical sets of LDS loads).
+    __asm__ __volatile__("" ::: "memory");
+    static_for<0, 32>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        constexpr int v = slot_n(j) + 16;
+        float r = S.re[lane * kS1 + v], i = S.im[lane * kS1 + v];
+        if constexpr (HP == 1) twiddle_mul<64, (slot_n(j) + 64) % 64, +1>(r, i);
+        re[j] = r;
+        im[j] = i;
+    });
+    fft_dif<32, +1>(re, im);
+}
+
+// Row blur of one image value I at column b of this lane's row into the R
+// accumulators: T(a, j) += h1[(j - R/2 - b) & 63] * I.
+template <int R, int B>
+__device__ __forceinline__ void row_blur_add(const Model64& M, float I, float (&acc)[R]) {
+#pragma unroll
+    for (int j = 0; j < R; j++) acc[j] += M.h1[(j - R / 2 - B) & 63] * I;
+}
+
+// Stage 2 of the E field: fills the row-blur accumulators, optionally keeps E.
+template <int R, bool KEEP>
+__device__ __forceinline__ void stage2_E64(const Model64& M, const WaveScratch64& S, int lane,
+                                           float (&acc)[R], float (&Er)[2][32],
+                                           float (&Ei)[2][32]) {
+#pragma unroll
+    for (int j = 0; j < R; j++) acc[j] = 0.f;
+    static_for<0, 2>([&](auto hpc) {
+        constexpr int HP = decltype(hpc)::value;
+        float re[32], im[32];
+        stage2_half64<HP>(S, lane, re, im);
+        static_for<0, 32>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const float er = re[fidx<32>(m)], ei = im[fidx<32>(m)];
+            if constexpr (KEEP) {
+                Er[HP][m] = er;
+                Ei[HP][m] = ei;
+            }
+            row_blur_add<R, 2 * m + HP>(M, er * er + ei * ei, acc);
+        });
+    });
+}
+
+// Stage 2 of a tangent field: image 2 Re(conj(E) dE).
+template <int R>
+__device__ __forceinline__ void stage2_T64(const Model64& M, const WaveScratch64& S, int lane,
+                                           float (&acc)[R], const float (&Er)[2][32],
+                                           const float (&Ei)[2][32]) {
+#pragma unroll
+    for (int j = 0; j < R; j++) acc[j] = 0.f;
+    static_for<0, 2>([&](auto hpc) {
+        constexpr int HP = decltype(hpc)::value;
+        float re[32], im[32];
+        stage2_half64<HP>(S, lane, re, im);
+        static_for<0, 32>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const float dr = re[fidx<32>(m)], di = im[fidx<32>(m)];
+            const float val = 2.f * (Er[HP][m] * dr + Ei[HP][m] * di);
+            row_blur_add<R, 2 * m + HP>(M, val, acc);
+        });
+    });
+}
+
+// Column blur: stores this lane's row accumulators transposed in LDS, then
+// lane (j, g) forms out(g*RPL + t, j) = sum_a h1[(i - R/2 - a) & 63] T(a, j).
+template <int R>
+__device__ __forceinline__ void column_blur64(const BlockTables64& T, WaveScratch64& S, int lane,
+                                              const float (&acc)[R],
+                                              float (&out)[RoiLayout<R>::RPL]) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    wave_lds_fence();  // all lanes are done reading the stage-1 data
+    float* Tt = S.re;
+#pragma unroll
+    for (int j = 0; j < R; j++) Tt[j * kTS + lane] = acc[j];
+    wave_lds_fence();
+    const int j = lane % R;
+    const int g = lane / R;
+    const int base = 64 + ((g * RPL - R / 2) & 63);
+#pragma unroll
+    for (int t = 0; t < RPL; t++) out[t] = 0.f;
+#pragma unroll
+    for (int a = 0; a < 64; a++) {
+        const float tv = Tt[j * kTS + a];
+#pragma unroll
+        for (int t = 0; t < RPL; t++) out[t] += T.h3[base + t - a] * tv;
+    }
+    wave_lds_fence();  // the scratch is reused by the next stage 1
+}
+
+}  // namespace zola

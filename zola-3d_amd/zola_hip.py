@@ -1,0 +1,231 @@
+"""ctypes binding of libzola_hip.so (include/zola_hip.h).
+
+Mirrors the reference's host-side objects for the hot path: a context plays
+the role of DataPhase_ + PSFmany_float_ + Modelmany_double_
+(src/main/java/org/pasteur/imagej/process/gpu/DataPhase_.java,
+PSFmany_float_.java, Modelmany_double_.java); errors raise ZolaError with the
+library's message, as the reference logs cudaResult failures.  The library
+must be built (zola-3d_amd/build.py); there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzola_hip.so")
+
+FIT_REFERENCE = 0
+FIT_LM = 1
+ST_CONVERGED, ST_MAXITER, ST_NAN, ST_LOWPHOTON = 1, 2, 4, 8
+
+SYMBOLS = [
+    "zola_ctx_create", "zola_ctx_destroy", "zola_last_error", "zola_ctx_info",
+    "zola_set_coefficients", "zola_set_sigma", "zola_set_stream", "zola_psf_forward",
+    "zola_psf_forward_dev", "zola_psf_backward", "zola_psf_backward_dev", "zola_fit_batch",
+    "zola_fit_batch_dev", "zola_fit_init_dev",
+]
+
+
+class ZolaError(RuntimeError):
+    pass
+
+
+class Optics(C.Structure):
+    _fields_ = [("size_fft", C.c_int), ("xystep", C.c_double), ("zstep", C.c_double),
+                ("wavelength", C.c_double), ("noil", C.c_double), ("nwat", C.c_double),
+                ("na", C.c_double), ("sigma", C.c_double), ("with_apo", C.c_int),
+                ("reserved", C.c_int)]
+
+
+class FitConfig(C.Structure):
+    _fields_ = [("mode", C.c_int), ("max_iter", C.c_int), ("zmin", C.c_double),
+                ("zmax", C.c_double), ("zfocus", C.c_double), ("photon_threshold", C.c_double),
+                ("tol", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ZolaError("libzola_hip.so is not built: run zola-3d_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.zola_ctx_create.argtypes = [C.POINTER(Optics), C.c_int, vp, C.c_int, C.POINTER(vp)]
+        L.zola_ctx_destroy.argtypes = [vp]
+        L.zola_last_error.restype = C.c_char_p
+        L.zola_last_error.argtypes = [vp]
+        L.zola_ctx_info.argtypes = [vp, vp]
+        L.zola_set_coefficients.argtypes = [vp, vp, C.c_int]
+        L.zola_set_sigma.argtypes = [vp, C.c_double]
+        L.zola_set_stream.argtypes = [vp, vp]
+        for name in ("zola_psf_forward", "zola_psf_forward_dev"):
+            getattr(L, name).argtypes = [vp, C.c_int, vp, vp]
+        for name in ("zola_psf_backward", "zola_psf_backward_dev"):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        for name in ("zola_fit_batch", "zola_fit_batch_dev"):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = [vp, C.POINTER(FitConfig), C.c_int, vp, vp, vp, vp,
+                                             vp, vp, vp]
+        if hasattr(L, "zola_fit_init_dev"):
+            L.zola_fit_init_dev.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
+                                            C.c_double, vp]
+        _lib = L
+    return _lib
+
+
+def optics(size_fft=64, xystep=0.11, zstep=0.05, wavelength=0.68, noil=1.518, nwat=1.518,
+           na=1.45, sigma=1.0, with_apo=0):
+    return Optics(size_fft, xystep, zstep, wavelength, noil, nwat, na, sigma, with_apo, 0)
+
+
+def _ptr(a):
+    """Data pointer of a numpy array or a torch tensor."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """A PSF model (optics + Zernike coefficients) for ROIs of side R."""
+
+    def __init__(self, opt, R, coeffs):
+        self.opt = opt
+        self.R = int(R)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        self.J = len(c)
+        h = C.c_void_p()
+        rc = lib().zola_ctx_create(C.byref(opt), self.R, _ptr(c), self.J, C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = lib().zola_last_error(h).decode() if h.value else "zola_ctx_create failed"
+            lib().zola_ctx_destroy(h)
+            self._h = None
+            raise ZolaError(f"zola_ctx_create: {msg} (rc={rc})")
+        info = np.zeros(5, np.int32)
+        lib().zola_ctx_info(self._h, _ptr(info))
+        self.N, self.D, self.s2 = int(info[0]), int(info[1]), int(info[2])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().zola_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise ZolaError(f"{what}: {lib().zola_last_error(self._h).decode()} (rc={rc})")
+
+    def set_coefficients(self, coeffs):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        self._check(lib().zola_set_coefficients(self._h, _ptr(c), len(c)), "set_coefficients")
+
+    def set_sigma(self, sigma):
+        self._check(lib().zola_set_sigma(self._h, float(sigma)), "set_sigma")
+
+    def set_stream(self, stream_handle):
+        self._check(lib().zola_set_stream(self._h, C.c_void_p(stream_handle)), "set_stream")
+
+    # -- PSF ---------------------------------------------------------------
+    def psf_forward(self, params):
+        """params: (B, 4) [x, y, z, zoil] um -> (B, R, R) float32 (host arrays)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        B = p.shape[0]
+        out = np.zeros((B, self.R, self.R), np.float32)
+        self._check(lib().zola_psf_forward(self._h, B, _ptr(p), _ptr(out)), "psf_forward")
+        return out
+
+    def psf_forward_dev(self, params, out):
+        """Device form: torch float32 tensors already on the GPU."""
+        self._check(lib().zola_psf_forward_dev(self._h, params.shape[0], _ptr(params), _ptr(out)),
+                    "psf_forward_dev")
+
+    def psf_backward(self, params, w, with_coeffs=True):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        B = p.shape[0]
+        dp = np.zeros((B, 4), np.float32)
+        da = np.zeros(self.J, np.float64) if with_coeffs else None
+        self._check(lib().zola_psf_backward(self._h, B, _ptr(p), _ptr(w), _ptr(dp), _ptr(da)),
+                    "psf_backward")
+        return dp, da
+
+    def psf_backward_dev(self, params, w, dparams, da=None):
+        self._check(lib().zola_psf_backward_dev(self._h, params.shape[0], _ptr(params), _ptr(w),
+                                                _ptr(dparams), _ptr(da)), "psf_backward_dev")
+
+    # -- fits --------------------------------------------------------------
+    @staticmethod
+    def fit_config(mode=FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, zfocus=0.0,
+                   photon_threshold=0.0, tol=1e-4):
+        return FitConfig(mode, max_iter, zmin, zmax, zfocus, photon_threshold, tol)
+
+    def fit(self, cfg, pixels, init):
+        """Host form. pixels (B, R, R) float32, init (B, 5) float64 [A, B, x, y, z]."""
+        px = np.ascontiguousarray(pixels, dtype=np.float32)
+        ini = np.ascontiguousarray(init, dtype=np.float64)
+        B = px.shape[0]
+        theta = np.zeros((B, 5)); nll = np.zeros(B); crlb = np.zeros((B, 3))
+        iters = np.zeros(B, np.int32); status = np.zeros(B, np.int32)
+        self._check(lib().zola_fit_batch(self._h, C.byref(cfg), B, _ptr(px), _ptr(ini),
+                                         _ptr(theta), _ptr(nll), _ptr(crlb), _ptr(iters),
+                                         _ptr(status)), "fit_batch")
+        return dict(theta=theta, nll=nll, crlb=crlb, iters=iters, status=status)
+
+    def fit_dev(self, cfg, pixels, init, theta, nll, crlb, iters, status):
+        self._check(lib().zola_fit_batch_dev(self._h, C.byref(cfg), pixels.shape[0],
+                                             _ptr(pixels), _ptr(init), _ptr(theta), _ptr(nll),
+                                             _ptr(crlb), _ptr(iters), _ptr(status)),
+                    "fit_batch_dev")
+
+    def fit_init_dev(self, pixels, nZ, zmin_t, zstep_t, zfocus, init):
+        self._check(lib().zola_fit_init_dev(self._h, pixels.shape[0], _ptr(pixels), int(nZ),
+                                            float(zmin_t), float(zstep_t), float(zfocus),
+                                            _ptr(init)), "fit_init_dev")
+
+
+# -- Zernike index helpers (calibration-file conventions) ------------------
+def noll_to_nm(j):
+    """Noll index j >= 1 -> (n, m), m > 0 for cos and m < 0 for sin terms."""
+    n, j1 = 0, j - 1
+    while j1 > n:
+        n += 1
+        j1 -= n
+    m = (-1) ** j * ((n % 2) + 2 * int((j1 + ((n + 1) % 2)) / 2))
+    return n, m
+
+
+def nm_to_slot(n, m):
+    """(n, m) -> 0-based slot of the reference basis order (utils/Zernike.java:26-79)."""
+    return (n * (n + 2) + m) // 2
+
+
+def coeffs_from_noll(noll_values, J):
+    """Dict {noll j: rad} -> coefficient vector in the reference order."""
+    a = np.zeros(J)
+    for j, v in noll_values.items():
+        s = nm_to_slot(*noll_to_nm(j))
+        if s >= J:
+            raise ValueError(f"Noll {j} does not fit in {J} coefficients")
+        a[s] = v
+    return a
+
+
+def astigmatic_coeffs(J=28, seed=0x20A1D3, astig=1.0, sd=0.05):
+    """Astigmatic PSF of the isolated-emitter workload: Noll 5 = astig rad,
+    Noll 4..J (other than 5) ~ Normal(0, sd)."""
+    rng = np.random.default_rng(seed)
+    vals = {j: rng.normal(0.0, sd) for j in range(4, J + 1)}
+    vals[5] = astig
+    return coeffs_from_noll(vals, J)
