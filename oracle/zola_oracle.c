@@ -871,8 +871,8 @@ static int solve_spd(int n, double* A, double* b) {
 /* theta order everywhere in this section: [A, B, x, y, z]. */
 
 /* Relative noise floor of the likelihood comparison (times A + npx*B);
- * 1e-6 matches FP32 synthesis, 1e-12 is used for the FP64 optimum. */
-static double lm_noise_floor = 1e-12;
+ * 0 for the FP64 optimum; 2e-7 mirrors the FP32 GPU fit. */
+static double lm_noise_floor = 0.0;
 void zo_set_lm_noise_floor(double v) { lm_noise_floor = v; }
 static double lm_eval(zo_model* m, const double* roi, const double* th, double zfocus,
                       double* psf, double* dpsf, double* g, double* H) {
@@ -914,6 +914,10 @@ void zo_fisher_analytic(zo_model* m, const double* theta, double zfocus, double*
 int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, double zmin,
               double zmax, int maxit, double tol, double* out_nll, int* out_iter,
               double* out_crlb) {
+    /* Fisher-scoring Levenberg-Marquardt with Nielsen's damping update:
+     * success: lambda *= max(1/3, 1-(2 rho-1)^3), nu = 2 (rho = actual/predicted
+     * decrease); failure: lambda *= nu, nu *= 2.  A trial is accepted when the
+     * likelihood rises by at most lm_noise_floor * (A + npx B). */
     int n = m->R * m->R;
     double* psf = (double*)xcalloc(n, sizeof(double));
     double* dpsf = (double*)xcalloc((size_t)3 * n, sizeof(double));
@@ -921,7 +925,7 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
     memcpy(th, theta, sizeof(th));
     th[0] = fmax(th[0], 1.0);
     th[1] = fmax(th[1], 0.01);
-    double lambda = 1e-3;
+    double lambda = 1e-3, nu = 2.0;
     double dx = m->o.xystep;
     double nll = lm_eval(m, roi, th, zfocus, psf, dpsf, g, H);
     int it, conv = 0;
@@ -931,7 +935,13 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             double A[25], d[5], tn[5];
             for (int p = 0; p < 25; p++) A[p] = H[p];
             for (int p = 0; p < 5; p++) { A[p * 5 + p] *= (1.0 + lambda); d[p] = -g[p]; }
-            if (solve_spd(5, A, d) != 0) { lambda *= 10; continue; }
+            if (solve_spd(5, A, d) != 0) { lambda *= nu; nu *= 2; continue; }
+            double pred = 0.0;
+            for (int p = 0; p < 5; p++) {
+                double hd = 0.0;
+                for (int q = 0; q < 5; q++) hd += H[p * 5 + q] * d[q];
+                pred -= g[p] * d[p] + 0.5 * d[p] * hd;
+            }
             for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
             for (int p = 2; p < 4; p++) {
                 if (d[p] > dx) tn[p] = th[p] + dx;
@@ -942,24 +952,20 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             tn[0] = fmax(tn[0], 1.0);
             tn[1] = fmax(tn[1], 0.01);
             tn[4] = fmin(fmax(tn[4], zmin), zmax);
-            /* predicted decrease of the quadratic model: below the likelihood's
-             * noise floor the step is taken without the comparison */
-            double pred = 0.0;
-            for (int p = 0; p < 5; p++) {
-                double hd = 0.0;
-                for (int q = 0; q < 5; q++) hd += H[p * 5 + q] * d[q];
-                pred -= g[p] * d[p] + 0.5 * d[p] * hd;
-            }
             double nn = lm_nll(m, roi, tn, zfocus, psf);
-            if (pred < lm_noise_floor * (th[0] + (double)n * th[1]) || nn <= nll) {
+            if (nn <= nll + lm_noise_floor * (th[0] + (double)n * th[1])) {
                 double step = fmax(fabs(tn[2] - th[2]), fmax(fabs(tn[3] - th[3]), fabs(tn[4] - th[4])));
+                double rho = pred > 0 ? (nll - nn) / pred : 0.0;
+                double t3 = 2.0 * rho - 1.0;
+                lambda *= fmax(1.0 / 3.0, 1.0 - t3 * t3 * t3);
+                nu = 2.0;
                 memcpy(th, tn, sizeof(th));
-                lambda = fmax(lambda * 0.1, 1e-9);
                 accepted = 1;
                 if (step < tol) conv = 1;
                 nll = lm_eval(m, roi, th, zfocus, psf, dpsf, g, H);
             } else {
-                lambda *= 10;
+                lambda *= nu;
+                nu *= 2;
             }
         }
         if (!accepted) conv = 1;   /* no decrease possible: numerical optimum */

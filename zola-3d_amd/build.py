@@ -3,6 +3,8 @@
 hipcc compiles every source as HIP for --offload-arch=gfx950.  Packed FP32
 instructions are disabled (-packed-fp32-ops) on purpose.
 """
+import concurrent.futures as cf
+import glob
 import os
 import subprocess
 import sys
@@ -10,21 +12,25 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libzola_hip.so")
-SOURCES = ["zola_kernels.hip", "zola_host.cpp"]
+SOURCES = (["zola_dispatch.hip", "zola_host.cpp"] +
+           sorted(os.path.basename(p) for p in glob.glob(os.path.join(CSRC, "zola_inst_r*.hip"))))
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 
-def build(verbose=False):
-    objs = []
-    for s in SOURCES:
-        src = os.path.join(CSRC, s)
-        obj = os.path.join(CSRC, os.path.splitext(s)[0] + ".o")
-        cmd = ["hipcc", "-x", "hip"] + FLAGS + ["-c", src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+def _compile(src, verbose):
+    obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+    cmd = ["hipcc", "-x", "hip"] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build(verbose=False, jobs=None):
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 1)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = OUT + ".tmp"
     cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True)

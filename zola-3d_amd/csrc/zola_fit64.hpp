@@ -23,6 +23,7 @@ struct FitArgs {
     int max_iter;
     float xystep;
     double zmin, zmax, zfocus, thr, tol;
+    double noise_rel;         // LM: likelihood noise tolerance, relative to A + npx*B
     const float* pixels;      // B*R*R
     const double* init;       // B*5 [A, B, x, y, z]
     double* theta;            // B*5
@@ -44,21 +45,17 @@ struct Px {
     }
 };
 
-// One pupil field at (x, y, z, zoil): field 0 = E (kept in Er/Ei when KEEP),
-// 1..3 = tangent fields d/dx, d/dy, d/dz whose image is 2 Re(conj(E) dE).
-// Returns the raw blurred values v (signed) of this lane's ROI pixels.  Every
-// caller runs it from a single site inside its iteration loop.
-template <int R, bool KEEP>
-__device__ __forceinline__ void eval_field64(const Model64& M, const BlockTables64& T,
-                                             WaveScratch64& S, int lane, int field, float x,
-                                             float y, float z, float zoil, float (&Er)[2][32],
-                                             float (&Ei)[2][32],
-                                             float (&v)[RoiLayout<R>::RPL]) {
-    stage1_64(T, S, lane, field, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
+// E field at (x, y, z, zoil): raw blurred values v (signed) of this lane's
+// ROI pixels.  Every caller runs it from a single site in its iteration loop.
+template <int R>
+__device__ __forceinline__ void eval_E64(const Model64& M, const BlockTables64& T,
+                                         WaveScratch64& S, int lane, float h1reg, float x,
+                                         float y, float z, float zoil,
+                                         float (&v)[RoiLayout<R>::RPL]) {
+    stage1_64(T, S, lane, 0, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
     wave_lds_fence();
     float acc[R];
-    if (!KEEP || field == 0) stage2_E64<R, KEEP>(M, S, lane, acc, Er, Ei);
-    else stage2_T64<R>(M, S, lane, acc, Er, Ei);
+    stage2_E64<R>(h1reg, S, lane, acc);
     column_blur64<R>(T, S, lane, acc, v);
 }
 
@@ -146,9 +143,48 @@ __device__ __forceinline__ bool solve5(double (&A)[25], double (&b)[5]) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Levenberg-Marquardt (theta order [A, B, x, y, z])
+//
+// Every trial point is evaluated with its Jacobian: two pupil transforms (E and
+// dE/dz) and four separable blurs of the same intensity.  d/dx and d/dy need
+// no transform: moving the emitter by x moves the band-limited intensity by
+// x/dx samples, so the blurred x/y tangents are the intensity blurred with the
+// spectral-derivative kernel hd along that axis.
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ __forceinline__ void lm_eval64(const Model64& M, const BlockTables64& T, float* scr,
+                                          int lane, float x, float y, float z, float zoil,
+                                          float (&vB)[RoiLayout<R>::RPL],
+                                          float (&vX)[RoiLayout<R>::RPL],
+                                          float (&vY)[RoiLayout<R>::RPL],
+                                          float (&vZ)[RoiLayout<R>::RPL]) {
+    WaveScratch64& SE = *reinterpret_cast<WaveScratch64*>(scr);
+    WaveScratch64& SZ = *reinterpret_cast<WaveScratch64*>(scr + 2 * kN * kS1);
+    stage1_64(T, SE, lane, 0, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
+    stage1_64(T, SZ, lane, 3, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
+    wave_lds_fence();
+    {
+        float acc[3][R];
+        stage2_EZ64<R>(T.h3[lane], T.hd3[lane], SE, SZ, lane, acc);
+        wave_lds_fence();
+        store_rows64<R>(scr, lane, acc[0]);
+        store_rows64<R>(scr + R * kTS, lane, acc[1]);
+        store_rows64<R>(scr + 2 * R * kTS, lane, acc[2]);
+    }
+    wave_lds_fence();
+    column_pass2_64<R>(T.h3, T.hd3, scr, lane, vB, vX);
+    column_pass64<R>(T.h3, scr + R * kTS, lane, vY);
+    column_pass64<R>(T.h3, scr + 2 * R * kTS, lane, vZ);
+    wave_lds_fence();
+}
+
 struct LmState {
     double H[15];   // normal matrix, packed upper triangle
     double gr[5];   // gradient
+    double th[5];   // current point
+    double tn[5];   // trial point
+    double nll, lambda, nu, pred;
 };
 
 __device__ __forceinline__ double hget(const double (&H)[15], int p, int q) {
@@ -158,64 +194,66 @@ __device__ __forceinline__ double hget(const double (&H)[15], int p, int q) {
     return H[off + (q - p)];
 }
 
-// ---------------------------------------------------------------------------
-// Levenberg-Marquardt (theta order [A, B, x, y, z])
-// ---------------------------------------------------------------------------
 template <int R, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
-    __shared__ BlockTables64 T;
-    __shared__ WaveScratch64 Sall[WPB];
-    __shared__ LmState Lst[WPB];
+    static_assert(3 * R * kTS <= 4 * kN * kS1, "transposed blur rows must fit the scratch");
+    // one LDS struct fixes the layout: tables first (low addresses, so lane
+    // addresses plus immediate offsets reach them), then per-wave state.
+    struct Lds {
+        BlockTables64 T;
+        LmState L[WPB];
+        WaveScratch64 S[2 * WPB];
+    };
+    __shared__ Lds lds;
+    BlockTables64& T = lds.T;
     load_tables64(T, M);
     __syncthreads();
     constexpr int RPL = RoiLayout<R>::RPL;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     const int b = blockIdx.x * WPB + w;
     if (b >= A.B) return;
-    WaveScratch64& S = Sall[w];
+    float* scr = lds.S[2 * w].re;
+    LmState& L = lds.L[w];
     const int j = lane % R, g = lane / R;
-    float n[RPL];
-#pragma unroll
-    for (int t = 0; t < RPL; t++)
-        n[t] = Px<R>::valid(lane, t) ? A.pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+    const float* pix = A.pixels + (size_t)b * R * R + j;
     const float zoil = (float)A.zfocus;
-    double th[5], tn[5];
+    // Wave-uniform FP64 state lives in LDS between uses; registers are kept
+    // for the transforms.  Damping follows Nielsen's rule: on success
+    // lambda *= max(1/3, 1-(2 rho-1)^3) with rho the ratio of actual to
+    // predicted decrease; on failure lambda *= nu, nu *= 2.  A trial is
+    // accepted when the likelihood does not rise by more than the FP32
+    // synthesis noise (noise_rel * (A + npx B)).
 #pragma unroll
-    for (int p = 0; p < 5; p++) th[p] = A.init[(size_t)b * 5 + p];
-    th[0] = fmax(th[0], 1.0);
-    th[1] = fmax(th[1], 0.01);
+    for (int p = 0; p < 5; p++) L.th[p] = A.init[(size_t)b * 5 + p];
+    L.th[0] = fmax(L.th[0], 1.0);
+    L.th[1] = fmax(L.th[1], 0.01);
 #pragma unroll
-    for (int p = 0; p < 5; p++) tn[p] = th[p];
-
-    float Er[2][32], Ei[2][32];
-    float psf[RPL], sgn[RPL], dp[3][RPL];
-    float Ssum = 1.f;
-    // wave-uniform FP64 state lives in LDS between uses (keeps VGPRs for E)
-    LmState& L = Lst[w];
-    double nll = 0.0, lambda = 1e-3;
+    for (int p = 0; p < 5; p++) L.tn[p] = L.th[p];
+    L.nll = 0.0;
+    L.lambda = 1e-3;
+    L.nu = 2.0;
+    L.pred = 0.0;
+    wave_lds_fence();
     const double dxmax = (double)A.xystep;
     int it = 0, tries = 0, conv = 0, nan = 0, nfield = 0;
     bool first = true;
     while (true) {
         bool eval_now = true;
-        bool noise_floor = false;
         if (!first) {
             double Am[25], d[5];
 #pragma unroll
             for (int p = 0; p < 5; p++) {
 #pragma unroll
                 for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
-                Am[p * 5 + p] *= (1.0 + lambda);
+                Am[p * 5 + p] *= (1.0 + L.lambda);
                 d[p] = -L.gr[p];
             }
             if (!solve5(Am, d)) {
-                lambda *= 10;
+                L.lambda *= L.nu;
+                L.nu *= 2;
                 eval_now = false;
             } else {
-                // Predicted decrease of the quadratic model; below the FP32
-                // synthesis noise of the likelihood the step is taken without
-                // the likelihood comparison (the gradient, not the noisy NLL,
-                // then drives convergence).
                 double pred = 0.0;
 #pragma unroll
                 for (int p = 0; p < 5; p++) {
@@ -224,85 +262,91 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
                     for (int q = 0; q < 5; q++) hd += hget(L.H, p, q) * d[q];
                     pred -= L.gr[p] * d[p] + 0.5 * d[p] * hd;
                 }
-                noise_floor = pred < 1e-6 * (th[0] + (double)(R * R) * th[1]);
+                L.pred = pred;
+                double tn[5];
 #pragma unroll
-                for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
+                for (int p = 0; p < 5; p++) tn[p] = L.th[p] + d[p];
 #pragma unroll
                 for (int p = 2; p < 4; p++) {
-                    if (d[p] > dxmax) tn[p] = th[p] + dxmax;
-                    if (d[p] < -dxmax) tn[p] = th[p] - dxmax;
+                    if (d[p] > dxmax) tn[p] = L.th[p] + dxmax;
+                    if (d[p] < -dxmax) tn[p] = L.th[p] - dxmax;
                 }
-                if (d[4] > 0.2) tn[4] = th[4] + 0.2;
-                if (d[4] < -0.2) tn[4] = th[4] - 0.2;
+                if (d[4] > 0.2) tn[4] = L.th[4] + 0.2;
+                if (d[4] < -0.2) tn[4] = L.th[4] - 0.2;
                 tn[0] = fmax(tn[0], 1.0);
                 tn[1] = fmax(tn[1], 0.01);
                 tn[4] = fmin(fmax(tn[4], A.zmin), A.zmax);
+#pragma unroll
+                for (int p = 0; p < 5; p++) L.tn[p] = tn[p];
             }
+            wave_lds_fence();
         }
         bool accepted = false;
         if (eval_now) {
-            // field 0 at the trial point, then (if accepted) the 3 tangent fields
-            for (int f = 0; f < 4; f++) {
-                float v[RPL];
-                eval_field64<R, true>(M, T, S, lane, f, (float)tn[2], (float)tn[3], (float)tn[4],
-                                      zoil, Er, Ei, v);
-                nfield++;
-                if (f == 0) {
-                    float psfn[RPL];
-                    const float Sn = normalise<R>(lane, v, psfn);
-                    const double nn = nll64<R>(lane, psfn, n, tn[0], tn[1]);
-                    if (nn != nn) nan = 1;
-                    accepted = first || noise_floor || nn <= nll;
-                    if (!accepted) {
-                        lambda *= 10;
-                        break;
-                    }
-                    if (!first) {
-                        const double step = fmax(fabs(tn[2] - th[2]),
-                                                 fmax(fabs(tn[3] - th[3]), fabs(tn[4] - th[4])));
-                        lambda = fmax(lambda * 0.1, 1e-9);
-                        if (step < A.tol) conv = 1;
-                    }
+            // Opaque copy of the lane id: keeps the compiler from hoisting the
+            // hundreds of per-lane LDS addresses of one evaluation out of the
+            // iteration loop (they would stay live across it and spill).
+            int lane_it = lane;
+            __asm__ __volatile__("" : "+v"(lane_it));
+            float vB[RPL], vX[RPL], vY[RPL], vZ[RPL];
+            lm_eval64<R>(M, T, scr, lane_it, (float)L.tn[2], (float)L.tn[3], (float)L.tn[4], zoil,
+                         vB, vX, vY, vZ);
+            nfield += 2;
+            float psf[RPL], n[RPL];
 #pragma unroll
-                    for (int p = 0; p < 5; p++) th[p] = tn[p];
+            for (int t = 0; t < RPL; t++)
+                n[t] = Px<R>::valid(lane, t) ? pix[(g * RPL + t) * R] : 0.f;
+            const float Sn = normalise<R>(lane, vB, psf);
+            const double tA = L.tn[0], tB = L.tn[1];
+            const double nn = nll64<R>(lane, psf, n, tA, tB);
+            if (nn != nn) nan = 1;
+            const double noise = A.noise_rel * (L.th[0] + (double)(R * R) * L.th[1]);
+            accepted = first || nn <= L.nll + noise;
+            if (!accepted) {
+                L.lambda *= L.nu;
+                L.nu *= 2;
+            } else {
+                if (!first) {
+                    const double step = fmax(fabs(L.tn[2] - L.th[2]),
+                                             fmax(fabs(L.tn[3] - L.th[3]), fabs(L.tn[4] - L.th[4])));
+                    const double rho = L.pred > 0.0 ? (L.nll - nn) / L.pred : 0.0;
+                    const double t3 = 2.0 * rho - 1.0;
+                    L.lambda *= fmax(1.0 / 3.0, 1.0 - t3 * t3 * t3);
+                    L.nu = 2.0;
+                    if (step < A.tol) conv = 1;
+                }
 #pragma unroll
-                    for (int t = 0; t < RPL; t++) {
-                        psf[t] = psfn[t];
-                        sgn[t] = v[t] < 0.f ? -1.f : 1.f;
-                    }
-                    Ssum = Sn;
-                    nll = nn;
-                } else {
-                    // d psf = (sgn dv - psf * sum(sgn dv)) / S
-                    float sv = 0.f;
+                for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
+                L.nll = nn;
+                // d psf = (sgn dv - psf * sum(sgn dv)) / S for the three tangents
+                float sx = 0.f, sy = 0.f, sz = 0.f;
 #pragma unroll
-                    for (int t = 0; t < RPL; t++) {
-                        v[t] *= sgn[t];
-                        if (Px<R>::valid(lane, t)) sv += v[t];
-                    }
-                    sv = wave_sum(sv);
-                    const float inv = 1.f / Ssum;
-#pragma unroll
-                    for (int t = 0; t < RPL; t++) {
-                        const float d = (v[t] - psf[t] * sv) * inv;
-                        dp[0][t] = f == 1 ? d : dp[0][t];
-                        dp[1][t] = f == 2 ? d : dp[1][t];
-                        dp[2][t] = f == 3 ? d : dp[2][t];
+                for (int t = 0; t < RPL; t++) {
+                    const float sg = vB[t] < 0.f ? -1.f : 1.f;
+                    vX[t] *= sg;
+                    vY[t] *= sg;
+                    vZ[t] *= sg;
+                    if (Px<R>::valid(lane, t)) {
+                        sx += vX[t];
+                        sy += vY[t];
+                        sz += vZ[t];
                     }
                 }
-            }
-            if (accepted) {
-                // gradient and normal matrix sum J J^T / mu (Fisher form)
+                sx = wave_sum(sx);
+                sy = wave_sum(sy);
+                sz = wave_sum(sz);
+                const float inv = 1.f / Sn;
                 double acc[20];
 #pragma unroll
                 for (int k = 0; k < 20; k++) acc[k] = 0.0;
-                const double Av = th[0], Bg = th[1];
 #pragma unroll
                 for (int t = 0; t < RPL; t++) {
                     if (!Px<R>::valid(lane, t)) continue;
-                    const double mu = (double)psf[t] * Av + Bg;
-                    const double J[5] = {(double)psf[t], 1.0, Av * dp[0][t], Av * dp[1][t],
-                                         Av * dp[2][t]};
+                    const double mu = (double)psf[t] * tA + tB;
+                    const double J[5] = {(double)psf[t], 1.0,
+                                         tA * (double)((vX[t] - psf[t] * sx) * inv),
+                                         tA * (double)((vY[t] - psf[t] * sy) * inv),
+                                         tA * (double)((vZ[t] - psf[t] * sz) * inv)};
                     const double wg = 1.0 - (double)n[t] / mu, wh = 1.0 / mu;
                     int k = 5;
 #pragma unroll
@@ -318,8 +362,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
                 for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
 #pragma unroll
                 for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
-                wave_lds_fence();
             }
+            wave_lds_fence();
         }
         if (first) {
             first = false;
@@ -349,8 +393,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
     }
     if (lane == 0) {
 #pragma unroll
-        for (int p = 0; p < 5; p++) A.theta[(size_t)b * 5 + p] = th[p];
-        A.nll[b] = nll;
+        for (int p = 0; p < 5; p++) A.theta[(size_t)b * 5 + p] = L.th[p];
+        A.nll[b] = L.nll;
 #pragma unroll
         for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = cr[c];
         A.iters[b] = it;
@@ -472,9 +516,9 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
 
     while (st != kRsDone) {
         if (want) {
-            float Er[2][32], Ei[2][32], v[RPL];
-            eval_field64<R, false>(M, T, S, lane, 0, (float)req[0], (float)req[1], (float)req[2],
-                                   zoil, Er, Ei, v);
+            float v[RPL];
+            eval_E64<R>(M, T, S, lane, T.h3[lane], (float)req[0], (float)req[1], (float)req[2],
+                        zoil, v);
             normalise<R>(lane, v, tmp);
             want = false;
             nfield++;

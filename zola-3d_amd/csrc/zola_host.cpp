@@ -107,6 +107,19 @@ static void compute_h1(zola_ctx* c) {
         for (int k = 0; k < s2; k++) acc += G[k] * std::cos(2 * PI * (double)k * d / s2);
         c->model.h1[d] = (float)(acc / s2);
     }
+    // Spectral derivative of the blurred image along one axis, per um of
+    // emitter shift.  Shifting the emitter by x moves the (band-limited,
+    // |k| <= 31) intensity by x/dx samples, so d(blur)/dx = (1/dx) hd (*) I with
+    // hd = IDFT(i 2 pi k_s / s2 . G1); the Nyquist bin carries no intensity.
+    for (int d = 0; d < s2; d++) {
+        double acc = 0;
+        for (int k = 0; k < s2; k++) {
+            int ks = k < s2 / 2 ? k : k - s2;
+            if (k == s2 / 2) ks = 0;
+            acc -= G[k] * (2 * PI * ks / s2) * std::sin(2 * PI * (double)k * d / s2);
+        }
+        c->model.hd[d] = (float)(acc / s2 / c->o.xystep);
+    }
 }
 
 static int upload_tables(zola_ctx* c) {
@@ -328,6 +341,7 @@ int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const flo
     A.zfocus = cfg->zfocus;
     A.thr = cfg->photon_threshold;
     A.tol = cfg->tol;
+    A.noise_rel = 2e-7;
     A.pixels = pixels;
     A.init = init;
     A.theta = theta;

@@ -1,0 +1,9 @@
+// Kernel instantiations for ROI side 31.
+#include "zola_kernels_impl.hpp"
+
+namespace zola {
+template hipError_t launch_psf_forward_r<31>(const Model64&, int, const float*, float*, hipStream_t);
+template hipError_t launch_fit_r<31>(const Model64&, int, const FitArgs&, hipStream_t);
+template hipError_t launch_init_r<31>(int, const float*, int, const float*, double, double, double*,
+                                      hipStream_t);
+}  // namespace zola

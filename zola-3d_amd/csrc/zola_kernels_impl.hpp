@@ -1,8 +1,8 @@
-// zola_kernels.hip -- gfx950 kernels of the ZOLA-3D localization hot path and
-// their host launchers (called from zola_host.cpp).
+#pragma once
+// zola_kernels_impl.hpp -- gfx950 kernels of the ZOLA-3D localization hot path and
+// their launch templates; instantiated per ROI size in zola_inst_r*.hip.
 #include <hip/hip_runtime.h>
 
-#include "zola_kernels.h"
 #include "zola_psf64.hpp"
 
 namespace zola {
@@ -28,8 +28,7 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_forward64(Model64 M, int B,
     stage1_64(T, S[w], lane, 0, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
     wave_lds_fence();
     float acc[R];
-    float Er[2][32], Ei[2][32];
-    stage2_E64<R, false>(M, S[w], lane, acc, Er, Ei);
+    stage2_E64<R>(T.h3[lane], S[w], lane, acc);
     float o[RPL];
     column_blur64<R>(T, S[w], lane, acc, o);
     const int j = lane % R, g = lane / R;
@@ -52,7 +51,7 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_forward64(Model64 M, int B,
 constexpr int kWPB = 8;
 
 template <int R>
-static hipError_t launch_psf_forward_r(const Model64& M, int B, const float* params, float* out,
+hipError_t launch_psf_forward_r(const Model64& M, int B, const float* params, float* out,
                                        hipStream_t st) {
     dim3 grid((B + kWPB - 1) / kWPB), block(64 * kWPB);
     hipLaunchKernelGGL((k_psf_forward64<R, kWPB>), grid, block, 0, st, M, B, params, out);
@@ -61,27 +60,6 @@ static hipError_t launch_psf_forward_r(const Model64& M, int B, const float* par
 
 }  // namespace zola
 
-using namespace zola;
-
-#define ZOLA_R_DISPATCH(R, CALL) \
-    switch (R) {                 \
-        case 17: return CALL(17); \
-        case 21: return CALL(21); \
-        case 31: return CALL(31); \
-        case 41: return CALL(41); \
-        default: return hipErrorInvalidValue; \
-    }
-
-bool zola_kernels_support_roi(int R) {
-    return R == 17 || R == 21 || R == 31 || R == 41;
-}
-
-hipError_t zola_launch_psf_forward64(const Model64& M, int R, int B, const float* params,
-                                     float* out, hipStream_t st) {
-#define CALL(RR) launch_psf_forward_r<RR>(M, B, params, out, st)
-    ZOLA_R_DISPATCH(R, CALL)
-#undef CALL
-}
 
 // ---------------------------------------------------------------------------
 // Fits and their initial values
@@ -177,7 +155,7 @@ constexpr int kWPB_LM = 4;
 constexpr int kWPB_REF = 4;
 
 template <int R>
-static hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st) {
+hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st) {
     if (mode == 1) {
         dim3 grid((A.B + kWPB_LM - 1) / kWPB_LM), block(64 * kWPB_LM);
         hipLaunchKernelGGL((k_fit_lm64<R, kWPB_LM>), grid, block, 0, st, M, A);
@@ -189,7 +167,7 @@ static hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hip
 }
 
 template <int R>
-static hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
+hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
                                 double zstep_t, double* init, hipStream_t st) {
     constexpr int W = 4;
     dim3 grid((B + W - 1) / W), block(64 * W);
@@ -199,16 +177,3 @@ static hipError_t launch_init_r(int B, const float* pixels, int nZ, const float*
 }
 
 }  // namespace zola
-
-hipError_t zola_launch_fit64(const Model64& M, int R, int mode, const FitArgs& A, hipStream_t st) {
-#define CALL(RR) launch_fit_r<RR>(M, mode, A, st)
-    ZOLA_R_DISPATCH(R, CALL)
-#undef CALL
-}
-
-hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,
-                                  double zmin_t, double zstep_t, double* init, hipStream_t st) {
-#define CALL(RR) launch_init_r<RR>(B, pixels, nZ, tmpl, zmin_t, zstep_t, init, st)
-    ZOLA_R_DISPATCH(R, CALL)
-#undef CALL
-}

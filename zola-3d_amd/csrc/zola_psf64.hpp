@@ -41,6 +41,7 @@ struct alignas(16) BlockTables64 {
     float kz[kW * kW];     // kz - kz0 (rad/um)
     float kzo[kW * kW];    // kzoil - kzoil0 (rad/um)
     float h3[192];         // blur kernel h1, repeated three times
+    float hd3[192];        // derivative blur kernel hd (includes 1/dx), repeated
 };
 
 struct alignas(16) WaveScratch64 {
@@ -56,6 +57,7 @@ struct Model64 {
     const float* kzo;
     float kappa;       // 2 pi / (dx N): kx = kappa (n_u + 1/2)
     float h1[64];      // blur kernel over signed offset d (index d & 63)
+    float hd[64];      // d/dt of the blur kernel (spectral), divided by dx
 };
 
 template <int R>
@@ -71,7 +73,10 @@ __device__ __forceinline__ void load_tables64(BlockTables64& T, const Model64& M
         T.kz[t] = M.kz[t];
         T.kzo[t] = M.kzo[t];
     }
-    for (int t = threadIdx.x; t < 192; t += blockDim.x) T.h3[t] = M.h1[t & 63];
+    for (int t = threadIdx.x; t < 192; t += blockDim.x) {
+        T.h3[t] = M.h1[t & 63];
+        T.hd3[t] = M.hd[t & 63];
+    }
 }
 
 // Signed frequency of FFT input slot j (j < 16: j, else j - 32).
@@ -136,54 +141,52 @@ __device__ __forceinline__ void stage2_half64(const WaveScratch64& S, int lane, 
     fft_dif<32, +1>(re, im);
 }
 
-// Row blur of one image value I at column b of this lane's row into the R
-// accumulators: T(a, j) += h1[(j - R/2 - b) & 63] * I.
-template <int R, int B>
-__device__ __forceinline__ void row_blur_add(const Model64& M, float I, float (&acc)[R]) {
+// Row blur of one half of a grid row.  The lane owns row a; img[m] is the
+// image at column b = 2m + HP.  acc[j] += h[(j - R/2 - b) & 63] * img[m].
+// The kernel is held as one VGPR (lane i holds h[i]) and broadcast one
+// coefficient at a time with readlane, grouping the FMAs by coefficient so
+// that only one SGPR per kernel is live (64 coefficients do not fit the SGPR
+// file once the compiler hoists them).
+template <int R, int HP, int NK>
+__device__ __forceinline__ void row_blur_half(const float (&hreg)[NK], const float (&img)[NK][32],
+                                              float (&acc)[NK][R]) {
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        float sv[NK];
 #pragma unroll
-    for (int j = 0; j < R; j++) acc[j] += M.h1[(j - R / 2 - B) & 63] * I;
-}
-
-// Stage 2 of the E field: fills the row-blur accumulators, optionally keeps E.
-template <int R, bool KEEP>
-__device__ __forceinline__ void stage2_E64(const Model64& M, const WaveScratch64& S, int lane,
-                                           float (&acc)[R], float (&Er)[2][32],
-                                           float (&Ei)[2][32]) {
-#pragma unroll
-    for (int j = 0; j < R; j++) acc[j] = 0.f;
-    static_for<0, 2>([&](auto hpc) {
-        constexpr int HP = decltype(hpc)::value;
-        float re[32], im[32];
-        stage2_half64<HP>(S, lane, re, im);
+        for (int q = 0; q < NK; q++)
+            sv[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hreg[q]), c));
         static_for<0, 32>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            const float er = re[fidx<32>(m)], ei = im[fidx<32>(m)];
-            if constexpr (KEEP) {
-                Er[HP][m] = er;
-                Ei[HP][m] = ei;
+            constexpr int jj = (c + R / 2 + 2 * m + HP) & 63;
+            if constexpr (jj < R) {
+#pragma unroll
+                for (int q = 0; q < NK; q++) acc[q][jj] += sv[q] * img[q][m];
             }
-            row_blur_add<R, 2 * m + HP>(M, er * er + ei * ei, acc);
         });
+        __builtin_amdgcn_sched_barrier(0);
     });
 }
 
-// Stage 2 of a tangent field: image 2 Re(conj(E) dE).
+// Stage 2 of the E field: fills the row-blur accumulators (kernel h1 in hreg).
 template <int R>
-__device__ __forceinline__ void stage2_T64(const Model64& M, const WaveScratch64& S, int lane,
-                                           float (&acc)[R], const float (&Er)[2][32],
-                                           const float (&Ei)[2][32]) {
+__device__ __forceinline__ void stage2_E64(float h1reg, const WaveScratch64& S, int lane,
+                                           float (&acc)[R]) {
 #pragma unroll
     for (int j = 0; j < R; j++) acc[j] = 0.f;
     static_for<0, 2>([&](auto hpc) {
         constexpr int HP = decltype(hpc)::value;
         float re[32], im[32];
         stage2_half64<HP>(S, lane, re, im);
+        float img[1][32];
         static_for<0, 32>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            const float dr = re[fidx<32>(m)], di = im[fidx<32>(m)];
-            const float val = 2.f * (Er[HP][m] * dr + Ei[HP][m] * di);
-            row_blur_add<R, 2 * m + HP>(M, val, acc);
+            const float er = re[fidx<32>(m)], ei = im[fidx<32>(m)];
+            img[0][m] = er * er + ei * ei;
         });
+        const float hr[1] = {h1reg};
+        float (&a1)[1][R] = *reinterpret_cast<float (*)[1][R]>(&acc);
+        row_blur_half<R, HP, 1>(hr, img, a1);
     });
 }
 
@@ -211,6 +214,87 @@ __device__ __forceinline__ void column_blur64(const BlockTables64& T, WaveScratc
         for (int t = 0; t < RPL; t++) out[t] += T.h3[base + t - a] * tv;
     }
     wave_lds_fence();  // the scratch is reused by the next stage 1
+}
+
+// Fused stage 2 of E (from SE) and dE/dz (from SZ) for row a = lane:
+//   T0 = h1 (x) I,  T1 = hd (x) I  (row direction),  Tz = h1 (x) 2 Re(conj(E) dE/dz).
+template <int R>
+__device__ __forceinline__ void stage2_EZ64(float h1reg, float hdreg, const WaveScratch64& SE,
+                                            const WaveScratch64& SZ, int lane,
+                                            float (&acc)[3][R]) {
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int j = 0; j < R; j++) acc[q][j] = 0.f;
+    static_for<0, 2>([&](auto hpc) {
+        constexpr int HP = decltype(hpc)::value;
+        float er[32], ei[32];
+        stage2_half64<HP>(SE, lane, er, ei);
+        float zr[32], zi[32];
+        stage2_half64<HP>(SZ, lane, zr, zi);
+        float img[3][32];
+        static_for<0, 32>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            constexpr int k = fidx<32>(m);
+            img[0][m] = er[k] * er[k] + ei[k] * ei[k];
+            img[1][m] = img[0][m];
+            img[2][m] = 2.f * (er[k] * zr[k] + ei[k] * zi[k]);
+        });
+        const float hr[3] = {h1reg, hdreg, h1reg};
+        row_blur_half<R, HP, 3>(hr, img, acc);
+    });
+}
+
+// Stores a lane's row accumulators transposed at Tt[j * kTS + lane].
+template <int R>
+__device__ __forceinline__ void store_rows64(float* Tt, int lane, const float (&acc)[R]) {
+#pragma unroll
+    for (int j = 0; j < R; j++) Tt[j * kTS + lane] = acc[j];
+}
+
+// Column pass over a transposed row-blurred image Tt for this lane's pixels
+// (lane = (ROI column j, row group g)), with kernel table K3 (h3 or hd3).
+template <int R>
+__device__ __forceinline__ void column_pass64(const float* K3, const float* Tt, int lane,
+                                              float (&out)[RoiLayout<R>::RPL]) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    const int j = lane % R;
+    const int g = lane / R;
+    const int base = 64 + ((g * RPL - R / 2) & 63);
+#pragma unroll
+    for (int t = 0; t < RPL; t++) out[t] = 0.f;
+#pragma unroll
+    for (int a = 0; a < 64; a++) {
+        const float tv = Tt[j * kTS + a];
+#pragma unroll
+        for (int t = 0; t < RPL; t++) out[t] += K3[base + t - a] * tv;
+    }
+}
+
+// Two column passes sharing the T reads: out1 with K3a, out2 with K3b.
+template <int R>
+__device__ __forceinline__ void column_pass2_64(const float* K3a, const float* K3b,
+                                                const float* Tt, int lane,
+                                                float (&o1)[RoiLayout<R>::RPL],
+                                                float (&o2)[RoiLayout<R>::RPL]) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    const int j = lane % R;
+    const int g = lane / R;
+    const int base = 64 + ((g * RPL - R / 2) & 63);
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        o1[t] = 0.f;
+        o2[t] = 0.f;
+    }
+#pragma unroll
+    for (int a = 0; a < 64; a++) {
+        const float tv = Tt[j * kTS + a];
+#pragma unroll
+        for (int t = 0; t < RPL; t++) {
+            o1[t] += K3a[base + t - a] * tv;
+            o2[t] += K3b[base + t - a] * tv;
+        }
+    }
 }
 
 }  // namespace zola
