@@ -1,0 +1,295 @@
+"""Pins the FP64 oracle (oracle/zola_oracle.c) independently of the code it
+restates.  The reference ships no tests or golden vectors and cannot run here
+(Java 8 + ImageJ + jcuda), so each oracle function is checked against closed
+forms, brute-force transforms, invariants of the model or finite differences,
+chosen so that a dropped term, wrong sign, index or transposition fails."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import simdata
+import zola_hip as Z
+
+
+def brute_dft2(x, sign):
+    n1, n2 = x.shape
+    k1 = np.arange(n1)[:, None]
+    k2 = np.arange(n2)[:, None]
+    W1 = np.exp(sign * 2j * np.pi * k1 * k1.T / n1) / math.sqrt(n1)
+    W2 = np.exp(sign * 2j * np.pi * k2 * k2.T / n2) / math.sqrt(n2)
+    return W1 @ x @ W2.T
+
+
+@pytest.mark.parametrize("n1,n2", [(8, 8), (16, 8), (16, 16)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_matches_brute_force_dft(n1, n2, sign):
+    # utils/FastFourierTransform.java:680-785: unitary, forward e^{-i}, inverse e^{+i}
+    rng = np.random.default_rng(n1 * 100 + n2 + sign)
+    x = rng.normal(size=(n1, n2)) + 1j * rng.normal(size=(n1, n2))
+    re, im = O.fft2d(x.real, x.imag, sign)
+    np.testing.assert_allclose(re + 1j * im, brute_dft2(x, sign), atol=1e-12)
+
+
+def test_fft_parseval_and_numpy_ortho():
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(64, 64)) + 1j * rng.normal(size=(64, 64))
+    re, im = O.fft2d(x.real, x.imag, -1)
+    y = re + 1j * im
+    np.testing.assert_allclose(y, np.fft.fft2(x, norm="ortho"), atol=1e-11)
+    assert abs(np.sum(np.abs(y) ** 2) - np.sum(np.abs(x) ** 2)) < 1e-9 * np.sum(np.abs(x) ** 2)
+
+
+def test_zernike_order_and_closed_forms():
+    # utils/Zernike.java:26-79 ordering (n, m = -n..n step 2), 545-605 values (unnormalised)
+    m = O.Model(O.optics(), 21, np.zeros(15))
+    Zb = m.basis()
+    t = m.tables()
+    c = m.N / 2 - 0.5
+    x = (t["di"] - c) / m.radius_px()
+    y = (t["dj"] - c) / m.radius_px()
+    rho = np.hypot(x, y)
+    th = np.arctan2(y, x)
+    inside = rho < 1
+    expect = {
+        0: np.ones_like(rho),
+        1: rho * np.sin(th), 2: rho * np.cos(th),
+        3: rho ** 2 * np.sin(2 * th), 4: 2 * rho ** 2 - 1, 5: rho ** 2 * np.cos(2 * th),
+        12: 6 * rho ** 4 - 6 * rho ** 2 + 1,
+        14: rho ** 4 * np.cos(4 * th),
+    }
+    for j, v in expect.items():
+        np.testing.assert_allclose(Zb[j][inside], v[inside], atol=1e-12, err_msg=f"Z{j}")
+        assert np.all(Zb[j][~inside] == 0)
+
+
+def test_noll_index_map():
+    assert [Z.noll_to_nm(j) for j in range(1, 12)] == [
+        (0, 0), (1, 1), (1, -1), (2, 0), (2, -2), (2, 2), (3, -1), (3, 1), (3, -3), (3, 3), (4, 0)]
+    assert Z.nm_to_slot(2, -2) == 3 and Z.nm_to_slot(2, 0) == 4 and Z.nm_to_slot(2, 2) == 5
+    a = Z.coeffs_from_noll({5: 1.0, 11: 0.2}, 28)
+    assert a[3] == 1.0 and a[12] == 0.2 and np.count_nonzero(a) == 2
+
+
+@pytest.mark.parametrize("N,D", [(64, 716), (128, 2836)])
+def test_pupil_disk_counts_and_geometry(N, D):
+    # PhaseParameters.java:303-362 (disk), PSFPhaseMany.java:145-187 (k vectors)
+    m = O.Model(O.optics(size_fft=N), 41, np.zeros(3))
+    assert m.D == D
+    t = m.tables()
+    c = N / 2 - 0.5
+    dx = 0.11
+    np.testing.assert_allclose(t["kx"] * dx * N / (2 * np.pi), t["di"] - c, atol=1e-12)
+    np.testing.assert_allclose(t["ky"] * dx * N / (2 * np.pi), t["dj"] - c, atol=1e-12)
+    k2 = t["kx"] ** 2 + t["ky"] ** 2
+    np.testing.assert_allclose(t["kz"] ** 2 + k2, (2 * np.pi * 1.518 / 0.68) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(t["pupil"], 1 / math.sqrt(D))
+
+
+def test_nonpropagating_pixels_and_reset_kz():
+    # n_sample 1.33 < NA: kz = 0 with flag 0 (PSFPhaseMany.java:169-177); resetKz
+    # keeps a non-zero kz with flag 0 (PSFPhaseMany.java:431-470)
+    o = O.optics(size_fft=128, nwat=1.33)
+    t = O.Model(o, 41, np.zeros(3)).tables()
+    evan = t["kzflag"] == 0
+    assert evan.sum() > 0 and np.all(t["kz"][evan] == 0)
+    o.reset_kz = 1
+    t2 = O.Model(o, 41, np.zeros(3)).tables()
+    assert np.all(t2["kz"][evan] > 0) and np.all(t2["kzflag"][evan] == 0)
+    np.testing.assert_allclose(t2["kz"][~evan], t["kz"][~evan])
+
+
+def numpy_psf(m, x, y, zoil, zwat):
+    """Independent numpy evaluation of the model with numpy's FFT and the
+    reference index maps written out from the geometry (centre = pixel R//2)."""
+    t = m.tables()
+    N, s2, R = m.N, m.s2, m.R
+    P = np.zeros((N, N), complex)
+    q = t["kx"] * x + t["ky"] * y + t["phase"] + t["kz"] * zwat - t["kzoil"] * zoil
+    P[(t["di"] + N // 2) % N, (t["dj"] + N // 2) % N] = t["pupil"] * np.exp(1j * q)
+    E = np.fft.ifft2(P, norm="ortho")
+    I = np.abs(E) ** 2
+    off = (np.arange(s2) - s2 // 2)          # signed offsets of the blur grid
+    Ic = I[np.ix_(off % N, off % N)]
+    Ic = np.fft.ifftshift(Ic)                 # offset 0 at index 0
+    g = m.gauss()                             # indexed by frequency (k + s2/2) % s2
+    G = g[np.ix_((np.arange(s2) + s2 // 2) % s2, (np.arange(s2) + s2 // 2) % s2)]
+    Bl = np.fft.ifft2(np.fft.fft2(Ic, norm="ortho") * G, norm="ortho")
+    o = (np.arange(R) - R // 2) % s2
+    out = np.abs(Bl[np.ix_(o, o)])
+    return out / out.sum()
+
+
+@pytest.mark.parametrize("R,N", [(21, 64), (17, 64), (41, 128)])
+def test_psf_matches_independent_numpy(R, N):
+    a = Z.astigmatic_coeffs(28)
+    m = O.Model(O.optics(size_fft=N), R, a)
+    for (x, y, zo, zw) in [(0.0, 0.0, 0.0, 0.0), (0.05, -0.03, 0.0, 0.4), (-0.1, 0.08, 0.2, -0.7)]:
+        np.testing.assert_allclose(m.psf(x, y, zo, zw), numpy_psf(m, x, y, zo, zw), atol=1e-13)
+
+
+def test_psf_invariants():
+    R = 21
+    m0 = O.Model(O.optics(), R, np.zeros(15))
+    p = m0.psf(0, 0, 0, 0)
+    assert abs(p.sum() - 1) < 1e-13
+    # aberration-free, focus: symmetric under transposition and index reversal about R//2
+    np.testing.assert_allclose(p, p.T, atol=1e-14)
+    np.testing.assert_allclose(p, p[::-1, ::-1], atol=1e-14)
+    # index matched, no aberration: PSF(z) == PSF(-z)
+    np.testing.assert_allclose(m0.psf(0, 0, 0, 0.37), m0.psf(0, 0, 0, -0.37), atol=1e-14)
+    # a constant added to the phase (piston) leaves the PSF unchanged
+    a = Z.astigmatic_coeffs(15)
+    m1 = O.Model(O.optics(), R, a)
+    a2 = a.copy(); a2[0] += 1.234
+    m2 = O.Model(O.optics(), R, a2)
+    np.testing.assert_allclose(m1.psf(0.02, 0.01, 0, 0.3), m2.psf(0.02, 0.01, 0, 0.3), atol=1e-14)
+    # astigmatism flips sign with z (Noll 5 is odd in z for index-matched optics)
+    pa = m1.psf(0, 0, 0, 0.4)
+    assert not np.allclose(pa, pa.T, atol=1e-4)
+
+
+def test_psf_shift_equivariance():
+    # moving the emitter by +dx moves the image by one pixel towards lower first
+    # index (emitter at pixel R//2 - x/dx); only the ROI normalisation changes
+    big = O.Model(O.optics(), 31, Z.astigmatic_coeffs(15))
+    p0 = big.psf(0, 0, 0, 0.2)
+    p1 = big.psf(0.11, 0, 0, 0.2)
+    ratio = p1[5:25, 5:25] / p0[6:26, 5:25]
+    assert ratio.max() - ratio.min() < 1e-10 * ratio.mean()
+    py = big.psf(0, -0.11, 0, 0.2)
+    ratio = py[5:25, 5:25] / p0[5:25, 4:24]
+    assert ratio.max() - ratio.min() < 1e-10 * ratio.mean()
+
+
+def test_tiny_sigma_is_no_blur():
+    m = O.Model(O.optics(sigma=1e-6), 21, Z.astigmatic_coeffs(15))
+    t = m.tables()
+    N = m.N
+    P = np.zeros((N, N), complex)
+    q = t["kx"] * 0.03 + t["ky"] * 0.02 + t["phase"] + t["kz"] * 0.2
+    P[(t["di"] + N // 2) % N, (t["dj"] + N // 2) % N] = t["pupil"] * np.exp(1j * q)
+    I = np.abs(np.fft.ifft2(P, norm="ortho")) ** 2
+    o = (np.arange(21) - 10) % N
+    ref = I[np.ix_(o, o)]
+    # at sigma = 1e-6 px the float-rounded Gaussian (PSFPhaseMany.java:398) is exactly 1
+    np.testing.assert_allclose(m.psf(0.03, 0.02, 0, 0.2), ref / ref.sum(), atol=1e-14)
+
+
+def test_tangents_and_adjoint_match_finite_differences():
+    m = O.Model(O.optics(), 21, Z.astigmatic_coeffs(28))
+    x, y, z = 0.03, -0.02, 0.25
+    p, d = m.psf_jac(x, y, 0, z)
+    h = 1e-6
+    fd = [(m.psf(x + h, y, 0, z) - m.psf(x - h, y, 0, z)) / (2 * h),
+          (m.psf(x, y + h, 0, z) - m.psf(x, y - h, 0, z)) / (2 * h),
+          (m.psf(x, y, 0, z + h) - m.psf(x, y, 0, z - h)) / (2 * h)]
+    for k in range(3):
+        assert np.abs(d[k] - fd[k]).max() <= 1e-7 * np.abs(fd[k]).max()
+    w = np.random.default_rng(0).normal(size=(21, 21))
+    g, ga = m.psf_vjp(x, y, 0, z, w)
+    for k in range(3):
+        assert abs(g[k] - (d[k] * w).sum()) <= 1e-9 * abs(g[k]) + 1e-12
+    a = Z.astigmatic_coeffs(28)
+    for j in [3, 5, 10, 27]:
+        ap = a.copy(); ap[j] += 1e-5
+        am = a.copy(); am[j] -= 1e-5
+        m.set_coeffs(ap); fp = (m.psf(x, y, 0, z) * w).sum()
+        m.set_coeffs(am); fm = (m.psf(x, y, 0, z) * w).sum()
+        assert abs(ga[j] - (fp - fm) / 2e-5) <= 1e-6 * abs(ga[j]) + 1e-10
+    m.set_coeffs(a)
+    # zoil derivative against finite differences
+    h = 1e-6
+    fdo = ((m.psf(x, y, 0.1 + h, z) - m.psf(x, y, 0.1 - h, z)) / (2 * h) * w).sum()
+    g2, _ = m.psf_vjp(x, y, 0.1, z, w)
+    assert abs(g2[3] - fdo) <= 1e-6 * abs(fdo)
+
+
+def test_nll_closed_form_derivatives():
+    rng = np.random.default_rng(1)
+    psf = rng.uniform(0, 1, 441); psf /= psf.sum()
+    roi = rng.poisson(1000 * psf + 10).astype(float)
+    A, Bg = 1000.0, 10.0
+    mu = A * psf + Bg
+    assert abs(O.nll(psf, roi, A, Bg) - np.sum(mu - roi * np.log(mu))) < 1e-9
+    h = 1e-4
+    dA = (O.nll(psf, roi, A + h, Bg) - O.nll(psf, roi, A - h, Bg)) / (2 * h)
+    dB = (O.nll(psf, roi, A, Bg + h) - O.nll(psf, roi, A, Bg - h)) / (2 * h)
+    assert abs(dA - np.sum(psf * (1 - roi / mu))) < 1e-6
+    assert abs(dB - np.sum(1 - roi / mu)) < 1e-6
+
+
+def test_cofactor_inverse_matches_lapack():
+    rng = np.random.default_rng(2)
+    a = rng.normal(size=(5, 5)); a = a @ a.T + 5 * np.eye(5)
+    np.testing.assert_allclose(O.inverse_cofactor(a), np.linalg.inv(a), rtol=1e-10)
+
+
+def test_get_a_and_b_closed_form():
+    psf = np.zeros((5, 5)); psf[2, 2] = 1.0; psf[2, 3] = 0.5; psf[0, 0] = 0.05
+    patch = np.full((5, 5), 10.0); patch[2, 2] = 110; patch[2, 3] = 60; patch[0, 0] = 15
+    A, B, mx = O.get_a_and_b(patch, psf, 0.1)
+    # background = mean over psf < 0.1*max (23 pixels incl. the 0.05 one)
+    assert B == pytest.approx((22 * 10 + 15) / 23)
+    assert A == pytest.approx(110 + 60 - 2 * B)
+    assert mx == 110
+
+
+@pytest.mark.parametrize("fit", ["reference", "lm"])
+def test_noise_free_fit_recovers_truth(fit):
+    m = O.Model(O.optics(), 21, Z.astigmatic_coeffs(28))
+    th = np.array([3000.0, 20.0, 0.04, -0.06, 0.35])
+    roi = th[0] * m.psf(th[2], th[3], 0, th[4]) + th[1]
+    start = np.array([2500.0, 15.0, 0.0, 0.0, 0.3])
+    if fit == "lm":
+        r = m.lm_fit(roi, start, maxit=200, tol=1e-10)
+        np.testing.assert_allclose(r["theta"][2:], th[2:], atol=1e-8)
+        np.testing.assert_allclose(r["theta"][:2], th[:2], rtol=1e-7)
+    else:
+        r = m.localize(roi, start, iterMax=50)
+        assert r["ok"] == 1
+        # the reference stops when a sweep moves (x, y, z) by less than 0.1 nm
+        assert np.abs(r["theta"][2:] - th[2:]).max() < 1e-3
+
+
+def test_fisher_crlb_of_noise_free_model():
+    # the FD Fisher of the reference (with its A-derivative quirk) and the
+    # analytic Fisher agree on the x, y, z block
+    m = O.Model(O.optics(), 21, Z.astigmatic_coeffs(28))
+    th = np.array([3000.0, 20.0, 0.04, -0.06, 0.35])
+    F = m.fisher(th)
+    crlb = np.sqrt(np.diag(np.linalg.inv(F)))[2:]
+    roi = th[0] * m.psf(th[2], th[3], 0, th[4]) + th[1]
+    r = m.lm_fit(roi, th, maxit=5, tol=1e-12)
+    np.testing.assert_allclose(r["crlb"], crlb, rtol=1e-9)
+    assert np.all(crlb > 0.001) and np.all(crlb < 0.1)
+
+
+def test_detector_finds_planted_template():
+    m = O.Model(O.optics(size_fft=64), 21, Z.astigmatic_coeffs(28))
+    det = O.Detector(m, 48, 48, -0.5, 0.5, 0.1, thr=0.1)
+    tm = det.templates()
+    r0, i0, j0 = 6, 20, 27
+    img = np.full((48, 48), 5.0)
+    img[i0 - 10:i0 + 11, j0 - 10:j0 + 11] += 2000 * tm[r0]
+    c = det.run(img)
+    assert len(c) >= 1
+    assert tuple(int(v) for v in c[0][1:4]) == (r0, i0, j0)
+    assert c[0][0] > 0.99
+    assert np.all(np.diff(c[:, 0]) <= 0)   # sorted by score, descending
+
+
+def test_detector_tie_order_is_stable():
+    # two identical emitters: equal scores keep the scan order (r, then j, then i)
+    m = O.Model(O.optics(size_fft=64), 21, np.zeros(15))
+    det = O.Detector(m, 64, 40, -0.2, 0.2, 0.1, thr=0.1)
+    tm = det.templates()
+    img = np.full((64, 40), 3.0)
+    for (i, j) in [(15, 20), (48, 20)]:
+        img[i - 10:i + 11, j - 10:j + 11] += 1000 * tm[2]
+    c = det.run(img)
+    top = [(int(v[1]), int(v[2]), int(v[3])) for v in c[:2]]
+    assert {t[1:] for t in top} == {(15, 20), (48, 20)}
+    if c[0][0] == c[1][0]:
+        assert top[0][1] < top[1][1]

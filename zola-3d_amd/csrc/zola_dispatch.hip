@@ -2,7 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include "zola_kernels.h"
-#include "zola_fit64.hpp"
+#include "zola_adjoint64.hpp"
 
 namespace zola {
 template <int R>
@@ -11,8 +11,35 @@ hipError_t launch_psf_forward_r(const Model64& M, int B, const float* params, fl
 template <int R>
 hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st);
 template <int R>
+hipError_t launch_adjoint_r(const Model64& M, const AdjArgs& A, hipStream_t st);
+template <int R>
 hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
                          double zstep_t, double* init, hipStream_t st);
+
+// da_j += sum_b sum_p Z_j(p) qbar[b][p] (FP64), one block per coefficient.
+__global__ __launch_bounds__(256) void k_project_zernike(int B, int J, const double* __restrict__ Zw,
+                                                         const float* __restrict__ qbar,
+                                                         double* __restrict__ da) {
+    const int jc = blockIdx.x;
+    if (jc >= J) return;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < 1024; p += blockDim.x) {
+        const double zv = Zw[(size_t)jc * 1024 + p];
+        if (zv == 0.0) continue;
+        double s = 0.0;
+        for (int b = 0; b < B; b++) s += (double)qbar[(size_t)b * 1024 + p];
+        acc += zv * s;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) da[jc] += red[0];
+}
+
 }  // namespace zola
 
 using namespace zola;
@@ -49,4 +76,17 @@ hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, con
 #define CALL(RR) launch_init_r<RR>(B, pixels, nZ, tmpl, zmin_t, zstep_t, init, st)
     ZOLA_R_DISPATCH(R, CALL)
 #undef CALL
+}
+
+hipError_t zola_launch_adjoint64(const Model64& M, int R, const AdjArgs& A, hipStream_t st) {
+#define CALL(RR) launch_adjoint_r<RR>(M, A, st)
+    ZOLA_R_DISPATCH(R, CALL)
+#undef CALL
+}
+
+hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* da,
+                                       hipStream_t st) {
+    if (J <= 0 || B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_project_zernike, dim3(J), dim3(256), 0, st, B, J, Zw, qbar, da);
+    return hipGetLastError();
 }

@@ -18,7 +18,7 @@
 
 #include "../../include/zola_hip.h"
 #include "zola_kernels.h"
-#include "zola_fit64.hpp"
+#include "zola_adjoint64.hpp"
 
 namespace {
 
@@ -64,6 +64,9 @@ struct zola_ctx {
     size_t stage_bytes = 0;
     float* d_tmpl = nullptr;      // template stack for zola_fit_init_dev
     size_t tmpl_bytes = 0;
+    double* d_Zw = nullptr;       // Zernike basis on the 32x32 window, J x 1024 (FP64)
+    float* d_qbar = nullptr;      // per-PSF dL/dphase for the adjoint, grow-only
+    size_t qbar_bytes = 0;
 };
 
 static int set_err(zola_ctx* c, int code, const char* fmt, ...) {
@@ -251,6 +254,14 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
     c->model.kappa = (float)c->kappa;
     int rc = upload_tables(c);
     if (rc) return rc;
+    if (J > 0) {
+        std::vector<double> zw((size_t)J * 1024, 0.0);
+        for (int jz = 0; jz < J; jz++)
+            for (int p = 0; p < D; p++)
+                zw[(size_t)jz * 1024 + c->wu[p] * 32 + c->wv[p]] = c->Z[(size_t)jz * D + p];
+        HIP_TRY(c, hipMalloc((void**)&c->d_Zw, zw.size() * sizeof(double)));
+        HIP_TRY(c, hipMemcpy(c->d_Zw, zw.data(), zw.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     c->err.clear();
     return ZOLA_OK;
 }
@@ -260,6 +271,8 @@ int zola_ctx_destroy(zola_ctx* c) {
     if (c->d_tab) (void)hipFree(c->d_tab);
     if (c->d_stage) (void)hipFree(c->d_stage);
     if (c->d_tmpl) (void)hipFree(c->d_tmpl);
+    if (c->d_Zw) (void)hipFree(c->d_Zw);
+    if (c->d_qbar) (void)hipFree(c->d_qbar);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ZOLA_OK;
@@ -316,6 +329,62 @@ int zola_psf_forward(zola_ctx* c, int B, const float* params, float* out) {
     rc = zola_psf_forward_dev(c, B, dp, dout);
     if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
+}
+
+int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* w,
+                          float* dparams, double* da) {
+    if (!c) return ZOLA_EINVAL;
+    if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
+    if (B == 0) return ZOLA_OK;
+    float* qbar = nullptr;
+    if (da && c->J > 0) {
+        const size_t need = (size_t)B * 1024 * sizeof(float);
+        if (need > c->qbar_bytes) {
+            if (c->d_qbar) HIP_TRY(c, hipFree(c->d_qbar));
+            c->d_qbar = nullptr;
+            c->qbar_bytes = 0;
+            HIP_TRY(c, hipMalloc((void**)&c->d_qbar, need));
+            c->qbar_bytes = need;
+        }
+        qbar = c->d_qbar;
+    }
+    zola::AdjArgs A{};
+    A.B = B;
+    A.params = params;
+    A.w = w;
+    A.dparams = dparams;
+    A.qbar = qbar;
+    HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    return ZOLA_OK;
+}
+
+int zola_psf_backward(zola_ctx* c, int B, const float* params, const float* w, float* dparams,
+                      double* da) {
+    if (!c) return ZOLA_EINVAL;
+    if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
+    if (B == 0) return ZOLA_OK;
+    const size_t R2 = (size_t)c->R * c->R;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t bp = (size_t)B * 4 * sizeof(float), bw = B * R2 * sizeof(float);
+    const size_t bd = (size_t)B * 4 * sizeof(float), ba = (size_t)(c->J > 0 ? c->J : 1) * sizeof(double);
+    int rc = stage(c, al(bp) + al(bw) + al(bd) + al(ba));
+    if (rc) return rc;
+    char* base = (char*)c->d_stage;
+    float* dpar = (float*)base; base += al(bp);
+    float* dw = (float*)base; base += al(bw);
+    float* ddp = (float*)base; base += al(bd);
+    double* dda = (double*)base;
+    HIP_TRY(c, hipMemcpyAsync(dpar, params, bp, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dw, w, bw, hipMemcpyHostToDevice, c->stream));
+    if (da) HIP_TRY(c, hipMemsetAsync(dda, 0, ba, c->stream));
+    rc = zola_psf_backward_dev(c, B, dpar, dw, ddp, da ? dda : nullptr);
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(dparams, ddp, bd, hipMemcpyDeviceToHost, c->stream));
+    if (da && c->J > 0)
+        HIP_TRY(c, hipMemcpyAsync(da, dda, (size_t)c->J * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ZOLA_OK;
 }

@@ -5,6 +5,7 @@
 namespace zola {
 struct Model64;
 struct FitArgs;
+struct AdjArgs;
 }
 
 bool zola_kernels_support_roi(int R);
@@ -17,3 +18,9 @@ hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, const zola
 
 hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,
                                   double zmin_t, double zstep_t, double* init, hipStream_t st);
+
+hipError_t zola_launch_adjoint64(const zola::Model64& M, int R, const zola::AdjArgs& A,
+                                 hipStream_t st);
+
+hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar,
+                                       double* da, hipStream_t st);

@@ -65,6 +65,7 @@ hipError_t launch_psf_forward_r(const Model64& M, int B, const float* params, fl
 // Fits and their initial values
 // ---------------------------------------------------------------------------
 #include "zola_fit64.hpp"
+#include "zola_adjoint64.hpp"
 
 namespace zola {
 
@@ -173,6 +174,14 @@ hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, 
     dim3 grid((B + W - 1) / W), block(64 * W);
     hipLaunchKernelGGL((k_fit_init64<R, W>), grid, block, 0, st, B, pixels, nZ, tmpl, zmin_t,
                        zstep_t, init);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_adjoint_r(const Model64& M, const AdjArgs& A, hipStream_t st) {
+    constexpr int W = 4;
+    dim3 grid((A.B + W - 1) / W), block(64 * W);
+    hipLaunchKernelGGL((k_psf_adjoint64<R, W>), grid, block, 0, st, M, A);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@ library's message, as the reference logs cudaResult failures.  The library
 must be built (zola-3d_amd/build.py); there is no CPU fallback.
 """
 import ctypes as C
+import importlib.util
 import os
 
 import numpy as np
@@ -52,6 +53,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ZolaError("libzola_hip.so is not built: run zola-3d_amd/build.py")
+        # A process holds one HIP runtime (soname libamdhip64.so.7): when PyTorch
+        # is installed, load it first so that its bundled runtime is the one
+        # both use and torch streams/tensors can be passed to the *_dev calls.
+        if importlib.util.find_spec("torch") is not None:
+            import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.zola_ctx_create.argtypes = [C.POINTER(Optics), C.c_int, vp, C.c_int, C.POINTER(vp)]
