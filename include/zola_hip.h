@@ -136,6 +136,32 @@ int  zola_fit_batch_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
 int  zola_fit_init_dev(zola_ctx* ctx, int B, const float* pixels, int nZ, double zmin_t,
                        double zstep_t, double zfocus, double* init);
 
+/* The context's HIP stream (hipStream_t as void*). */
+int  zola_get_stream(const zola_ctx* ctx, void** stream);
+
+/* One calibration evaluation over B bead slices (device pointers): the model
+ * of process/cpu/ZernikePhaseRetrieval.java:1842-1879, mu = amp*psf + bg
+ * (clamped at 1e-8), psf at params B x 4 [x, y, z, zoil] (calibration slices use
+ * zoil = slice defocus + bead z0, z = 0).  Writes nll B, dparams B x 4 (dNLL/dx,
+ * dNLL/dy, dNLL/dz, dNLL/dzoil), damp B, dbg B and, when da is not NULL,
+ * accumulates dNLL/da_j into da (J doubles).  One forward + adjoint pass replaces
+ * the per-coefficient finite differences of
+ * process/gpu/ZernikePhaseRetrieval_.java:1108-1187 and getLikelihood 2838-2862. */
+int  zola_calib_eval_dev(zola_ctx* ctx, int B, const float* params, const float* pixels,
+                         const double* amp, const double* bg, double* nll, float* dparams,
+                         double* damp, double* dbg, double* da);
+
+/* PSF calibration from nb bead z-stacks of S slices (host arrays), replacing
+ * ZernikePhaseRetrieval_.run (process/gpu/ZernikePhaseRetrieval_.java:362-474).
+ * pixels nb*S*R*R photons, dz S slice defocus (um), bead nb*5 in/out
+ * [x, y, z0, A, bg], coeffs J in/out (context coefficients are updated),
+ * fit_mask J (1 = fitted; NULL = all).  FP64 L-BFGS on exact adjoint gradients;
+ * stops at max_iter, when the largest scaled gradient component is below gtol, or
+ * when the relative NLL decrease falls below 1e-13. */
+int  zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const double* dz,
+                    double* bead, double* coeffs, const int* fit_mask, int max_iter,
+                    double gtol, double* out_nll, int* out_iters, int* out_evals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,9 +19,16 @@ namespace zola {
 struct AdjArgs {
     int B;
     const float* params;   // B x 4 [x, y, z, zoil]
-    const float* w;        // B x R x R
+    const float* w;        // B x R x R (adjoint form: dL/dpsf)
     float* dparams;        // B x 4
     float* qbar;           // B x 1024 (window layout), may be null
+    // calibration form: w = dNLL/dpsf is formed from the data
+    const float* pixels;   // B x R x R photons
+    const double* amp;     // B photons per PSF
+    const double* bg;      // B background per pixel
+    double* nll;           // B  sum(mu - n ln mu), mu = amp psf + bg (clamped at 1e-8)
+    double* damp;          // B  dNLL/damp = sum psf (1 - n/mu)
+    double* dbg;           // B  dNLL/dbg  = sum (1 - n/mu)
 };
 
 // Stage 2 of E keeping E (row a = lane) plus the forward row blur.
@@ -48,7 +55,7 @@ __device__ __forceinline__ void stage2_E_keep64(float h1reg, const WaveScratch64
     });
 }
 
-template <int R, int WPB>
+template <int R, int WPB, bool CALIB>
 __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A) {
     struct Lds {
         BlockTables64 T;
@@ -85,14 +92,50 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
         const bool ok = g < G && g * RPL + t < R;
-        wv[t] = ok ? A.w[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
-        const float mg = fabsf(bl[t]);
-        if (ok) {
-            ssum += mg;
-            wm += wv[t] * mg;
-        }
+        if (ok) ssum += fabsf(bl[t]);
     }
     ssum = wave_sum(ssum);
+    if constexpr (CALIB) {
+        // ZernikePhaseRetrieval.Model.computeLikelihood (1859-1867): mu = psf A + bg,
+        // clamped at 1e-8; w = dNLL/dpsf = A (1 - n/mu)
+        const double av = A.amp[b], bv = A.bg[b];
+        double nl = 0.0, da = 0.0, db = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; t++) {
+            const bool ok = g < G && g * RPL + t < R;
+            wv[t] = 0.f;
+            if (ok) {
+                const double ps = (double)(fabsf(bl[t]) / ssum);
+                const double n = A.pixels[(size_t)b * R * R + (g * RPL + t) * R + j];
+                double mu = ps * av + bv;
+                if (mu <= 0) mu = 1e-8;
+                nl += mu - n * log(mu);
+                const double r = 1.0 - n / mu;
+                da += ps * r;
+                db += r;
+                wv[t] = (float)(av * r);
+            }
+        }
+        nl = wave_sum(nl);
+        da = wave_sum(da);
+        db = wave_sum(db);
+        if (lane == 0) {
+            A.nll[b] = nl;
+            A.damp[b] = da;
+            A.dbg[b] = db;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < RPL; t++) {
+            const bool ok = g < G && g * RPL + t < R;
+            wv[t] = ok ? A.w[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        const bool ok = g < G && g * RPL + t < R;
+        if (ok) wm += wv[t] * fabsf(bl[t]);
+    }
     wm = wave_sum(wm);
     const float inv = 1.f / ssum, c2 = wm * inv * inv;
 

@@ -389,6 +389,48 @@ int zola_psf_backward(zola_ctx* c, int B, const float* params, const float* w, f
     return ZOLA_OK;
 }
 
+int zola_get_stream(const zola_ctx* c, void** stream) {
+    if (!c || !stream) return ZOLA_EINVAL;
+    *stream = (void*)c->stream;
+    return ZOLA_OK;
+}
+
+int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pixels,
+                        const double* amp, const double* bg, double* nll, float* dparams,
+                        double* damp, double* dbg, double* da) {
+    if (!c) return ZOLA_EINVAL;
+    if (B < 0) return set_err(c, ZOLA_EINVAL, "bad batch");
+    if (B == 0) return ZOLA_OK;
+    if (!params || !pixels || !amp || !bg || !nll || !dparams || !damp || !dbg)
+        return set_err(c, ZOLA_EINVAL, "null buffer");
+    float* qbar = nullptr;
+    if (da && c->J > 0) {
+        const size_t need = (size_t)B * 1024 * sizeof(float);
+        if (need > c->qbar_bytes) {
+            if (c->d_qbar) HIP_TRY(c, hipFree(c->d_qbar));
+            c->d_qbar = nullptr;
+            c->qbar_bytes = 0;
+            HIP_TRY(c, hipMalloc((void**)&c->d_qbar, need));
+            c->qbar_bytes = need;
+        }
+        qbar = c->d_qbar;
+    }
+    zola::AdjArgs A{};
+    A.B = B;
+    A.params = params;
+    A.dparams = dparams;
+    A.qbar = qbar;
+    A.pixels = pixels;
+    A.amp = amp;
+    A.bg = bg;
+    A.nll = nll;
+    A.damp = damp;
+    A.dbg = dbg;
+    HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    return ZOLA_OK;
+}
+
 int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                        const double* init, double* theta, double* nll, double* crlb, int* iters,
                        int* status) {

@@ -24,7 +24,8 @@ SYMBOLS = [
     "zola_ctx_create", "zola_ctx_destroy", "zola_last_error", "zola_ctx_info",
     "zola_set_coefficients", "zola_set_sigma", "zola_set_stream", "zola_psf_forward",
     "zola_psf_forward_dev", "zola_psf_backward", "zola_psf_backward_dev", "zola_fit_batch",
-    "zola_fit_batch_dev", "zola_fit_init_dev",
+    "zola_fit_batch_dev", "zola_fit_init_dev", "zola_get_stream", "zola_calib_eval_dev",
+    "zola_calib_fit",
 ]
 
 
@@ -77,6 +78,10 @@ def lib():
             if hasattr(L, name):
                 getattr(L, name).argtypes = [vp, C.POINTER(FitConfig), C.c_int, vp, vp, vp, vp,
                                              vp, vp, vp]
+        if hasattr(L, "zola_calib_fit"):
+            L.zola_calib_eval_dev.argtypes = [vp, C.c_int] + [vp] * 9
+            L.zola_calib_fit.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int,
+                                         C.c_double, vp, vp, vp]
         if hasattr(L, "zola_fit_init_dev"):
             L.zola_fit_init_dev.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                             C.c_double, vp]
@@ -194,6 +199,29 @@ class Context:
                                              _ptr(pixels), _ptr(init), _ptr(theta), _ptr(nll),
                                              _ptr(crlb), _ptr(iters), _ptr(status)),
                     "fit_batch_dev")
+
+    # -- calibration -------------------------------------------------------
+    def calib_eval_dev(self, params, pixels, amp, bg, nll, dparams, damp, dbg, da=None):
+        self._check(lib().zola_calib_eval_dev(self._h, params.shape[0], _ptr(params), _ptr(pixels),
+                                              _ptr(amp), _ptr(bg), _ptr(nll), _ptr(dparams),
+                                              _ptr(damp), _ptr(dbg), _ptr(da)), "calib_eval_dev")
+
+    def calib_fit(self, stacks, dz, bead, coeffs, fit_mask=None, max_iter=100, gtol=1e-6):
+        """ZernikePhaseRetrieval: stacks (nb, S, R, R), dz (S,), bead (nb, 5) [x, y, z0, A, bg]."""
+        px = np.ascontiguousarray(stacks, dtype=np.float32)
+        nb, S = px.shape[:2]
+        bd = np.ascontiguousarray(bead, dtype=np.float64).copy()
+        co = np.ascontiguousarray(coeffs, dtype=np.float64).copy()
+        mask = None if fit_mask is None else np.ascontiguousarray(fit_mask, dtype=np.int32)
+        nll = C.c_double()
+        it = C.c_int()
+        ev = C.c_int()
+        self._check(lib().zola_calib_fit(self._h, nb, S, _ptr(px),
+                                         _ptr(np.ascontiguousarray(dz, dtype=np.float64)),
+                                         _ptr(bd), _ptr(co), _ptr(mask), int(max_iter),
+                                         float(gtol), C.byref(nll), C.byref(it), C.byref(ev)),
+                    "calib_fit")
+        return dict(bead=bd, coeffs=co, nll=nll.value, iters=it.value, evals=ev.value)
 
     def fit_init_dev(self, pixels, nZ, zmin_t, zstep_t, zfocus, init):
         self._check(lib().zola_fit_init_dev(self._h, pixels.shape[0], _ptr(pixels), int(nZ),
