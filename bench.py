@@ -41,7 +41,10 @@ def parse():
     p.add_argument("--plain-batch", type=int, default=16384)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-calib", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=0, help="ROIs for the CPU baseline (0: auto)")
+    p.add_argument("--copy-probe-mb", type=int, default=0,
+                   help="also run one device copy of this size (reference for HBM counters)")
     return p.parse_args()
 
 
@@ -61,6 +64,70 @@ def synth(ctx, torch, B, seed):
     px = torch.poisson(mu, generator=g).float().contiguous()
     truth = torch.stack([A, bg, x, y, z], 1)
     return px, truth
+
+
+def calibration_bench(torch, Z, stream):
+    """PSF calibration from 5 bead z-stacks of 41 slices (50 nm, -1..+1 um), 31x31 px,
+    2e5 photons/slice, background 10, 45 Zernike modes ~ N(0, 0.15 rad) with
+    Noll 5 = 0.8 rad; all 42 non-registration coefficients plus per-bead
+    (x, y, z0, N, bg) fitted by L-BFGS on adjoint gradients from a zero start."""
+    Rc, J, nb, S = 31, 45, 5, 41
+    rng = np.random.default_rng(0x20A1D3 + 3)
+    truth = rng.normal(0.0, 0.15, J)
+    truth[:3] = 0.0
+    truth[Z.nm_to_slot(*Z.noll_to_nm(5))] = 0.8
+    dz = (np.arange(S) - S // 2) * 0.05
+    beads = np.stack([rng.uniform(-0.05, 0.05, nb), rng.uniform(-0.05, 0.05, nb),
+                      rng.uniform(-0.05, 0.05, nb), np.full(nb, 2e5), np.full(nb, 10.0)], 1)
+    gen = Z.Context(Z.optics(), Rc, truth)
+    gen.set_stream(stream.cuda_stream)
+    par = np.zeros((nb * S, 4), np.float32)
+    for b in range(nb):
+        for s_ in range(S):
+            par[b * S + s_] = [beads[b, 0], beads[b, 1], 0.0, dz[s_] + beads[b, 2]]
+    P = torch.from_numpy(par).cuda()
+    psf = torch.empty(nb * S, Rc, Rc, device="cuda")
+    gen.psf_forward_dev(P, psf)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x20A1D3 + 3)
+    stacks = torch.poisson(2e5 * psf + 10.0, generator=g).float()
+    ctx = Z.Context(Z.optics(), Rc, np.zeros(J))
+    ctx.set_stream(stream.cuda_stream)
+    # one forward+adjoint evaluation (device form), HIP events on the context stream
+    amp = torch.full((nb * S,), 2e5, dtype=torch.float64, device="cuda")
+    bg = torch.full((nb * S,), 10.0, dtype=torch.float64, device="cuda")
+    nll = torch.empty(nb * S, dtype=torch.float64, device="cuda")
+    dpar = torch.empty(nb * S, 4, device="cuda")
+    dA = torch.empty_like(nll)
+    dB = torch.empty_like(nll)
+    da = torch.zeros(J, dtype=torch.float64, device="cuda")
+    ctx.calib_eval_dev(P, stacks, amp, bg, nll, dpar, dA, dB, da)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    reps = 20
+    for _ in range(reps):
+        ctx.calib_eval_dev(P, stacks, amp, bg, nll, dpar, dA, dB, da)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eval_ms = e0.elapsed_time(e1) / reps
+    start = beads.copy()
+    start[:, :3] = 0.0
+    host = stacks.cpu().numpy().reshape(nb, S, Rc, Rc)
+    start[:, 3] = host.reshape(nb, S, -1).sum(2).mean(1) - Rc * Rc * 10.0
+    mask = np.ones(J, np.int32)
+    mask[:3] = 0
+    t0 = time.perf_counter()
+    res = ctx.calib_fit(host, dz, start, np.zeros(J), fit_mask=mask, max_iter=500, gtol=1e-3)
+    dt = time.perf_counter() - t0
+    return {
+        "workload": "5 beads x 41 slices x 31x31 px, 45 Zernike modes, L-BFGS on adjoint "
+                    "gradients (forward+adjoint per evaluation), zero start",
+        "iterations": res["iters"], "evaluations": res["evals"],
+        "iters_per_s": round(res["iters"] / dt, 2),
+        "evals_per_s": round(res["evals"] / dt, 2),
+        "eval_kernel_ms": round(eval_ms, 4),
+        "max_coeff_err_rad": round(float(np.abs(res["coeffs"] - truth)[3:].max()), 5),
+    }
 
 
 def main():
@@ -206,6 +273,18 @@ def main():
         # both forms fit the same ROIs from the same start: compare on common ROIs
         lm_rate_kernel = B / ((np.mean(fit_ms) + np.mean(init_ms)) / 1e3)
         out["batched_vs_plain"] = round(lm_rate_kernel / (Bp / ((pms + np.mean(init_ms) * Bp / B) / 1e3)), 3)
+
+    if not args.no_calib:
+        out["calibration"] = calibration_bench(torch, Z, stream)
+
+    if args.copy_probe_mb:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 2 * n * 4
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
