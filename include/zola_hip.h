@@ -162,6 +162,82 @@ int  zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const dou
                     double* bead, double* coeffs, const int* fit_mask, int max_iter,
                     double gtol, double* out_nll, int* out_iters, int* out_evals);
 
+/* ---------------------------------------------------------------------------
+ * Frame detector: normalised FFT cross-correlation of whole frames against a
+ * z-stack of PSF templates, local maxima and parabola refinement, in FP64.
+ * Replaces process/cpu/Predetection.java:63-593 (the CPU path of the plugin;
+ * process/gpu/Predetection_cross_.java:104-777 is its FP32 GPU twin).
+ * Frames are W x H floats in photons, laid out [i][ii] with i the ImageJ x
+ * column, after the plugin's rescale and clamp at 0
+ * (process/cpu/LocalizationPipeline.java:611-630). */
+typedef struct zola_detector zola_detector;
+
+/* Candidate record: 7 doubles [score, r, i, j, dz, dx, dy] as built by
+ * Predetection.getVectMaxPosition (Predetection.java:553-563): correlation
+ * score, template index r, pixel (i, j), parabola offsets (um).  Lists are
+ * ordered by score, descending, ties in the reference's scan order. */
+#define ZOLA_CAND_FIELDS 7
+
+/* Builds the template dictionary for W x H frames: nZ = ceil((zmax-zmin)/zstep)
+ * templates PSF(0, 0, zoil = zfocus, z = zmin + r*zstep) of side R (the
+ * context's), padded FFT size pow2(2W) x pow2(2H) (at most 2048 each).
+ * threshold is thresholdCrossCorrelation (LocalizationPipeline.java:59, 0.1).
+ * Replaces new Predetection(width, height, dp, minZ, maxZ, stepZ, threshold)
+ * (LocalizationPipeline.java:455). */
+int  zola_detector_create(zola_ctx* ctx, int W, int H, double zmin, double zmax, double zstep,
+                          double threshold, double zfocus, zola_detector** out);
+int  zola_detector_destroy(zola_detector* det);
+/* info[0]=nZ info[1]=FX info[2]=FY info[3]=W info[4]=H */
+int  zola_detector_info(const zola_detector* det, int* info);
+/* Template z values (Predetection.getRange), nZ doubles. */
+int  zola_detector_range(const zola_detector* det, double* range);
+/* Templates nZ x R x R, each normalised to sum 1 (Predetection.getPSFNonNormalized). */
+int  zola_detector_templates(const zola_detector* det, double* psf);
+/* Candidates of F frames (host arrays): frames F x W x H, cand F x cap x 7
+ * (frame f's list at cand + f*cap*7, at most cap records), counts F (number
+ * found, may exceed cap).  Replaces Predetection.setImage +
+ * convolveNormalizedInFourierDomain (Predetection.java:292-496). */
+int  zola_detect_frames(zola_detector* det, int F, const float* frames, int cap, double* cand,
+                        int* counts);
+/* The normalised correlation maps of one frame, nZ x W x H doubles
+ * (Predetection.resultCorrelation). */
+int  zola_detector_correlation(zola_detector* det, const float* frame, double* corr);
+
+/* ---------------------------------------------------------------------------
+ * Frame localization: detection, then the per-frame greedy walk of
+ * process/cpu/LocalizationPipeline.java:721-1073 (LocalizationThread.run) over
+ * the ordered candidates: skip candidates whose patch touches the mask, fit
+ * with getAandB + parabola initial values, accept on photons, shift, centre
+ * pixel and z range, mask the model core (accepted) or the bright patch pixels
+ * (not accepted).  Fits run batched on the GPU; decisions are replayed in the
+ * reference's order on the host. */
+typedef struct zola_accept_config {
+    double photon_threshold; /* photonThreshold (ZOLA.java:248, default 500)          */
+    double min_z, max_z;     /* accepted z: ZfocusCenter -/+ axialRange/2              */
+    double snr_min;          /* maxSNR: fit only if maxPatch/B > snr_min (1)           */
+    double model_mask_frac;  /* accepted: mask model-B > frac*max(model-B) (0.2)       */
+    double patch_mask_frac;  /* not accepted: mask patch > frac*maxPatch (0.5)          */
+    double max_shift_px;     /* accept |x|,|y| < max_shift_px * xystep (10)             */
+} zola_accept_config;
+
+/* Localization record: 12 doubles [frame, x_nm, y_nm, z_nm, A, B, score,
+ * crlb_x_nm, crlb_y_nm, crlb_z_nm, nll, candidate index] -- the fields of
+ * PLocalization as LocalizationPipeline.java:890-905 fills them (x_nm =
+ * 1000 (i*xystep - x), score = scoreCompute), plus the fit's NLL and the
+ * position of the candidate in the frame's ordered list. */
+#define ZOLA_LOC_FIELDS 12
+
+/* Localizes F frames (host arrays).  fit: mode and iteration settings of the
+ * per-candidate fit (the reference CPU pipeline: mode ZOLA_FIT_REFERENCE,
+ * max_iter 20, z clamp [min_z-0.3, max_z+0.3], photon_threshold 0 -- the
+ * pipeline never sets Localization.setMinimumPhotonNumber).  locs: cap x 12,
+ * frames in order, each frame's records in acceptance order; n_locs = total
+ * found (may exceed cap); per_frame (F ints, may be NULL); rounds (may be
+ * NULL) = number of batched fit launches. */
+int  zola_localize_frames(zola_detector* det, const zola_fit_config* fit,
+                          const zola_accept_config* acc, int F, const float* frames, int cap,
+                          double* locs, int* n_locs, int* per_frame, int* rounds);
+
 #ifdef __cplusplus
 }
 #endif

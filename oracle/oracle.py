@@ -81,6 +81,8 @@ def lib():
         L.zo_detector_run.restype = C.c_int
         L.zo_detector_run.argtypes = [vp, _dp, _dp, C.c_int, C.c_void_p]
         L.zo_calib_nll.restype = C.c_double
+        L.zo_frame_localize.argtypes = [vp, vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_double, _dp, C.c_int]
         L.zo_calib_nll.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]
         _lib = L
     return _lib
@@ -261,6 +263,15 @@ class Detector:
         n = lib().zo_detector_run(self._h, _d(image), out, cap,
                                   corr.ctypes.data if with_corr else None)
         return (out[: min(n, cap)], corr) if with_corr else out[: min(n, cap)]
+
+
+def frame_localize(model, det, image, zfocus, minZ, maxZ, iter_max=20, photon_thr=500.0,
+                   cap=10000):
+    """LocalizationThread.run on one frame: (n, 12) records."""
+    out = np.zeros((cap, 12))
+    n = lib().zo_frame_localize(model._h, det._h, _d(image), zfocus, minZ, maxZ, iter_max,
+                                photon_thr, out, cap)
+    return out[:min(n, cap)]
 
 
 def calib_nll(model, stack, dz, xs, ys, zs, Aslice, bg, zwat=0.0):

@@ -1257,6 +1257,102 @@ int zo_detector_run(zo_detector* d, const double* image, double* out, int cap, d
 }
 
 /* ------------------------------------------------------------------ */
+/* Frame walk (cpu/LocalizationPipeline.java:721-1073, 1099-1250)        */
+/* ------------------------------------------------------------------ */
+
+int zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double zfocus,
+                      double minZ, double maxZ, int iterMax, double photonThr,
+                      double* out, int cap) {
+    int W = d->W, H = d->H, R = m->R, n = R * R, h = R / 2;
+    double dx = m->o.xystep;
+    int ccap = 1 << 16;
+    double* cand = (double*)xcalloc((size_t)ccap * 7, sizeof(double));
+    int nc = zo_detector_run(d, image, cand, ccap, NULL);
+    if (nc > ccap) {
+        free(cand);
+        ccap = nc;
+        cand = (double*)xcalloc((size_t)ccap * 7, sizeof(double));
+        nc = zo_detector_run(d, image, cand, ccap, NULL);
+    }
+    unsigned char* mask = (unsigned char*)xcalloc((size_t)W * H, 1);
+    double* patch = (double*)xcalloc(n, sizeof(double));
+    double* model = (double*)xcalloc(n, sizeof(double));
+    int ptr = 0, nloc = 0;
+    for (;;) {
+        /* myGetMaxPositions (1218-1250) */
+        int k = -1;
+        for (int i = ptr; i < nc; i++) {
+            int ci = (int)floor(cand[i * 7 + 2] + 0.5), cj = (int)floor(cand[i * 7 + 3] + 0.5);
+            if (!mask[ci * H + cj] && ci > h + 1 && cj > h + 1 && ci < W - h - 1 && cj < H - h - 1) {
+                k = i;
+                ptr = i + 1;
+                break;
+            }
+        }
+        if (k < 0) break;
+        int r = (int)floor(cand[k * 7 + 1] + 0.5);
+        int ci = (int)floor(cand[k * 7 + 2] + 0.5), cj = (int)floor(cand[k * 7 + 3] + 0.5);
+        /* patch, stopping at the first masked pixel (757-770) */
+        int already = 0;
+        for (int a = 0; a < R && !already; a++)
+            for (int b = 0; b < R; b++) {
+                if (mask[(ci + a - h) * H + cj + b - h]) { already = 1; break; }
+                patch[a * R + b] = image[(ci + a - h) * H + cj + b - h];
+            }
+        if (already) continue;
+        double ab[3];
+        zo_get_a_and_b(n, patch, d->psf + (long)r * n, .1, ab);
+        int treated = 0;
+        if ((ab[2] / ab[1] > 1) && (ab[1] > 0)) {
+            double theta[5] = {ab[0], ab[1], -cand[k * 7 + 5], -cand[k * 7 + 6],
+                               d->range[r] + cand[k * 7 + 4]};
+            double crlb[3], lik;
+            int it, nanf;
+            int ok = zo_localize(m, patch, theta, zfocus, minZ - .3, maxZ + .3, iterMax, 0.0, crlb,
+                                 &lik, &it, model, &nanf);
+            if (ok) {
+                double B = theta[1], thr = 0;
+                for (int p = 0; p < n; p++)
+                    if (thr < (model[p] - B)) thr = model[p] - B;
+                thr *= .2;
+                if (theta[0] > photonThr && fabs(theta[2]) < 10 * dx && fabs(theta[3]) < 10 * dx) {
+                    int px = (int)(ci - (theta[2] / dx)), py = (int)(cj - (theta[3] / dx));
+                    if (px >= 0 && px < W && py >= 0 && py < H && !mask[px * H + py] &&
+                        theta[4] > minZ && theta[4] < maxZ) {
+                        if (nloc < cap) {
+                            double* o = out + (long)nloc * 12;
+                            o[0] = 0;
+                            o[1] = 1000 * (ci * dx - theta[2]);
+                            o[2] = 1000 * (cj * dx - theta[3]);
+                            o[3] = 1000 * theta[4];
+                            o[4] = theta[0];
+                            o[5] = theta[1];
+                            o[6] = zo_score(n, patch, model);
+                            o[7] = 1000 * crlb[0];
+                            o[8] = 1000 * crlb[1];
+                            o[9] = 1000 * crlb[2];
+                            o[10] = lik;
+                            o[11] = k;
+                        }
+                        nloc++;
+                        for (int a = 0; a < R; a++)
+                            for (int b = 0; b < R; b++)
+                                if ((model[a * R + b] - B) > thr) mask[(ci + a - h) * H + cj + b - h] = 1;
+                        treated = 1;
+                    }
+                }
+            }
+        }
+        if (!treated)
+            for (int a = 0; a < R; a++)
+                for (int b = 0; b < R; b++)
+                    if (patch[a * R + b] > ab[2] * .5) mask[(ci + a - h) * H + cj + b - h] = 1;
+    }
+    free(cand); free(mask); free(patch); free(model);
+    return nloc;
+}
+
+/* ------------------------------------------------------------------ */
 /* Calibration model (ZernikePhaseRetrieval.Model, 1842-1879)           */
 /* ------------------------------------------------------------------ */
 

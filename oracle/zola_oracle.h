@@ -19,6 +19,7 @@
  *   zo_get_a_and_b       process/cpu/LocalizationPipeline.java:1119-1162
  *   zo_score             process/cpu/LocalizationPipeline.java:1099-1115
  *   zo_detector_*        process/cpu/Predetection.java:63-593
+ *   zo_frame_localize    process/cpu/LocalizationPipeline.java:721-1073,1099-1250
  *   zo_calib_nll         process/cpu/ZernikePhaseRetrieval.java:131-140,1729-1736,1842-1879
  * Functions with no reference counterpart (analytic tangents, the adjoint,
  * the FP64 Levenberg-Marquardt optimum) are derived from the same model and
@@ -131,6 +132,19 @@ void zo_detector_templates(const zo_detector* d, double* psf /* nZ*R*R */);
  * writes up to cap records of 7 doubles [score, r, i, j, dz, dx, dy],
  * sorted descending by score (stable).  corr (nZ*W*H) may be NULL. */
 int  zo_detector_run(zo_detector* d, const double* image, double* out, int cap, double* corr);
+
+/* One frame through LocalizationThread.run: detector candidates in score order,
+ * myGetMaxPositions, mask test over the patch, getAandB with the candidate's
+ * template, zo_localize (z clamp [minZ-0.3, maxZ+0.3], photon break threshold 0
+ * as the CPU pipeline leaves it), acceptance on photons (> photonThr), shift
+ * (< 10 xystep), centre pixel and z in (minZ, maxZ), masking of the model core
+ * (model - B > 0.2 max) or of the bright patch pixels (> 0.5 maxPatch).
+ * Writes up to cap records of 12 doubles [frame(0), x_nm, y_nm, z_nm, A, B,
+ * score, crlbx_nm, crlby_nm, crlbz_nm, lik, candidate index]; returns the
+ * number of localizations. */
+int  zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double zfocus,
+                       double minZ, double maxZ, int iterMax, double photonThr,
+                       double* out, int cap);
 
 /* Calibration negative log-likelihood of one bead stack (ZernikePhaseRetrieval.Model). */
 double zo_calib_nll(zo_model* m, int S, const double* stack /* S*R*R */, const double* dz,

@@ -51,3 +51,39 @@ def init_restated(rois, tmpl, zmin, zstep):
         AB = O.get_a_and_b(rois[b].astype(np.float64), tmpl[r], 0.1)
         out[b] = [AB[0], AB[1], 0.0, 0.0, z0]
     return out
+
+
+def render_frame(model, W, H, emitters, bg, seed=None):
+    """Frame W x H ([i][ii], i the x column) with emitters (pi, pj, z, A): each
+    rendered as A * PSF on the R x R patch around round(pi), round(pj), clipped
+    at the frame edge (the
+    model's emitter sits at pixel R//2 - x/dx, so x = (ci - pi) dx), plus a
+    uniform background, Poisson noise when seed is given; float32."""
+    R = model.R
+    mu = np.full((W, H), float(bg))
+    for (pi, pj, z, A) in emitters:
+        ci, cj = int(round(pi)), int(round(pj))
+        p = A * model.psf((ci - pi) * DX, (cj - pj) * DX, 0.0, z)
+        i0, j0 = ci - R // 2, cj - R // 2
+        a0, b0 = max(0, -i0), max(0, -j0)
+        a1, b1 = min(R, W - i0), min(R, H - j0)
+        mu[i0 + a0:i0 + a1, j0 + b0:j0 + b1] += p[a0:a1, b0:b1]
+    if seed is None:
+        return mu.astype(np.float32)
+    return np.random.default_rng(seed).poisson(mu).astype(np.float32)
+
+
+def random_emitters(W, H, n, seed, margin=12, zrange=(-0.6, 0.6), photons=(1500, 5000), min_sep=0.0):
+    """n emitters uniform in the frame interior, z uniform, photons log-uniform;
+    with min_sep > 0, positions closer than min_sep px to an earlier one are redrawn."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        pi = rng.uniform(margin, W - margin)
+        pj = rng.uniform(margin, H - margin)
+        if min_sep > 0 and any((pi - a) ** 2 + (pj - b) ** 2 < min_sep ** 2 for a, b, _, _ in out):
+            continue
+        z = rng.uniform(*zrange)
+        A = float(np.exp(rng.uniform(np.log(photons[0]), np.log(photons[1]))))
+        out.append((pi, pj, z, A))
+    return out

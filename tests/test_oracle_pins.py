@@ -293,3 +293,70 @@ def test_detector_tie_order_is_stable():
     assert {t[1:] for t in top} == {(15, 20), (48, 20)}
     if c[0][0] == c[1][0]:
         assert top[0][1] < top[1][1]
+
+
+def _frame_model():
+    return O.Model(O.optics(size_fft=64), 21, Z.astigmatic_coeffs(28))
+
+
+def _match(locs, truth):
+    """For each localization, the nearest planted emitter (nm distances)."""
+    t = np.array([[pi * simdata.DX * 1000, pj * simdata.DX * 1000, z * 1000, A] for pi, pj, z, A in truth])
+    out = []
+    for l in locs:
+        d = np.hypot(t[:, 0] - l[1], t[:, 1] - l[2])
+        k = int(np.argmin(d))
+        out.append((k, d[k], abs(t[k, 2] - l[3]), l[4] / t[k, 3]))
+    return out
+
+
+def test_frame_walk_localizes_each_planted_emitter_once():
+    # LocalizationThread.run: every emitter is found at many templates; the mask
+    # left by the first accepted fit removes the others, so each emitter gives
+    # exactly one record, at its position (x_nm = 1000 (i dx - x)).
+    m = _frame_model()
+    em = [(30.3, 25.6, 0.2, 3000), (60.8, 70.1, -0.3, 2500), (25.2, 70.4, 0.05, 4000),
+          (70.5, 30.5, 0.4, 1500)]
+    img = simdata.render_frame(m, 96, 96, em, 20.0, seed=5).astype(np.float64)
+    det = O.Detector(m, 96, 96, -0.7, 0.7, 0.1, thr=0.1)
+    locs = O.frame_localize(m, det, img, 0.0, -0.7, 0.7)
+    assert len(locs) == 4
+    mt = _match(locs, em)
+    assert sorted(k for k, *_ in mt) == [0, 1, 2, 3]
+    for k, dxy, dz, ra in mt:
+        assert dxy < 40 and dz < 80 and 0.85 < ra < 1.15
+    assert np.all(np.diff(locs[:, 11]) > 0)           # acceptance follows the candidate order
+    assert np.all(locs[:, 7:10] > 0) and np.all(locs[:, 7:10] < 60)   # CRLB in nm
+
+
+def test_frame_walk_masks_overlapping_emitter():
+    # two emitters 6 px apart: the second one's patch touches the model-core mask
+    # of the first accepted fit, so the reference skips it
+    m = _frame_model()
+    em = [(40.0, 40.0, 0.0, 4000), (46.0, 40.0, 0.0, 3000)]
+    img = simdata.render_frame(m, 80, 80, em, 10.0, seed=7).astype(np.float64)
+    det = O.Detector(m, 80, 80, -0.5, 0.5, 0.1, thr=0.1)
+    locs = O.frame_localize(m, det, img, 0.0, -1.0, 1.0)
+    assert len(locs) == 1
+    # with the accepted z range (-0.5, 0.5) the blended first fit (z ~ -0.53) is
+    # rejected; the bright-pixel mask of the rejected patch then covers both
+    assert len(O.frame_localize(m, det, img, 0.0, -0.5, 0.5)) == 0
+    # far apart (> patch + shift window), both are kept
+    em2 = [(25.0, 25.0, 0.0, 4000), (55.0, 55.0, 0.0, 3000)]
+    img2 = simdata.render_frame(m, 80, 80, em2, 10.0, seed=7).astype(np.float64)
+    assert len(O.frame_localize(m, det, img2, 0.0, -0.5, 0.5)) == 2
+
+
+def test_frame_walk_photon_threshold_and_border():
+    # A below photonThreshold is fitted but not accepted; an emitter closer to the
+    # border than R/2 + 1 is never selected (myGetMaxPositions bounds)
+    m = _frame_model()
+    em = [(40.0, 30.0, 0.0, 380), (40.0, 60.0, 0.1, 3000), (8.0, 45.0, 0.0, 5000)]
+    img = simdata.render_frame(m, 80, 80, em, 5.0, seed=11).astype(np.float64)
+    det = O.Detector(m, 80, 80, -0.5, 0.5, 0.1, thr=0.1)
+    locs = O.frame_localize(m, det, img, 0.0, -0.5, 0.5, photon_thr=500.0)
+    assert len(locs) == 1
+    assert abs(locs[0, 2] - 60.0 * simdata.DX * 1000) < 40
+    # lowering the threshold accepts the dim one too
+    locs2 = O.frame_localize(m, det, img, 0.0, -0.5, 0.5, photon_thr=100.0)
+    assert len(locs2) == 2

@@ -1,0 +1,71 @@
+// zola_ctx.hpp -- the context behind the opaque zola_ctx handle, shared by
+// the translation units of the library (C ABI, detector, calibration).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/zola_hip.h"
+#include "zola_psf64.hpp"
+
+struct zola_ctx {
+    zola_optics o{};
+    int N = 0, R = 0, s2 = 0, D = 0, J = 0;
+    double center = 0, ringsize = 0, radius_px = 0, kappa = 0, kz0 = 0, kzo0 = 0;
+    std::vector<int> di, dj, wu, wv;
+    std::vector<double> kz, kzo, amp, Z, coef, phase;
+    zola::Model64 model{};
+    float* d_tab = nullptr;  // 4 x 1024 floats: phi, amp, kz', kzo'
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    // grow-only staging for the host-pointer entry points
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    float* d_tmpl = nullptr;      // template stack for zola_fit_init_dev
+    size_t tmpl_bytes = 0;
+    double* d_Zw = nullptr;       // Zernike basis on the 32x32 window, J x 1024 (FP64)
+    float* d_qbar = nullptr;      // per-PSF dL/dphase for the adjoint, grow-only
+    size_t qbar_bytes = 0;
+};
+
+inline int set_err(zola_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_err(ctx, ZOLA_EHIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+
+inline int stage(zola_ctx* c, size_t bytes) {
+    if (bytes <= c->stage_bytes) return ZOLA_OK;
+    if (c->d_stage) HIP_TRY(c, hipFree(c->d_stage));
+    c->d_stage = nullptr;
+    c->stage_bytes = 0;
+    HIP_TRY(c, hipMalloc(&c->d_stage, bytes));
+    c->stage_bytes = bytes;
+    return ZOLA_OK;
+}
+
+
+// FP64 PSF of the reference pipeline on the host (process/cpu/PSFPhaseMany.java:553-622):
+// N x N pupil field, unitary inverse FFT, |E|^2 on the s2 grid, Gaussian blur by
+// FFT with the float-rounded kernel of PSFPhaseMany.java:387-404, modulus, R x R
+// crop, normalised to sum 1.  Used where a result must agree with an FP64
+// computation to rounding (detector templates).
+void zola_psf_f64_host(const zola_ctx* c, double x, double y, double zoil, double zwat, double* out);

@@ -19,6 +19,7 @@
 #include "../../include/zola_hip.h"
 #include "zola_kernels.h"
 #include "zola_adjoint64.hpp"
+#include "zola_ctx.hpp"
 
 namespace {
 
@@ -47,46 +48,6 @@ double zernike_value(int n, int m, double theta, double rho) {
 }
 
 }  // namespace
-
-struct zola_ctx {
-    zola_optics o{};
-    int N = 0, R = 0, s2 = 0, D = 0, J = 0;
-    double center = 0, ringsize = 0, radius_px = 0, kappa = 0, kz0 = 0, kzo0 = 0;
-    std::vector<int> di, dj, wu, wv;
-    std::vector<double> kz, kzo, amp, Z, coef, phase;
-    zola::Model64 model{};
-    float* d_tab = nullptr;  // 4 x 1024 floats: phi, amp, kz', kzo'
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    std::string err;
-    // grow-only staging for the host-pointer entry points
-    void* d_stage = nullptr;
-    size_t stage_bytes = 0;
-    float* d_tmpl = nullptr;      // template stack for zola_fit_init_dev
-    size_t tmpl_bytes = 0;
-    double* d_Zw = nullptr;       // Zernike basis on the 32x32 window, J x 1024 (FP64)
-    float* d_qbar = nullptr;      // per-PSF dL/dphase for the adjoint, grow-only
-    size_t qbar_bytes = 0;
-};
-
-static int set_err(zola_ctx* c, int code, const char* fmt, ...) {
-    if (c) {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
-        c->err = buf;
-    }
-    return code;
-}
-
-#define HIP_TRY(ctx, expr)                                                              \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return set_err(ctx, ZOLA_EHIP, "%s: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
 
 static void compute_phase(zola_ctx* c) {
     c->phase.assign(c->D, 0.0);
@@ -137,16 +98,6 @@ static int upload_tables(zola_ctx* c) {
     HIP_TRY(c, hipMemcpyAsync(c->d_tab, tab.data(), tab.size() * sizeof(float),
                               hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return ZOLA_OK;
-}
-
-static int stage(zola_ctx* c, size_t bytes) {
-    if (bytes <= c->stage_bytes) return ZOLA_OK;
-    if (c->d_stage) HIP_TRY(c, hipFree(c->d_stage));
-    c->d_stage = nullptr;
-    c->stage_bytes = 0;
-    HIP_TRY(c, hipMalloc(&c->d_stage, bytes));
-    c->stage_bytes = bytes;
     return ZOLA_OK;
 }
 

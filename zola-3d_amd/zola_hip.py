@@ -25,8 +25,12 @@ SYMBOLS = [
     "zola_set_coefficients", "zola_set_sigma", "zola_set_stream", "zola_psf_forward",
     "zola_psf_forward_dev", "zola_psf_backward", "zola_psf_backward_dev", "zola_fit_batch",
     "zola_fit_batch_dev", "zola_fit_init_dev", "zola_get_stream", "zola_calib_eval_dev",
-    "zola_calib_fit",
+    "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
+    "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
+    "zola_detector_correlation", "zola_localize_frames",
 ]
+CAND_FIELDS = 7
+LOC_FIELDS = 12
 
 
 class ZolaError(RuntimeError):
@@ -44,6 +48,12 @@ class FitConfig(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_iter", C.c_int), ("zmin", C.c_double),
                 ("zmax", C.c_double), ("zfocus", C.c_double), ("photon_threshold", C.c_double),
                 ("tol", C.c_double)]
+
+
+class AcceptConfig(C.Structure):
+    _fields_ = [("photon_threshold", C.c_double), ("min_z", C.c_double), ("max_z", C.c_double),
+                ("snr_min", C.c_double), ("model_mask_frac", C.c_double),
+                ("patch_mask_frac", C.c_double), ("max_shift_px", C.c_double)]
 
 
 _lib = None
@@ -85,6 +95,17 @@ def lib():
         if hasattr(L, "zola_fit_init_dev"):
             L.zola_fit_init_dev.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                             C.c_double, vp]
+        if hasattr(L, "zola_detector_create"):
+            L.zola_detector_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, C.c_double, C.POINTER(vp)]
+            L.zola_detector_destroy.argtypes = [vp]
+            L.zola_detector_info.argtypes = [vp, vp]
+            L.zola_detector_range.argtypes = [vp, vp]
+            L.zola_detector_templates.argtypes = [vp, vp]
+            L.zola_detect_frames.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp]
+            L.zola_detector_correlation.argtypes = [vp, vp, vp]
+            L.zola_localize_frames.argtypes = [vp, C.POINTER(FitConfig), C.POINTER(AcceptConfig),
+                                               C.c_int, vp, C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -230,6 +251,83 @@ class Context:
 
 
 # -- Zernike index helpers (calibration-file conventions) ------------------
+class Detector:
+    """Frame detector and frame localization pipeline on a context
+    (process/cpu/Predetection.java and the LocalizationThread walk of
+    process/cpu/LocalizationPipeline.java)."""
+
+    def __init__(self, ctx, W, H, zmin, zmax, zstep, threshold=0.1, zfocus=0.0):
+        self.ctx = ctx
+        self.W, self.H = int(W), int(H)
+        h = C.c_void_p()
+        rc = lib().zola_detector_create(ctx._h, self.W, self.H, zmin, zmax, zstep, threshold, zfocus,
+                                        C.byref(h))
+        self._h = h if h.value else None
+        if rc != 0:
+            self.close()
+            raise ZolaError("zola_detector_create: %s (%d)" % (lib().zola_last_error(ctx._h).decode(), rc))
+        info = np.zeros(5, dtype=np.int32)
+        lib().zola_detector_info(self._h, _ptr(info))
+        self.nZ, self.FX, self.FY = int(info[0]), int(info[1]), int(info[2])
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.zola_detector_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise ZolaError("%s: %s (%d)" % (what, lib().zola_last_error(self.ctx._h).decode(), rc))
+
+    def range(self):
+        r = np.zeros(self.nZ)
+        self._check(lib().zola_detector_range(self._h, _ptr(r)), "zola_detector_range")
+        return r
+
+    def templates(self):
+        t = np.zeros((self.nZ, self.ctx.R, self.ctx.R))
+        self._check(lib().zola_detector_templates(self._h, _ptr(t)), "zola_detector_templates")
+        return t
+
+    def correlation(self, frame):
+        f = np.ascontiguousarray(frame, dtype=np.float32).reshape(self.W, self.H)
+        out = np.zeros((self.nZ, self.W, self.H))
+        self._check(lib().zola_detector_correlation(self._h, _ptr(f), _ptr(out)), "zola_detector_correlation")
+        return out
+
+    def detect(self, frames, cap=100000):
+        """Candidate lists (one (n, 7) array per frame) of frames (F, W, H)."""
+        fr = np.ascontiguousarray(frames, dtype=np.float32).reshape(-1, self.W, self.H)
+        F = fr.shape[0]
+        cand = np.zeros((F, cap, CAND_FIELDS))
+        counts = np.zeros(F, dtype=np.int32)
+        self._check(lib().zola_detect_frames(self._h, F, _ptr(fr), cap, _ptr(cand), _ptr(counts)),
+                    "zola_detect_frames")
+        return [cand[f, :min(int(counts[f]), cap)].copy() for f in range(F)], counts
+
+    @staticmethod
+    def accept_config(photon_threshold=500.0, min_z=-1.0, max_z=1.0, snr_min=1.0,
+                      model_mask_frac=0.2, patch_mask_frac=0.5, max_shift_px=10.0):
+        return AcceptConfig(photon_threshold, min_z, max_z, snr_min, model_mask_frac,
+                            patch_mask_frac, max_shift_px)
+
+    def localize(self, frames, fit_cfg, acc_cfg, cap=1000000):
+        """Localizations (n, 12) of frames (F, W, H), per-frame counts and fit rounds."""
+        fr = np.ascontiguousarray(frames, dtype=np.float32).reshape(-1, self.W, self.H)
+        F = fr.shape[0]
+        locs = np.zeros((cap, LOC_FIELDS))
+        n = C.c_int()
+        per = np.zeros(F, dtype=np.int32)
+        rounds = C.c_int()
+        self._check(lib().zola_localize_frames(self._h, C.byref(fit_cfg), C.byref(acc_cfg), F, _ptr(fr),
+                                               cap, _ptr(locs), C.byref(n), _ptr(per), C.byref(rounds)),
+                    "zola_localize_frames")
+        return locs[:min(n.value, cap)].copy(), per, rounds.value
+
+
 def noll_to_nm(j):
     """Noll index j >= 1 -> (n, m), m > 0 for cos and m < 0 for sin terms."""
     n, j1 = 0, j - 1
