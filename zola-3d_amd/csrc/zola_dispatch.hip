@@ -3,6 +3,7 @@
 
 #include "zola_kernels.h"
 #include "zola_adjoint64.hpp"
+#include "zola_psf128.hpp"
 
 namespace zola {
 template <int R>
@@ -12,6 +13,10 @@ template <int R>
 hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st);
 template <int R>
 hipError_t launch_adjoint_r(const Model64& M, const AdjArgs& A, hipStream_t st);
+template <int R>
+hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params, float* out, hipStream_t st);
+template <int R>
+hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st);
 template <int R>
 hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
                          double zstep_t, double* init, hipStream_t st);
@@ -89,4 +94,17 @@ hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const flo
     if (J <= 0 || B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_project_zernike, dim3(J), dim3(256), 0, st, B, J, Zw, qbar, da);
     return hipGetLastError();
+}
+
+bool zola_kernels_support_roi128(int R) { return R == 41; }
+
+hipError_t zola_launch_psf_forward128(const Model128& M, int R, int B, const float* params, float* out,
+                                      hipStream_t st) {
+    if (R != 41) return hipErrorInvalidValue;
+    return launch_psf_forward128_r<41>(M, B, params, out, st);
+}
+
+hipError_t zola_launch_fit128(const Model128& M, int R, int mode, const FitArgs& A, hipStream_t st) {
+    if (R != 41) return hipErrorInvalidValue;
+    return launch_fit128_r<41>(M, mode, A, st);
 }

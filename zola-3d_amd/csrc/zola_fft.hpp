@@ -115,6 +115,25 @@ __device__ __forceinline__ void fft_dif(float (&re)[L], float (&im)[L]) {
 template <int L>
 __device__ __forceinline__ constexpr int fidx(int m) { return bitrev(m, ilog2(L)); }
 
+// sin and cos of q for moderate |q| (|q| < 1e4 rad), in a few registers:
+// quadrant reduction r = q - k pi/2 with pi/2 split into two floats
+// (c1 = fl(pi/2), c2 = pi/2 - c1) and fused multiply-adds, so each k*c is
+// exact before rounding; then the Taylor polynomials of sin (to r^9) and cos
+// (to r^8) on |r| <= pi/4, whose truncation errors (r^11/11!, r^10/10!) stay
+// below 3.5e-8.
+__device__ __forceinline__ void sincos_reduced(float q, float& s, float& c) {
+    const float k = rintf(q * 0.636619772367581343f);  // 2/pi
+    float r = fmaf(-k, 1.57079637050628662109375f, q);
+    r = fmaf(-k, -4.37113900018624283e-8f, r);
+    const float r2 = r * r;
+    const float sp = r + r * r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f + r2 * (1.f / 362880.f))));
+    const float cp = 1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f))));
+    const int qd = ((int)k) & 3;
+    const float sa = (qd & 1) ? cp : sp, ca = (qd & 1) ? sp : cp;
+    s = (qd & 2) ? -sa : sa;
+    c = ((qd + 1) & 2) ? -ca : ca;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);

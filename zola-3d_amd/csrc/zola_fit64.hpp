@@ -13,8 +13,14 @@
 //  * k_fit_ref64 -- the reference's coordinate descent with central finite
 //    differences (process/cpu/Localization.java:29-31,115,317-653,672-1143),
 //    every likelihood a full PSF synthesis (cached when only A/B change).
+//
+// The fit kernels are written against a synthesis policy (Synth64, Synth128:
+// pupil grid 64 or 128) that provides the value-only and the value+tangent
+// evaluations and the LDS each needs; the ROI pixel layout (RoiLayout<R>) and
+// all fitting logic are shared.
 #pragma once
 #include "zola_psf64.hpp"
+#include "zola_psf128.hpp"
 
 namespace zola {
 
@@ -60,17 +66,18 @@ __device__ __forceinline__ void eval_E64(const Model64& M, const BlockTables64& 
 }
 
 // |v| normalised to sum 1 over the ROI; returns the sum.
-template <int R>
-__device__ __forceinline__ float normalise(int lane, const float (&v)[RoiLayout<R>::RPL],
-                                           float (&psf)[RoiLayout<R>::RPL]) {
-    constexpr int RPL = RoiLayout<R>::RPL;
+template <class SY, int R>
+__device__ __forceinline__ float normalise(int lane, int wr, typename SY::Red& red,
+                                           const float (&v)[SY::template RPL<R>],
+                                           float (&psf)[SY::template RPL<R>]) {
+    constexpr int RPL = SY::template RPL<R>;
     float s = 0.f;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
         psf[t] = fabsf(v[t]);
-        if (Px<R>::valid(lane, t)) s += psf[t];
+        if (SY::template valid<R>(lane, wr, t)) s += psf[t];
     }
-    s = wave_sum(s);
+    s = SY::sum(s, red, lane, wr);
     const float inv = 1.f / s;
 #pragma unroll
     for (int t = 0; t < RPL; t++) psf[t] *= inv;
@@ -78,19 +85,20 @@ __device__ __forceinline__ float normalise(int lane, const float (&v)[RoiLayout<
 }
 
 // Poisson NLL without ln n! (Localization.computeLikelihood, 1125-1143), FP64.
-template <int R>
-__device__ __forceinline__ double nll64(int lane, const float (&psf)[RoiLayout<R>::RPL],
-                                        const float (&n)[RoiLayout<R>::RPL], double A, double Bg) {
-    constexpr int RPL = RoiLayout<R>::RPL;
+template <class SY, int R>
+__device__ __forceinline__ double nll64(int lane, int wr, typename SY::Red& red,
+                                        const float (&psf)[SY::template RPL<R>],
+                                        const float (&n)[SY::template RPL<R>], double A, double Bg) {
+    constexpr int RPL = SY::template RPL<R>;
     double acc = 0.0;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
-        if (Px<R>::valid(lane, t)) {
+        if (SY::template valid<R>(lane, wr, t)) {
             const double mu = (double)psf[t] * A + Bg;
             acc += mu - (double)n[t] * log(mu);
         }
     }
-    return wave_sum(acc);
+    return SY::sum(acc, red, lane, wr);
 }
 
 // Gaussian elimination with partial pivoting on a 5x5 system (A destroyed).
@@ -179,6 +187,121 @@ __device__ __forceinline__ void lm_eval64(const Model64& M, const BlockTables64&
     wave_lds_fence();
 }
 
+// Synthesis policies: tables shared by a block, per-wave scratch (floats) of
+// the LM and reference-form kernels, and the two evaluations.
+struct Synth64 {
+    using Model = Model64;
+    using Shared = BlockTables64;
+    static constexpr int kWaves = 1;  // waves per ROI
+    template <int R>
+    static constexpr int RPL = RoiLayout<R>::RPL;
+    // lane = (ROI column j, row group g), rows g*RPL + t
+    template <int R>
+    __device__ static bool valid(int lane, int, int t) { return Px<R>::valid(lane, t); }
+    template <int R>
+    __device__ static int pix(int lane, int, int t) { return ((lane / R) * RPL<R> + t) * R + lane % R; }
+    struct Red {};
+    __device__ static float sum(float v, Red&, int, int) { return wave_sum(v); }
+    __device__ static double sum(double v, Red&, int, int) { return wave_sum(v); }
+    template <int K>
+    __device__ static void sumv(double (&v)[K], Red&, int, int) {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    }
+    template <int K>
+    __device__ static void sumv(float (&v)[K], Red&, int, int) {
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    }
+    static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
+    static constexpr int kRefFloats = (int)(sizeof(WaveScratch64) / sizeof(float));
+    __device__ static void load(Shared& T, const Model& M) { load_tables64(T, M); }
+    template <int R>
+    __device__ static void eval(const Model& M, const Shared& T, float* scr, int lane, int, float x, float y,
+                                float z, float zoil, float (&v)[RoiLayout<R>::RPL]) {
+        eval_E64<R>(M, T, *reinterpret_cast<WaveScratch64*>(scr), lane, T.h3[lane], x, y, z, zoil, v);
+    }
+    template <int R>
+    __device__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int, float x, float y,
+                                   float z, float zoil, float (&vB)[RoiLayout<R>::RPL],
+                                   float (&vX)[RoiLayout<R>::RPL], float (&vY)[RoiLayout<R>::RPL],
+                                   float (&vZ)[RoiLayout<R>::RPL]) {
+        static_assert(3 * R * kTS <= 4 * kN * kS1, "transposed blur rows must fit the scratch");
+        lm_eval64<R>(M, T, scr, lane, x, y, z, zoil, vB, vX, vY, vZ);
+    }
+};
+
+// Four waves per ROI (zola_psf128.hpp); lane = ROI column j, wave wr owns
+// rows wr*RQ + t.  Every reduction is summed over the four waves through LDS
+// in a fixed order, so all waves see identical values and take identical
+// branches (the block barriers inside the evaluations stay matched).
+struct Synth128 {
+    using Model = Model128;
+    using Shared = NoShared;
+    static constexpr int kWaves = 4;
+    static constexpr int kLmFloats = 2 * (int)(sizeof(Buf128) / sizeof(float));
+    static constexpr int kRefFloats = kLmFloats;
+    template <int R>
+    static constexpr int RPL = Roi128<R>::RQ;
+    template <int R>
+    __device__ static bool valid(int lane, int wr, int t) { return lane < R && wr * RPL<R> + t < R; }
+    template <int R>
+    __device__ static int pix(int lane, int wr, int t) { return (wr * RPL<R> + t) * R + lane; }
+    struct Red {
+        double d[4][24];
+        float f[4][4];
+    };
+    template <int K>
+    __device__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
+        static_assert(K <= 24, "reduction width");
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            v[k] = wave_sum(v[k]);
+            if (lane == 0) r.d[wr][k] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = ((r.d[0][k] + r.d[1][k]) + r.d[2][k]) + r.d[3][k];
+        __syncthreads();
+    }
+    template <int K>
+    __device__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
+        static_assert(K <= 4, "reduction width");
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            v[k] = wave_sum(v[k]);
+            if (lane == 0) r.f[wr][k] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = ((r.f[0][k] + r.f[1][k]) + r.f[2][k]) + r.f[3][k];
+        __syncthreads();
+    }
+    __device__ static double sum(double v, Red& r, int lane, int wr) {
+        double a[1] = {v};
+        sumv<1>(a, r, lane, wr);
+        return a[0];
+    }
+    __device__ static float sum(float v, Red& r, int lane, int wr) {
+        float a[1] = {v};
+        sumv<1>(a, r, lane, wr);
+        return a[0];
+    }
+    __device__ static void load(Shared&, const Model&) {}
+    template <int R>
+    __device__ static void eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
+                                float z, float zoil, float (&v)[Roi128<R>::RQ]) {
+        float d0[Roi128<R>::RQ], d1[Roi128<R>::RQ], d2[Roi128<R>::RQ];
+        eval128<R, false>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, v, d0, d1, d2);
+    }
+    template <int R>
+    __device__ static void lm_eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
+                                   float z, float zoil, float (&vB)[Roi128<R>::RQ], float (&vX)[Roi128<R>::RQ],
+                                   float (&vY)[Roi128<R>::RQ], float (&vZ)[Roi128<R>::RQ]) {
+        eval128<R, true>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
+    }
+};
+
 struct LmState {
     double H[15];   // normal matrix, packed upper triangle
     double gr[5];   // gradient
@@ -194,29 +317,40 @@ __device__ __forceinline__ double hget(const double (&H)[15], int p, int q) {
     return H[off + (q - p)];
 }
 
-template <int R, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
-    static_assert(3 * R * kTS <= 4 * kN * kS1, "transposed blur rows must fit the scratch");
-    // one LDS struct fixes the layout: tables first (low addresses, so lane
-    // addresses plus immediate offsets reach them), then per-wave state.
-    struct Lds {
-        BlockTables64 T;
-        LmState L[WPB];
-        WaveScratch64 S[2 * WPB];
-    };
-    __shared__ Lds lds;
-    BlockTables64& T = lds.T;
-    load_tables64(T, M);
+// One LDS struct fixes the layout: shared tables first (low addresses, so lane
+// addresses plus immediate offsets reach them), then per-wave state and scratch.
+// WPB waves per block, WPB / SY::kWaves ROIs; the fit state is private to
+// each wave (the waves of a ROI keep identical copies).
+template <class SY, int WPB>
+struct LmLds {
+    typename SY::Shared T;
+    LmState L[WPB];
+    typename SY::Red red[WPB / SY::kWaves];
+    alignas(16) float S[WPB / SY::kWaves][SY::kLmFloats];
+};
+
+template <class SY, int WPB>
+struct RefLds {
+    typename SY::Shared T;
+    typename SY::Red red[WPB / SY::kWaves];
+    alignas(16) float S[WPB / SY::kWaves][SY::kRefFloats];
+};
+
+template <class SY, int R, int WPB>
+__device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const FitArgs& A, LmLds<SY, WPB>& lds) {
+    auto& T = lds.T;
+    SY::load(T, M);
     __syncthreads();
-    constexpr int RPL = RoiLayout<R>::RPL;
+    constexpr int RPL = SY::template RPL<R>;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int b = blockIdx.x * WPB + w;
+    const int wr = w % SY::kWaves, slot = w / SY::kWaves;
+    const int b = blockIdx.x * (WPB / SY::kWaves) + slot;
     if (b >= A.B) return;
-    float* scr = lds.S[2 * w].re;
+    float* scr = lds.S[slot];
+    auto& red = lds.red[slot];
     LmState& L = lds.L[w];
-    const int j = lane % R, g = lane / R;
-    const float* pix = A.pixels + (size_t)b * R * R + j;
+    const float* pix = A.pixels + (size_t)b * R * R;
     const float zoil = (float)A.zfocus;
     // Wave-uniform FP64 state lives in LDS between uses; registers are kept
     // for the transforms.  Damping follows Nielsen's rule: on success
@@ -289,16 +423,16 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
             int lane_it = lane;
             __asm__ __volatile__("" : "+v"(lane_it));
             float vB[RPL], vX[RPL], vY[RPL], vZ[RPL];
-            lm_eval64<R>(M, T, scr, lane_it, (float)L.tn[2], (float)L.tn[3], (float)L.tn[4], zoil,
-                         vB, vX, vY, vZ);
+            SY::template lm_eval<R>(M, T, scr, lane_it, wr, (float)L.tn[2], (float)L.tn[3], (float)L.tn[4],
+                                    zoil, vB, vX, vY, vZ);
             nfield += 2;
             float psf[RPL], n[RPL];
 #pragma unroll
             for (int t = 0; t < RPL; t++)
-                n[t] = Px<R>::valid(lane, t) ? pix[(g * RPL + t) * R] : 0.f;
-            const float Sn = normalise<R>(lane, vB, psf);
+                n[t] = SY::template valid<R>(lane, wr, t) ? pix[SY::template pix<R>(lane, wr, t)] : 0.f;
+            const float Sn = normalise<SY, R>(lane, wr, red, vB, psf);
             const double tA = L.tn[0], tB = L.tn[1];
-            const double nn = nll64<R>(lane, psf, n, tA, tB);
+            const double nn = nll64<SY, R>(lane, wr, red, psf, n, tA, tB);
             if (nn != nn) nan = 1;
             const double noise = A.noise_rel * (L.th[0] + (double)(R * R) * L.th[1]);
             accepted = first || nn <= L.nll + noise;
@@ -319,29 +453,28 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
                 for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
                 L.nll = nn;
                 // d psf = (sgn dv - psf * sum(sgn dv)) / S for the three tangents
-                float sx = 0.f, sy = 0.f, sz = 0.f;
+                float ss[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int t = 0; t < RPL; t++) {
                     const float sg = vB[t] < 0.f ? -1.f : 1.f;
                     vX[t] *= sg;
                     vY[t] *= sg;
                     vZ[t] *= sg;
-                    if (Px<R>::valid(lane, t)) {
-                        sx += vX[t];
-                        sy += vY[t];
-                        sz += vZ[t];
+                    if (SY::template valid<R>(lane, wr, t)) {
+                        ss[0] += vX[t];
+                        ss[1] += vY[t];
+                        ss[2] += vZ[t];
                     }
                 }
-                sx = wave_sum(sx);
-                sy = wave_sum(sy);
-                sz = wave_sum(sz);
+                SY::template sumv<3>(ss, red, lane, wr);
+                const float sx = ss[0], sy = ss[1], sz = ss[2];
                 const float inv = 1.f / Sn;
                 double acc[20];
 #pragma unroll
                 for (int k = 0; k < 20; k++) acc[k] = 0.0;
 #pragma unroll
                 for (int t = 0; t < RPL; t++) {
-                    if (!Px<R>::valid(lane, t)) continue;
+                    if (!SY::template valid<R>(lane, wr, t)) continue;
                     const double mu = (double)psf[t] * tA + tB;
                     const double J[5] = {(double)psf[t], 1.0,
                                          tA * (double)((vX[t] - psf[t] * sx) * inv),
@@ -356,8 +489,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
                         for (int q = p; q < 5; q++) acc[k++] += wh * J[p] * J[q];
                     }
                 }
-#pragma unroll
-                for (int k = 0; k < 20; k++) acc[k] = wave_sum(acc[k]);
+                SY::template sumv<20>(acc, red, lane, wr);
 #pragma unroll
                 for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
 #pragma unroll
@@ -391,7 +523,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
         }
         cr[c - 2] = solve5(Am, e) ? sqrt(e[c]) : (double)NAN;
     }
-    if (lane == 0) {
+    if (lane == 0 && wr == 0) {
 #pragma unroll
         for (int p = 0; p < 5; p++) A.theta[(size_t)b * 5 + p] = L.th[p];
         A.nll[b] = L.nll;
@@ -400,6 +532,18 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
         A.iters[b] = it;
         A.status[b] = (conv ? kStConverged : kStMaxIter) | (nan ? kStNaN : 0) | (nfield << 8);
     }
+}
+
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_lm64(Model64 M, FitArgs A) {
+    __shared__ LmLds<Synth64, WPB> lds;
+    fit_lm_body<Synth64, R, WPB>(M, A, lds);
+}
+
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_lm128(Model128 M, FitArgs A) {
+    __shared__ LmLds<Synth128, WPB> lds;
+    fit_lm_body<Synth128, R, WPB>(M, A, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,22 +604,22 @@ __device__ __forceinline__ void inverse_cofactor5(const double (&a)[25], double 
 
 enum : int { kRsInit, kRsAmpA, kRsAmpB, kRsPos, kRsCheck, kRsFisher, kRsDone };
 
-template <int R, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
-    __shared__ BlockTables64 T;
-    __shared__ WaveScratch64 Sall[WPB];
-    load_tables64(T, M);
+template <class SY, int R, int WPB>
+__device__ __forceinline__ void fit_ref_body(const typename SY::Model& M, const FitArgs& A, RefLds<SY, WPB>& lds) {
+    auto& T = lds.T;
+    SY::load(T, M);
     __syncthreads();
-    constexpr int RPL = RoiLayout<R>::RPL;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int b = blockIdx.x * WPB + w;
+    constexpr int RPL = SY::template RPL<R>;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int wr = w % SY::kWaves, slot = w / SY::kWaves;
+    const int b = blockIdx.x * (WPB / SY::kWaves) + slot;
     if (b >= A.B) return;
-    WaveScratch64& S = Sall[w];
-    const int j = lane % R, g = lane / R;
+    float* scr = lds.S[slot];
+    auto& red = lds.red[slot];
     float n[RPL];
 #pragma unroll
     for (int t = 0; t < RPL; t++)
-        n[t] = Px<R>::valid(lane, t) ? A.pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+        n[t] = SY::template valid<R>(lane, wr, t) ? A.pixels[(size_t)b * R * R + SY::template pix<R>(lane, wr, t)] : 0.f;
     const float zoil = (float)A.zfocus;
     double tA = A.init[(size_t)b * 5 + 0], tB = A.init[(size_t)b * 5 + 1];
     double th[3] = {A.init[(size_t)b * 5 + 2], A.init[(size_t)b * 5 + 3], A.init[(size_t)b * 5 + 4]};
@@ -493,7 +637,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
     double likelihood = 0, save = 0, l1 = 0, l3 = 0, grad = 0, prevLik = 0;
     double oldX = 1.7976931348623157e308, oldY = oldX, oldZ = oldX;
     auto at_theta = [&]() { return hc && pc[0] == th[0] && pc[1] == th[1] && pc[2] == th[2]; };
-    auto L = [&](const float (&ps)[RPL], double a, double bb) { return nll64<R>(lane, ps, n, a, bb); };
+    auto L = [&](const float (&ps)[RPL], double a, double bb) { return nll64<SY, R>(lane, wr, red, ps, n, a, bb); };
     // min_amp (929-977 for A with h*1000, 997-1058 for B with h*10), on the cached psf
     auto min_amp = [&](bool isB) {
         const double hh = isB ? h * 10 : h * 1000;
@@ -517,9 +661,8 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
     while (st != kRsDone) {
         if (want) {
             float v[RPL];
-            eval_E64<R>(M, T, S, lane, T.h3[lane], (float)req[0], (float)req[1], (float)req[2],
-                        zoil, v);
-            normalise<R>(lane, v, tmp);
+            SY::template eval<R>(M, T, scr, lane, wr, (float)req[0], (float)req[1], (float)req[2], zoil, v);
+            normalise<SY, R>(lane, wr, red, v, tmp);
             want = false;
             nfield++;
         }
@@ -710,7 +853,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
             for (int kk = 0; kk < 15; kk++) acc[kk] = 0.0;
 #pragma unroll
             for (int t = 0; t < RPL; t++) {
-                if (!Px<R>::valid(lane, t)) continue;
+                if (!SY::template valid<R>(lane, wr, t)) continue;
                 const double f1 = (double)cur[t] * tA + tB;
                 // x, y, z, then A (the reference differentiates the model f1) and B
                 const double J[5] = {(double)dd[0][t], (double)dd[1][t], (double)dd[2][t], f1, 1.0};
@@ -720,8 +863,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
 #pragma unroll
                     for (int q = p; q < 5; q++) acc[kk++] += (1 / f1) * J[p] * J[q];
             }
-#pragma unroll
-            for (int kk = 0; kk < 15; kk++) acc[kk] = wave_sum(acc[kk]);
+            SY::template sumv<15>(acc, red, lane, wr);
             double F[25], inv[25];
             int kk = 0;
 #pragma unroll
@@ -730,7 +872,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
                 for (int q = p; q < 5; q++) { F[p * 5 + q] = acc[kk]; F[q * 5 + p] = acc[kk]; kk++; }
             likelihood = L(cur, tA, tB);
             inverse_cofactor5(F, inv);
-            if (lane == 0) {
+            if (lane == 0 && wr == 0) {
 #pragma unroll
                 for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = sqrt(inv[c * 5 + c]);
             }
@@ -738,7 +880,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
             continue;
         }
     }
-    if (lane == 0) {
+    if (lane == 0 && wr == 0) {
         if (!ok) {
 #pragma unroll
             for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = (double)NAN;
@@ -753,6 +895,18 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
         A.status[b] = (conv ? kStConverged : 0) | (nbIter == A.max_iter && !conv ? kStMaxIter : 0) |
                       (nan ? kStNaN : 0) | (brk ? kStLowPhoton : 0) | (nfield << 8);
     }
+}
+
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_ref64(Model64 M, FitArgs A) {
+    __shared__ RefLds<Synth64, WPB> lds;
+    fit_ref_body<Synth64, R, WPB>(M, A, lds);
+}
+
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fit_ref128(Model128 M, FitArgs A) {
+    __shared__ RefLds<Synth128, WPB> lds;
+    fit_ref_body<Synth128, R, WPB>(M, A, lds);
 }
 
 }  // namespace zola

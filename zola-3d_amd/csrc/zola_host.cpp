@@ -55,7 +55,7 @@ static void compute_phase(zola_ctx* c) {
         for (int p = 0; p < c->D; p++) c->phase[p] += c->Z[(size_t)j * c->D + p] * c->coef[j];
 }
 
-static void compute_h1(zola_ctx* c) {
+static int compute_h1(zola_ctx* c) {
     // G1(k) = exp(-(k' - s2/2)^2 / (2 sigF^2)), k' = (k + s2/2) % s2, sigF = s2/(2 pi sigma);
     // h1(d) = (1/s2) sum_k G1(k) cos(2 pi k d / s2)  (G1 is even, so h1 is real).
     const int s2 = c->s2;
@@ -66,10 +66,11 @@ static void compute_h1(zola_ctx* c) {
         double kk = (double)((k + s2 / 2) % s2) - (double)(s2 / 2);
         G[k] = std::exp(-.5 * kk * kk / sigpow);
     }
+    std::vector<double> h1(s2), hd(s2);
     for (int d = 0; d < s2; d++) {
         double acc = 0;
         for (int k = 0; k < s2; k++) acc += G[k] * std::cos(2 * PI * (double)k * d / s2);
-        c->model.h1[d] = (float)(acc / s2);
+        h1[d] = acc / s2;
     }
     // Spectral derivative of the blurred image along one axis, per um of
     // emitter shift.  Shifting the emitter by x moves the (band-limited,
@@ -82,11 +83,43 @@ static void compute_h1(zola_ctx* c) {
             if (k == s2 / 2) ks = 0;
             acc -= G[k] * (2 * PI * ks / s2) * std::sin(2 * PI * (double)k * d / s2);
         }
-        c->model.hd[d] = (float)(acc / s2 / c->o.xystep);
+        hd[d] = acc / s2 / c->o.xystep;
     }
+    if (s2 == zola::kN) {
+        for (int d = 0; d < s2; d++) {
+            c->model.h1[d] = (float)h1[d];
+            c->model.hd[d] = (float)hd[d];
+        }
+        return ZOLA_OK;
+    }
+    // s2 = 128: all 128 taps of both kernels, uploaded by upload_taps128
+    c->taps128.resize(2 * s2);
+    for (int d = 0; d < s2; d++) {
+        c->taps128[d] = (float)h1[d];
+        c->taps128[s2 + d] = (float)hd[d];
+    }
+    return ZOLA_OK;
+}
+
+static int upload_taps128(zola_ctx* c) {
+    HIP_TRY(c, hipMemcpyAsync(c->d_taps128, c->taps128.data(), c->taps128.size() * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
 }
 
 static int upload_tables(zola_ctx* c) {
+    if (c->N == zola::kN2) {
+        std::vector<float4> tab(zola::kW2 * zola::kW2, make_float4(0.f, 0.f, 0.f, 0.f));
+        for (int p = 0; p < c->D; p++)
+            tab[c->wu[p] * zola::kW2 + c->wv[p]] =
+                make_float4((float)c->phase[p], (float)c->amp[p], (float)(c->kz[p] - c->kz0),
+                            (float)(c->kzo[p] - c->kzo0));
+        HIP_TRY(c, hipMemcpyAsync(c->d_tab128, tab.data(), tab.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return ZOLA_OK;
+    }
     std::vector<float> tab(4 * 1024, 0.f);
     for (int p = 0; p < c->D; p++) {
         int w = c->wu[p] * 32 + c->wv[p];
@@ -126,12 +159,13 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
     c->radius_px = std::sqrt(c->ringsize) * (N * o->xystep);
     c->kappa = 2 * PI / (o->xystep * N);
     *out = c;
-    if (N != zola::kN)
-        return set_err(c, ZOLA_EUNSUPPORTED, "pupil grid N=%d: this build covers N=64", N);
+    if (N != zola::kN && N != zola::kN2)
+        return set_err(c, ZOLA_EUNSUPPORTED, "pupil grid N=%d: this build covers N=64 and N=128", N);
     if (c->s2 != N)
         return set_err(c, ZOLA_EUNSUPPORTED, "blur grid s2=%d must equal N=%d (R >= N/4)", c->s2, N);
-    if (!zola_kernels_support_roi(R))
-        return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not instantiated", R);
+    if (N == zola::kN ? !zola_kernels_support_roi(R) : !zola_kernels_support_roi128(R))
+        return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not instantiated for N=%d", R, N);
+    const int halfw = N == zola::kN ? zola::kW / 2 : zola::kW2 / 2;
 
     const double left = (o->nwat / o->wavelength) * (o->nwat / o->wavelength);
     const double left_oil = (o->noil / o->wavelength) * (o->noil / o->wavelength);
@@ -162,9 +196,9 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
             else c->amp[id] = 1.0 / std::sqrt((double)D);
             int sx = (i + N / 2) % N, sy = (ii + N / 2) % N;
             int nu = sx < N / 2 ? sx : sx - N, nv = sy < N / 2 ? sy : sy - N;
-            c->wu[id] = nu + zola::kW / 2;
-            c->wv[id] = nv + zola::kW / 2;
-            if (c->wu[id] < 0 || c->wu[id] >= zola::kW || c->wv[id] < 0 || c->wv[id] >= zola::kW)
+            c->wu[id] = nu + halfw;
+            c->wv[id] = nv + halfw;
+            if (c->wu[id] < 0 || c->wu[id] >= 2 * halfw || c->wv[id] < 0 || c->wv[id] >= 2 * halfw)
                 return set_err(c, ZOLA_EUNSUPPORTED,
                                "pupil radius %.3f px exceeds N/4 = %d px", c->radius_px, N / 4);
             kzmin = std::fmin(kzmin, c->kz[id]); kzmax = std::fmax(kzmax, c->kz[id]);
@@ -193,10 +227,25 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
         for (int j = 0; j < J; j++) c->coef[j] = coeffs[j];
     }
     compute_phase(c);
-    compute_h1(c);
+    int rc0 = compute_h1(c);
+    if (rc0) return rc0;
 
     HIP_TRY(c, hipStreamCreate(&c->stream));
     c->own_stream = true;
+    if (N == zola::kN2) {
+        HIP_TRY(c, hipMalloc((void**)&c->d_tab128, zola::kW2 * zola::kW2 * sizeof(float4)));
+        HIP_TRY(c, hipMalloc((void**)&c->d_taps128, 2 * zola::kN2 * sizeof(float)));
+        c->model128.tab = c->d_tab128;
+        c->model128.h1 = c->d_taps128;
+        c->model128.hd = c->d_taps128 + zola::kN2;
+        c->model128.kappa = (float)c->kappa;
+        int rc = upload_tables(c);
+        if (rc) return rc;
+        rc = upload_taps128(c);
+        if (rc) return rc;
+        c->err.clear();
+        return ZOLA_OK;
+    }
     HIP_TRY(c, hipMalloc((void**)&c->d_tab, 4 * 1024 * sizeof(float)));
     c->model.phi = c->d_tab;
     c->model.amp = c->d_tab + 1024;
@@ -220,6 +269,8 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
 int zola_ctx_destroy(zola_ctx* c) {
     if (!c) return ZOLA_OK;
     if (c->d_tab) (void)hipFree(c->d_tab);
+    if (c->d_tab128) (void)hipFree(c->d_tab128);
+    if (c->d_taps128) (void)hipFree(c->d_taps128);
     if (c->d_stage) (void)hipFree(c->d_stage);
     if (c->d_tmpl) (void)hipFree(c->d_tmpl);
     if (c->d_Zw) (void)hipFree(c->d_Zw);
@@ -247,8 +298,9 @@ int zola_set_coefficients(zola_ctx* c, const double* coeffs, int J) {
 int zola_set_sigma(zola_ctx* c, double sigma) {
     if (!c || !(sigma > 0)) return set_err(c, ZOLA_EINVAL, "sigma must be > 0");
     c->o.sigma = sigma;
-    compute_h1(c);
-    return ZOLA_OK;
+    int rc = compute_h1(c);
+    if (rc == ZOLA_OK && c->N == zola::kN2) rc = upload_taps128(c);
+    return rc;
 }
 
 int zola_set_stream(zola_ctx* c, void* s) {
@@ -263,7 +315,8 @@ int zola_psf_forward_dev(zola_ctx* c, int B, const float* params, float* out) {
     if (!c) return ZOLA_EINVAL;
     if (B < 0 || (B > 0 && (!params || !out))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
-    HIP_TRY(c, zola_launch_psf_forward64(c->model, c->R, B, params, out, c->stream));
+    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_psf_forward128(c->model128, c->R, B, params, out, c->stream));
+    else HIP_TRY(c, zola_launch_psf_forward64(c->model, c->R, B, params, out, c->stream));
     return ZOLA_OK;
 }
 
@@ -287,6 +340,7 @@ int zola_psf_forward(zola_ctx* c, int B, const float* params, float* out) {
 int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* w,
                           float* dparams, double* da) {
     if (!c) return ZOLA_EINVAL;
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
     float* qbar = nullptr;
@@ -315,6 +369,7 @@ int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* 
 int zola_psf_backward(zola_ctx* c, int B, const float* params, const float* w, float* dparams,
                       double* da) {
     if (!c) return ZOLA_EINVAL;
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
     const size_t R2 = (size_t)c->R * c->R;
@@ -350,6 +405,7 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
                         const double* amp, const double* bg, double* nll, float* dparams,
                         double* damp, double* dbg, double* da) {
     if (!c) return ZOLA_EINVAL;
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
     if (!params || !pixels || !amp || !bg || !nll || !dparams || !damp || !dbg)
@@ -411,7 +467,8 @@ int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const flo
     A.crlb = crlb;
     A.iters = iters;
     A.status = status;
-    HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, A, c->stream));
+    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, c->stream));
+    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, A, c->stream));
     return ZOLA_OK;
 }
 

@@ -4,6 +4,7 @@
 
 namespace zola {
 struct Model64;
+struct Model128;
 struct FitArgs;
 struct AdjArgs;
 }
@@ -24,3 +25,10 @@ hipError_t zola_launch_adjoint64(const zola::Model64& M, int R, const zola::AdjA
 
 hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar,
                                        double* da, hipStream_t st);
+
+// 128x128 pupil grid
+bool zola_kernels_support_roi128(int R);
+hipError_t zola_launch_psf_forward128(const zola::Model128& M, int R, int B, const float* params,
+                                      float* out, hipStream_t st);
+hipError_t zola_launch_fit128(const zola::Model128& M, int R, int mode, const zola::FitArgs& A,
+                              hipStream_t st);

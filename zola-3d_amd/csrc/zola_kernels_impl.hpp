@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "zola_psf64.hpp"
+#include "zola_psf128.hpp"
 
 namespace zola {
 
@@ -164,6 +165,51 @@ hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_
         dim3 grid((A.B + kWPB_REF - 1) / kWPB_REF), block(64 * kWPB_REF);
         hipLaunchKernelGGL((k_fit_ref64<R, kWPB_REF>), grid, block, 0, st, M, A);
     }
+    return hipGetLastError();
+}
+
+// N = 128: four waves per PSF (one block), see zola_psf128.hpp.
+template <int R>
+__global__ __launch_bounds__(256) void k_psf_forward128(Model128 M, int B, const float* __restrict__ params,
+                                                        float* __restrict__ out) {
+    constexpr int RQ = Roi128<R>::RQ;
+    __shared__ Buf128 S[2];
+    __shared__ float red[4];
+    const int wr = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const float x = params[b * 4 + 0], y = params[b * 4 + 1];
+    const float z = params[b * 4 + 2], zoil = params[b * 4 + 3];
+    float o[RQ], d0[RQ], d1[RQ], d2[RQ];
+    eval128<R, false>(M, S, lane, wr, x, y, z, zoil, o, d0, d1, d2);
+    float s = 0.f;
+#pragma unroll
+    for (int t = 0; t < RQ; t++) {
+        o[t] = fabsf(o[t]);
+        if (lane < R && wr * RQ + t < R) s += o[t];
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wr] = s;
+    __syncthreads();
+    const float inv = 1.f / (((red[0] + red[1]) + red[2]) + red[3]);
+    if (lane < R) {
+#pragma unroll
+        for (int t = 0; t < RQ; t++)
+            if (wr * RQ + t < R) out[(size_t)b * R * R + (wr * RQ + t) * R + lane] = o[t] * inv;
+    }
+}
+
+template <int R>
+hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params, float* out, hipStream_t st) {
+    hipLaunchKernelGGL((k_psf_forward128<R>), dim3(B), dim3(256), 0, st, M, B, params, out);
+    return hipGetLastError();
+}
+
+// Fits: one ROI (four waves) per block.
+template <int R>
+hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st) {
+    if (mode == 1) hipLaunchKernelGGL((k_fit_lm128<R, 4>), dim3(A.B), dim3(256), 0, st, M, A);
+    else hipLaunchKernelGGL((k_fit_ref128<R, 4>), dim3(A.B), dim3(256), 0, st, M, A);
     return hipGetLastError();
 }
 

@@ -361,3 +361,14 @@ def astigmatic_coeffs(J=28, seed=0x20A1D3, astig=1.0, sd=0.05):
     vals = {j: rng.normal(0.0, sd) for j in range(4, J + 1)}
     vals[5] = astig
     return coeffs_from_noll(vals, J)
+
+
+def tetrapod_coeffs(J=45, seed=0x7E7A, primary=(5, 12, 23), amp=(2.0, 3.0), sd=0.1):
+    """Tetrapod-type PSF of the 128-pupil workloads: the astigmatism family of
+    orders 2, 4, 6 (Noll 5, 12, 23) ~ Uniform(amp) rad with random sign,
+    Noll 4..J otherwise ~ Normal(0, sd)."""
+    rng = np.random.default_rng(seed)
+    vals = {j: rng.normal(0.0, sd) for j in range(4, J + 1)}
+    for j in primary:
+        vals[j] = rng.uniform(*amp) * rng.choice([-1.0, 1.0])
+    return coeffs_from_noll(vals, J)
