@@ -42,6 +42,12 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-calib", action="store_true")
+    p.add_argument("--no-n128", action="store_true")
+    p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--no-frames", action="store_true")
+    p.add_argument("--n128-batch", type=int, default=65536)
+    p.add_argument("--mixed-total", type=int, default=200000)
+    p.add_argument("--frames", type=int, default=64)
     p.add_argument("--cpu-sample", type=int, default=0, help="ROIs for the CPU baseline (0: auto)")
     p.add_argument("--copy-probe-mb", type=int, default=0,
                    help="also run one device copy of this size (reference for HBM counters)")
@@ -128,6 +134,190 @@ def calibration_bench(torch, Z, stream):
         "eval_kernel_ms": round(eval_ms, 4),
         "max_coeff_err_rad": round(float(np.abs(res["coeffs"] - truth)[3:].max()), 5),
     }
+
+
+def _events(torch, n=2):
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def depth128_bench(torch, Z, stream, B):
+    """Large-range depth-dependent run on the 128x128 pupil: 45-mode tetrapod PSF
+    (Noll 5, 12, 23 at 2-3 rad, others N(0, 0.1)), n_sample 1.33 / n_imm 1.518,
+    focal depth U[0, 4] um per ROI, z U[-3, 3] um, x, y U(+-1 px), photons
+    log-uniform 1e3..5e4, background U[10, 100], 41x41 ROIs, Poisson noise.
+    Batched LM with per-ROI focus and CRLB; the plain reference form on B/8."""
+    Rn = 41
+    ctx = Z.Context(Z.optics(size_fft=128, nwat=1.33), Rn, Z.tetrapod_coeffs(45))
+    ctx.set_stream(stream.cuda_stream)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x128)
+    u = lambda: torch.rand(B, device="cuda", generator=g, dtype=torch.float64)
+    x, y = (2 * u() - 1) * DX, (2 * u() - 1) * DX
+    z, zf = 6 * u() - 3, 4 * u()
+    A = torch.exp(np.log(1e3) + (np.log(5e4) - np.log(1e3)) * u())
+    bg = 10 + 90 * u()
+    par = torch.stack([x, y, z, zf], 1).float().contiguous()
+    psf = torch.empty(B, Rn, Rn, device="cuda")
+    ctx.psf_forward_dev(par, psf)
+    px = torch.poisson(A.float()[:, None, None] * psf + bg.float()[:, None, None], generator=g).contiguous()
+    del psf
+    init = torch.stack([A * 1.1, bg * 0.9, x + 0.03, y - 0.03, z + 0.1], 1).contiguous()
+    zoil = zf.float().contiguous()
+    th = torch.empty(B, 5, dtype=torch.float64, device="cuda")
+    nl = torch.empty(B, dtype=torch.float64, device="cuda")
+    cr = torch.empty(B, 3, dtype=torch.float64, device="cuda")
+    it = torch.empty(B, dtype=torch.int32, device="cuda")
+    st = torch.empty(B, dtype=torch.int32, device="cuda")
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-3.3, zmax=3.3, tol=1e-4)
+    ctx.fit_zoil_dev(cfg, px, init, zoil, th, nl, cr, it, st)
+    e = _events(torch)
+    e[0].record(stream)
+    ctx.fit_zoil_dev(cfg, px, init, zoil, th, nl, cr, it, st)
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    ms = e[0].elapsed_time(e[1])
+    sn = st.cpu().numpy()
+    err = (th[:, 2:] - torch.stack([x, y, z], 1)).abs().cpu().numpy() * 1000
+    out = {
+        "workload": f"{B} ROIs 41x41, 128x128 pupil, tetrapod 45 modes, n 1.33/1.518, focal depth "
+                    "U[0,4] um per ROI, z U[-3,3] um, LM max 30 it, tol 1e-4 um, CRLB",
+        "rois": B, "kernel_ms": round(ms, 3), "value": round(B / (ms / 1e3), 1), "unit": "localizations/s",
+        "fields_per_roi": round(float((sn >> 8).mean()), 2),
+        "iters_mean": round(float(it.float().mean().item()), 3),
+        "converged_frac": round(float(np.mean((sn & Z.ST_CONVERGED) != 0)), 5),
+        "median_abs_err_vs_truth_nm": [round(float(v), 3) for v in np.median(err, axis=0)],
+        "median_crlb_nm": [round(float(v), 3) for v in np.median(cr.cpu().numpy() * 1000, axis=0)],
+    }
+    Bp = max(1, B // 8)
+    cfgp = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=-3.3, zmax=3.3, photon_threshold=0.0)
+    args = (px[:Bp].contiguous(), init[:Bp].contiguous(), zoil[:Bp].contiguous(), th[:Bp], nl[:Bp], cr[:Bp],
+            it[:Bp], st[:Bp])
+    ctx.fit_zoil_dev(cfgp, *args)
+    e[0].record(stream)
+    ctx.fit_zoil_dev(cfgp, *args)
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    pms = e[0].elapsed_time(e[1])
+    out["plain_gpu"] = {"rois": Bp, "kernel_ms": round(pms, 3), "value": round(Bp / (pms / 1e3), 1),
+                        "fields_per_roi": round(float((st[:Bp].cpu().numpy() >> 8).mean()), 2)}
+    out["batched_vs_plain"] = round((B / ms) / (Bp / pms), 3)
+    ctx.close()
+    return out
+
+
+def mixed_bench(torch, Z, stream, total):
+    """Uneven work: 40% 17x17, 30% 21x21, 20% 31x31, 10% 41x41 ROIs, astigmatic
+    28-mode PSF (64x64 pupil), photons log-uniform 200..20000 (so LM iteration
+    counts spread), background U[5, 50], z U[-1, 1]; initial values from the
+    template correlation; all groups fitted by one graph launch."""
+    sizes, frac = (17, 21, 31, 41), (0.4, 0.3, 0.2, 0.1)
+    coeffs = Z.astigmatic_coeffs(28)
+    groups, ctxs = [], []
+    for k, (Rg, fr) in enumerate(zip(sizes, frac)):
+        Bg = int(round(total * fr))
+        c = Z.Context(Z.optics(), Rg, coeffs)
+        c.set_stream(stream.cuda_stream)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(0x2000 + k)
+        u = lambda: torch.rand(Bg, device="cuda", generator=g, dtype=torch.float64)
+        x, y, z = (2 * u() - 1) * DX, (2 * u() - 1) * DX, 2 * u() - 1
+        A = torch.exp(np.log(200) + (np.log(2e4) - np.log(200)) * u())
+        bg = 5 + 45 * u()
+        psf = torch.empty(Bg, Rg, Rg, device="cuda")
+        c.psf_forward_dev(torch.stack([x, y, z, torch.zeros_like(x)], 1).float().contiguous(), psf)
+        px = torch.poisson(A.float()[:, None, None] * psf + bg.float()[:, None, None], generator=g).contiguous()
+        init = torch.empty(Bg, 5, dtype=torch.float64, device="cuda")
+        c.fit_init_dev(px, NZ, ZMIN_T, ZSTEP_T, 0.0, init)
+        groups.append(dict(R=Rg, pixels=px, init=init,
+                           theta=torch.empty(Bg, 5, dtype=torch.float64, device="cuda"),
+                           nll=torch.empty(Bg, dtype=torch.float64, device="cuda"),
+                           crlb=torch.empty(Bg, 3, dtype=torch.float64, device="cuda"),
+                           iters=torch.empty(Bg, dtype=torch.int32, device="cuda"),
+                           status=torch.empty(Bg, dtype=torch.int32, device="cuda")))
+        ctxs.append(c)
+    ctx = ctxs[1]
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    ctx.fit_mixed_dev(cfg, groups)
+    e = _events(torch)
+    e[0].record(stream)
+    ctx.fit_mixed_dev(cfg, groups)
+    e[1].record(stream)
+    torch.cuda.synchronize()
+    ms = e[0].elapsed_time(e[1])
+    B = sum(int(gr["pixels"].shape[0]) for gr in groups)
+    per = {}
+    for gr in groups:
+        itn = gr["iters"].cpu().numpy()
+        sn = gr["status"].cpu().numpy()
+        per[str(gr["R"])] = {"rois": int(itn.size), "iters_mean": round(float(itn.mean()), 2),
+                             "iters_min": int(itn.min()), "iters_max": int(itn.max()),
+                             "converged_frac": round(float(np.mean((sn & Z.ST_CONVERGED) != 0)), 4)}
+    for c in ctxs:
+        c.close()
+    return {"workload": f"{B} ROIs mixed 40/30/20/10% of 17/21/31/41 px, 64x64 pupil, photons 200..2e4, "
+                        "one graph launch (one kernel node per size)",
+            "rois": B, "graph_ms": round(ms, 3), "value": round(B / (ms / 1e3), 1), "unit": "localizations/s",
+            "groups": per}
+
+
+def frames_bench(torch, Z, stream, F):
+    """Detection + localization on full frames: F frames of 256x256 px, emitters
+    Poisson at 0.5 per um^2 per frame, tetrapod 45-mode PSF on the 128x128 pupil,
+    z U[-3, 3] um, photons log-uniform 1e3..2e4, background U[20, 100] per frame;
+    61 templates at 100 nm on [-3, 3] um, 41x41 patches, reference acceptance
+    (photons > 500, z in (-3, 3))."""
+    W = H = 256
+    Rn = 41
+    h = Rn // 2
+    ctx = Z.Context(Z.optics(size_fft=128), Rn, Z.tetrapod_coeffs(45))
+    ctx.set_stream(stream.cuda_stream)
+    rng = np.random.default_rng(0xF1)
+    counts = rng.poisson(0.5 * (W * DX) * (H * DX), F)
+    n = int(counts.sum())
+    fidx = np.repeat(np.arange(F), counts)
+    pi, pj = rng.uniform(0, W, n), rng.uniform(0, H, n)
+    ci, cj = np.rint(pi).astype(np.int64), np.rint(pj).astype(np.int64)
+    z = rng.uniform(-3, 3, n)
+    A = np.exp(rng.uniform(np.log(1e3), np.log(2e4), n))
+    par = torch.from_numpy(np.stack([(ci - pi) * DX, (cj - pj) * DX, z, np.zeros(n)], 1).astype(np.float32)).cuda()
+    psf = torch.empty(n, Rn, Rn, device="cuda")
+    ctx.psf_forward_dev(par, psf)
+    Wp, Hp = W + 2 * Rn, H + 2 * Rn
+    frames = torch.zeros(F * Wp * Hp, device="cuda")
+    a = torch.arange(Rn, device="cuda")
+    off = (a[:, None] * Hp + a[None, :]).reshape(-1)
+    base = torch.from_numpy(fidx * Wp * Hp + (ci - h + Rn) * Hp + (cj - h + Rn)).cuda()
+    frames.index_add_(0, (base[:, None] + off[None, :]).reshape(-1),
+                      (torch.from_numpy(A).float().cuda()[:, None, None] * psf).reshape(-1))
+    frames = frames.view(F, Wp, Hp)[:, Rn:Rn + W, Rn:Rn + H]
+    bg = torch.from_numpy(rng.uniform(20, 100, F)).float().cuda()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0xF1)
+    host = torch.poisson(frames + bg[:, None, None], generator=g).cpu().numpy().astype(np.float32)
+    det = Z.Detector(ctx, W, H, -3.0, 3.05, 0.1, 0.1, 0.0)
+    det.detect(host[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lists, cnt = det.detect(host)
+    t_det = time.perf_counter() - t0
+    out = {"workload": f"{F} frames 256x256, 0.5 emitters/um^2 ({n} emitters), tetrapod 45 modes, 128x128 "
+                       f"pupil, {det.nZ} templates at 100 nm, 41x41 patches",
+           "frames": F, "emitters": n, "templates": det.nZ,
+           "detect_ms_per_frame": round(t_det / F * 1e3, 3),
+           "candidates_per_frame": round(float(np.mean(cnt)), 1)}
+    acc = det.accept_config(photon_threshold=500.0, min_z=-3.0, max_z=3.0)
+    for mode, name in ((Z.FIT_LM, "lm"), (Z.FIT_REFERENCE, "plain")):
+        fit = ctx.fit_config(mode=mode, max_iter=30 if mode == Z.FIT_LM else 20, zmin=-3.3, zmax=3.3, tol=1e-4,
+                             photon_threshold=0.0)
+        t0 = time.perf_counter()
+        locs, per, rounds = det.localize(host, fit, acc)
+        dt = time.perf_counter() - t0
+        out[name] = {"s": round(dt, 3), "frames_per_s": round(F / dt, 2),
+                     "localizations": int(len(locs)), "localizations_per_s": round(len(locs) / dt, 1),
+                     "fit_rounds": rounds}
+    det.close()
+    ctx.close()
+    return out
 
 
 def main():
@@ -276,6 +466,12 @@ def main():
 
     if not args.no_calib:
         out["calibration"] = calibration_bench(torch, Z, stream)
+    if not args.no_n128:
+        out["depth_n128"] = depth128_bench(torch, Z, stream, args.n128_batch)
+    if not args.no_mixed:
+        out["mixed_sizes"] = mixed_bench(torch, Z, stream, args.mixed_total)
+    if not args.no_frames:
+        out["frames_n128"] = frames_bench(torch, Z, stream, args.frames)
 
     if args.copy_probe_mb:
         n = args.copy_probe_mb * (1 << 20) // 4

@@ -102,7 +102,7 @@ int  zola_psf_backward_dev(zola_ctx* ctx, int B, const float* params, const floa
 #define ZOLA_ST_MAXITER   2
 #define ZOLA_ST_NAN       4
 #define ZOLA_ST_LOWPHOTON 8   /* reference photon-count break (Localization.java:597-611) */
-/* Bits 8..31 of out_status hold the number of pupil fields (64x64 inverse
+/* Bits 8..31 of out_status hold the number of pupil fields (N x N inverse
  * transforms) the fit computed for that ROI: status >> 8. */
 #define ZOLA_ST_NFIELD_SHIFT 8
 
@@ -127,6 +127,28 @@ int  zola_fit_batch(zola_ctx* ctx, const zola_fit_config* cfg, int B, const floa
 int  zola_fit_batch_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
                         const float* pixels, const double* init, double* theta,
                         double* nll, double* crlb, int* iters, int* status);
+
+/* As zola_fit_batch_dev with a focus term per ROI: zoil B floats (um) in place
+ * of cfg->zfocus, e.g. emitters from fields at different focal depths under a
+ * refractive-index mismatch.  The reference fixes one Zfocus per data set
+ * (process/cpu/Localization.java through PSFPhaseMany.computePSF's z_oil). */
+int  zola_fit_batch_zoil_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
+                             const float* pixels, const double* init, const float* zoil,
+                             double* theta, double* nll, double* crlb, int* iters, int* status);
+
+/* Fits G groups of ROIs of different sides R[g] (device pointers, one set per
+ * group; 1 <= G <= 8) in one graph launch on the context's stream: one kernel
+ * node per group, the nodes independent so the groups run concurrently.  On the
+ * 64 grid every instantiated side (17, 21, 31, 41) shares the context's model
+ * (the blur grid is 64 for all of them); on the 128 grid all groups must have
+ * the context's R.  Results are those of zola_fit_batch_dev per group.  No
+ * reference counterpart: the plugin runs one fixed patch size per run
+ * (LocalizationPipeline sizePatch); this replaces several zola_fit_batch_dev
+ * launches when patches of several sizes are fitted together. */
+int  zola_fit_mixed_dev(zola_ctx* ctx, const zola_fit_config* cfg, int G, const int* R, const int* B,
+                        const float* const* pixels, const double* const* init, double* const* theta,
+                        double* const* nll, double* const* crlb, int* const* iters,
+                        int* const* status);
 
 /* Per-ROI initial values as the reference pipeline forms them for an isolated
  * ROI: z from the best zero-shift normalised correlation against nZ templates

@@ -118,3 +118,37 @@ def test_ref128_matches_restatement(setup):
     print("ref-form |d| nm:", np.abs(d).max(axis=0), "iters gpu", res["iters"], "cpu", it_ref)
     assert both.sum() >= B // 2
     assert np.abs(d[both][:, :2]).max() <= 0.5 and np.abs(d[both][:, 2]).max() <= 1.0
+
+
+def test_lm128_per_roi_focus_index_mismatch(torch_cuda):
+    torch = torch_cuda
+    print("case: N=128 LM, n_sample 1.33, per-ROI focus U[0,4], B=8", flush=True)
+    opt = dict(_opt(), nwat=1.33)
+    coef = Z.tetrapod_coeffs(45)
+    ctx = Z.Context(Z.optics(**opt), R, coef)
+    m = O.Model(O.optics(**opt), R, coef)
+    B = 8
+    rng = np.random.default_rng(130)
+    zf = rng.uniform(0.0, 4.0, B)
+    truth = np.stack([np.exp(rng.uniform(np.log(3000), np.log(30000), B)), rng.uniform(20, 80, B),
+                      rng.uniform(-0.1, 0.1, B), rng.uniform(-0.1, 0.1, B), rng.uniform(-1.0, 1.0, B)], 1)
+    rois = np.stack([rng.poisson(t[0] * m.psf(t[2], t[3], zf[b], t[4]) + t[1]) for b, t in enumerate(truth)]).astype(np.float32)
+    init = truth.copy()
+    init[:, 4] += 0.1
+    init[:, 2] += 0.02
+    d = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda")
+    th = torch.zeros(B, 5, dtype=torch.float64, device="cuda")
+    nl = torch.zeros(B, dtype=torch.float64, device="cuda")
+    cr = torch.zeros(B, 3, dtype=torch.float64, device="cuda")
+    it = torch.zeros(B, dtype=torch.int32, device="cuda")
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.5, zmax=1.5, tol=1e-4)
+    ctx.fit_zoil_dev(cfg, d(rois, torch.float32), d(init, torch.float64), d(zf, torch.float32), th, nl, cr, it, st)
+    torch.cuda.synchronize()
+    mm = O.Model(O.optics(**opt), R, coef)
+    ref = np.array([mm.lm_fit(rois[b].astype(np.float64), init[b], zfocus=float(np.float32(zf[b])), zmin=-1.5,
+                              zmax=1.5, maxit=200, tol=1e-10)["theta"] for b in range(B)])
+    dd = (th.cpu().numpy()[:, 2:] - ref[:, 2:]) * 1000
+    print("per-ROI focus |d| nm max:", np.abs(dd).max(axis=0))
+    assert np.abs(dd[:, :2]).max() <= 0.5 and np.abs(dd[:, 2]).max() <= 1.0
+    ctx.close()

@@ -35,6 +35,12 @@ struct zola_ctx {
     double* d_Zw = nullptr;       // Zernike basis on the 32x32 window, J x 1024 (FP64)
     float* d_qbar = nullptr;      // per-PSF dL/dphase for the adjoint, grow-only
     size_t qbar_bytes = 0;
+    int* d_queue = nullptr;       // ROI counters of the persistent fits (one per concurrent group)
+    // graph of the last mixed-size fit (one kernel node per ROI-size group)
+    hipGraphExec_t mixed_exec = nullptr;
+    hipStream_t side[8] = {};
+    hipStream_t cap = nullptr;    // private capture stream (the context stream may be the legacy default)
+    hipEvent_t fork = nullptr, join[8] = {};
 };
 
 inline int set_err(zola_ctx* c, int code, const char* fmt, ...) {

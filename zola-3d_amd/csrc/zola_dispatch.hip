@@ -108,3 +108,24 @@ hipError_t zola_launch_fit128(const Model128& M, int R, int mode, const FitArgs&
     if (R != 41) return hipErrorInvalidValue;
     return launch_fit128_r<41>(M, mode, A, st);
 }
+
+namespace {
+__global__ void k_zero_ints(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+}  // namespace
+
+hipError_t zola_launch_zero_ints(int* p, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_ints, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
+int zola_persistent_blocks(const void* kernel, int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess) return 0;
+    return (per > 0 ? per : 1) * cus;
+}

@@ -32,6 +32,8 @@ struct FitArgs {
     double noise_rel;         // LM: likelihood noise tolerance, relative to A + npx*B
     const float* pixels;      // B*R*R
     const double* init;       // B*5 [A, B, x, y, z]
+    const float* zoil;        // B per-ROI focus term z_oil (um), or null: zfocus for all
+    int* queue;               // persistent form: ROI counter (zeroed before launch), or null
     double* theta;            // B*5
     double* nll;              // B
     double* crlb;             // B*3
@@ -326,6 +328,7 @@ struct LmLds {
     typename SY::Shared T;
     LmState L[WPB];
     typename SY::Red red[WPB / SY::kWaves];
+    int q[WPB / SY::kWaves];
     alignas(16) float S[WPB / SY::kWaves][SY::kLmFloats];
 };
 
@@ -333,8 +336,26 @@ template <class SY, int WPB>
 struct RefLds {
     typename SY::Shared T;
     typename SY::Red red[WPB / SY::kWaves];
+    int q[WPB / SY::kWaves];
     alignas(16) float S[WPB / SY::kWaves][SY::kRefFloats];
 };
+
+// Next ROI of a persistent fit: the ROI's first wave takes it from the queue,
+// the other waves of the ROI (N = 128: one block) read it through LDS.
+template <class SY>
+__device__ __forceinline__ int next_roi(int* queue, int& slot, int lane, int wr) {
+    if constexpr (SY::kWaves == 1) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(queue, 1);
+        return __builtin_amdgcn_readfirstlane(v);
+    } else {
+        if (lane == 0 && wr == 0) slot = atomicAdd(queue, 1);
+        __syncthreads();
+        const int v = slot;
+        __syncthreads();
+        return v;
+    }
+}
 
 template <class SY, int R, int WPB>
 __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const FitArgs& A, LmLds<SY, WPB>& lds) {
@@ -345,192 +366,197 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int wr = w % SY::kWaves, slot = w / SY::kWaves;
-    const int b = blockIdx.x * (WPB / SY::kWaves) + slot;
-    if (b >= A.B) return;
-    float* scr = lds.S[slot];
-    auto& red = lds.red[slot];
-    LmState& L = lds.L[w];
-    const float* pix = A.pixels + (size_t)b * R * R;
-    const float zoil = (float)A.zfocus;
-    // Wave-uniform FP64 state lives in LDS between uses; registers are kept
-    // for the transforms.  Damping follows Nielsen's rule: on success
-    // lambda *= max(1/3, 1-(2 rho-1)^3) with rho the ratio of actual to
-    // predicted decrease; on failure lambda *= nu, nu *= 2.  A trial is
-    // accepted when the likelihood does not rise by more than the FP32
-    // synthesis noise (noise_rel * (A + npx B)).
-#pragma unroll
-    for (int p = 0; p < 5; p++) L.th[p] = A.init[(size_t)b * 5 + p];
-    L.th[0] = fmax(L.th[0], 1.0);
-    L.th[1] = fmax(L.th[1], 0.01);
-#pragma unroll
-    for (int p = 0; p < 5; p++) L.tn[p] = L.th[p];
-    L.nll = 0.0;
-    L.lambda = 1e-3;
-    L.nu = 2.0;
-    L.pred = 0.0;
-    wave_lds_fence();
-    const double dxmax = (double)A.xystep;
-    int it = 0, tries = 0, conv = 0, nan = 0, nfield = 0;
-    bool first = true;
-    while (true) {
-        bool eval_now = true;
-        if (!first) {
-            double Am[25], d[5];
-#pragma unroll
-            for (int p = 0; p < 5; p++) {
-#pragma unroll
-                for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
-                Am[p * 5 + p] *= (1.0 + L.lambda);
-                d[p] = -L.gr[p];
-            }
-            if (!solve5(Am, d)) {
-                L.lambda *= L.nu;
-                L.nu *= 2;
-                eval_now = false;
-            } else {
-                double pred = 0.0;
-#pragma unroll
+    // static form: one ROI per wave slot; persistent form: ROIs from the queue
+    int b = blockIdx.x * (WPB / SY::kWaves) + slot;
+    if (A.queue) b = next_roi<SY>(A.queue, lds.q[slot], lane, wr);
+    while (b < A.B) {
+        float* scr = lds.S[slot];
+        auto& red = lds.red[slot];
+        LmState& L = lds.L[w];
+        const float* pix = A.pixels + (size_t)b * R * R;
+        const float zoil = A.zoil ? A.zoil[b] : (float)A.zfocus;
+        // Wave-uniform FP64 state lives in LDS between uses; registers are kept
+        // for the transforms.  Damping follows Nielsen's rule: on success
+        // lambda *= max(1/3, 1-(2 rho-1)^3) with rho the ratio of actual to
+        // predicted decrease; on failure lambda *= nu, nu *= 2.  A trial is
+        // accepted when the likelihood does not rise by more than the FP32
+        // synthesis noise (noise_rel * (A + npx B)).
+    #pragma unroll
+        for (int p = 0; p < 5; p++) L.th[p] = A.init[(size_t)b * 5 + p];
+        L.th[0] = fmax(L.th[0], 1.0);
+        L.th[1] = fmax(L.th[1], 0.01);
+    #pragma unroll
+        for (int p = 0; p < 5; p++) L.tn[p] = L.th[p];
+        L.nll = 0.0;
+        L.lambda = 1e-3;
+        L.nu = 2.0;
+        L.pred = 0.0;
+        wave_lds_fence();
+        const double dxmax = (double)A.xystep;
+        int it = 0, tries = 0, conv = 0, nan = 0, nfield = 0;
+        bool first = true;
+        while (true) {
+            bool eval_now = true;
+            if (!first) {
+                double Am[25], d[5];
+    #pragma unroll
                 for (int p = 0; p < 5; p++) {
-                    double hd = 0.0;
-#pragma unroll
-                    for (int q = 0; q < 5; q++) hd += hget(L.H, p, q) * d[q];
-                    pred -= L.gr[p] * d[p] + 0.5 * d[p] * hd;
+    #pragma unroll
+                    for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
+                    Am[p * 5 + p] *= (1.0 + L.lambda);
+                    d[p] = -L.gr[p];
                 }
-                L.pred = pred;
-                double tn[5];
-#pragma unroll
-                for (int p = 0; p < 5; p++) tn[p] = L.th[p] + d[p];
-#pragma unroll
-                for (int p = 2; p < 4; p++) {
-                    if (d[p] > dxmax) tn[p] = L.th[p] + dxmax;
-                    if (d[p] < -dxmax) tn[p] = L.th[p] - dxmax;
-                }
-                if (d[4] > 0.2) tn[4] = L.th[4] + 0.2;
-                if (d[4] < -0.2) tn[4] = L.th[4] - 0.2;
-                tn[0] = fmax(tn[0], 1.0);
-                tn[1] = fmax(tn[1], 0.01);
-                tn[4] = fmin(fmax(tn[4], A.zmin), A.zmax);
-#pragma unroll
-                for (int p = 0; p < 5; p++) L.tn[p] = tn[p];
-            }
-            wave_lds_fence();
-        }
-        bool accepted = false;
-        if (eval_now) {
-            // Opaque copy of the lane id: keeps the compiler from hoisting the
-            // hundreds of per-lane LDS addresses of one evaluation out of the
-            // iteration loop (they would stay live across it and spill).
-            int lane_it = lane;
-            __asm__ __volatile__("" : "+v"(lane_it));
-            float vB[RPL], vX[RPL], vY[RPL], vZ[RPL];
-            SY::template lm_eval<R>(M, T, scr, lane_it, wr, (float)L.tn[2], (float)L.tn[3], (float)L.tn[4],
-                                    zoil, vB, vX, vY, vZ);
-            nfield += 2;
-            float psf[RPL], n[RPL];
-#pragma unroll
-            for (int t = 0; t < RPL; t++)
-                n[t] = SY::template valid<R>(lane, wr, t) ? pix[SY::template pix<R>(lane, wr, t)] : 0.f;
-            const float Sn = normalise<SY, R>(lane, wr, red, vB, psf);
-            const double tA = L.tn[0], tB = L.tn[1];
-            const double nn = nll64<SY, R>(lane, wr, red, psf, n, tA, tB);
-            if (nn != nn) nan = 1;
-            const double noise = A.noise_rel * (L.th[0] + (double)(R * R) * L.th[1]);
-            accepted = first || nn <= L.nll + noise;
-            if (!accepted) {
-                L.lambda *= L.nu;
-                L.nu *= 2;
-            } else {
-                if (!first) {
-                    const double step = fmax(fabs(L.tn[2] - L.th[2]),
-                                             fmax(fabs(L.tn[3] - L.th[3]), fabs(L.tn[4] - L.th[4])));
-                    const double rho = L.pred > 0.0 ? (L.nll - nn) / L.pred : 0.0;
-                    const double t3 = 2.0 * rho - 1.0;
-                    L.lambda *= fmax(1.0 / 3.0, 1.0 - t3 * t3 * t3);
-                    L.nu = 2.0;
-                    if (step < A.tol) conv = 1;
-                }
-#pragma unroll
-                for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
-                L.nll = nn;
-                // d psf = (sgn dv - psf * sum(sgn dv)) / S for the three tangents
-                float ss[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-                for (int t = 0; t < RPL; t++) {
-                    const float sg = vB[t] < 0.f ? -1.f : 1.f;
-                    vX[t] *= sg;
-                    vY[t] *= sg;
-                    vZ[t] *= sg;
-                    if (SY::template valid<R>(lane, wr, t)) {
-                        ss[0] += vX[t];
-                        ss[1] += vY[t];
-                        ss[2] += vZ[t];
-                    }
-                }
-                SY::template sumv<3>(ss, red, lane, wr);
-                const float sx = ss[0], sy = ss[1], sz = ss[2];
-                const float inv = 1.f / Sn;
-                double acc[20];
-#pragma unroll
-                for (int k = 0; k < 20; k++) acc[k] = 0.0;
-#pragma unroll
-                for (int t = 0; t < RPL; t++) {
-                    if (!SY::template valid<R>(lane, wr, t)) continue;
-                    const double mu = (double)psf[t] * tA + tB;
-                    const double J[5] = {(double)psf[t], 1.0,
-                                         tA * (double)((vX[t] - psf[t] * sx) * inv),
-                                         tA * (double)((vY[t] - psf[t] * sy) * inv),
-                                         tA * (double)((vZ[t] - psf[t] * sz) * inv)};
-                    const double wg = 1.0 - (double)n[t] / mu, wh = 1.0 / mu;
-                    int k = 5;
-#pragma unroll
+                if (!solve5(Am, d)) {
+                    L.lambda *= L.nu;
+                    L.nu *= 2;
+                    eval_now = false;
+                } else {
+                    double pred = 0.0;
+    #pragma unroll
                     for (int p = 0; p < 5; p++) {
-                        acc[p] += wg * J[p];
-#pragma unroll
-                        for (int q = p; q < 5; q++) acc[k++] += wh * J[p] * J[q];
+                        double hd = 0.0;
+    #pragma unroll
+                        for (int q = 0; q < 5; q++) hd += hget(L.H, p, q) * d[q];
+                        pred -= L.gr[p] * d[p] + 0.5 * d[p] * hd;
                     }
+                    L.pred = pred;
+                    double tn[5];
+    #pragma unroll
+                    for (int p = 0; p < 5; p++) tn[p] = L.th[p] + d[p];
+    #pragma unroll
+                    for (int p = 2; p < 4; p++) {
+                        if (d[p] > dxmax) tn[p] = L.th[p] + dxmax;
+                        if (d[p] < -dxmax) tn[p] = L.th[p] - dxmax;
+                    }
+                    if (d[4] > 0.2) tn[4] = L.th[4] + 0.2;
+                    if (d[4] < -0.2) tn[4] = L.th[4] - 0.2;
+                    tn[0] = fmax(tn[0], 1.0);
+                    tn[1] = fmax(tn[1], 0.01);
+                    tn[4] = fmin(fmax(tn[4], A.zmin), A.zmax);
+    #pragma unroll
+                    for (int p = 0; p < 5; p++) L.tn[p] = tn[p];
                 }
-                SY::template sumv<20>(acc, red, lane, wr);
-#pragma unroll
-                for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
-#pragma unroll
-                for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
+                wave_lds_fence();
             }
-            wave_lds_fence();
+            bool accepted = false;
+            if (eval_now) {
+                // Opaque copy of the lane id: keeps the compiler from hoisting the
+                // hundreds of per-lane LDS addresses of one evaluation out of the
+                // iteration loop (they would stay live across it and spill).
+                int lane_it = lane;
+                __asm__ __volatile__("" : "+v"(lane_it));
+                float vB[RPL], vX[RPL], vY[RPL], vZ[RPL];
+                SY::template lm_eval<R>(M, T, scr, lane_it, wr, (float)L.tn[2], (float)L.tn[3], (float)L.tn[4],
+                                        zoil, vB, vX, vY, vZ);
+                nfield += 2;
+                float psf[RPL], n[RPL];
+    #pragma unroll
+                for (int t = 0; t < RPL; t++)
+                    n[t] = SY::template valid<R>(lane, wr, t) ? pix[SY::template pix<R>(lane, wr, t)] : 0.f;
+                const float Sn = normalise<SY, R>(lane, wr, red, vB, psf);
+                const double tA = L.tn[0], tB = L.tn[1];
+                const double nn = nll64<SY, R>(lane, wr, red, psf, n, tA, tB);
+                if (nn != nn) nan = 1;
+                const double noise = A.noise_rel * (L.th[0] + (double)(R * R) * L.th[1]);
+                accepted = first || nn <= L.nll + noise;
+                if (!accepted) {
+                    L.lambda *= L.nu;
+                    L.nu *= 2;
+                } else {
+                    if (!first) {
+                        const double step = fmax(fabs(L.tn[2] - L.th[2]),
+                                                 fmax(fabs(L.tn[3] - L.th[3]), fabs(L.tn[4] - L.th[4])));
+                        const double rho = L.pred > 0.0 ? (L.nll - nn) / L.pred : 0.0;
+                        const double t3 = 2.0 * rho - 1.0;
+                        L.lambda *= fmax(1.0 / 3.0, 1.0 - t3 * t3 * t3);
+                        L.nu = 2.0;
+                        if (step < A.tol) conv = 1;
+                    }
+    #pragma unroll
+                    for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
+                    L.nll = nn;
+                    // d psf = (sgn dv - psf * sum(sgn dv)) / S for the three tangents
+                    float ss[3] = {0.f, 0.f, 0.f};
+    #pragma unroll
+                    for (int t = 0; t < RPL; t++) {
+                        const float sg = vB[t] < 0.f ? -1.f : 1.f;
+                        vX[t] *= sg;
+                        vY[t] *= sg;
+                        vZ[t] *= sg;
+                        if (SY::template valid<R>(lane, wr, t)) {
+                            ss[0] += vX[t];
+                            ss[1] += vY[t];
+                            ss[2] += vZ[t];
+                        }
+                    }
+                    SY::template sumv<3>(ss, red, lane, wr);
+                    const float sx = ss[0], sy = ss[1], sz = ss[2];
+                    const float inv = 1.f / Sn;
+                    double acc[20];
+    #pragma unroll
+                    for (int k = 0; k < 20; k++) acc[k] = 0.0;
+    #pragma unroll
+                    for (int t = 0; t < RPL; t++) {
+                        if (!SY::template valid<R>(lane, wr, t)) continue;
+                        const double mu = (double)psf[t] * tA + tB;
+                        const double J[5] = {(double)psf[t], 1.0,
+                                             tA * (double)((vX[t] - psf[t] * sx) * inv),
+                                             tA * (double)((vY[t] - psf[t] * sy) * inv),
+                                             tA * (double)((vZ[t] - psf[t] * sz) * inv)};
+                        const double wg = 1.0 - (double)n[t] / mu, wh = 1.0 / mu;
+                        int k = 5;
+    #pragma unroll
+                        for (int p = 0; p < 5; p++) {
+                            acc[p] += wg * J[p];
+    #pragma unroll
+                            for (int q = p; q < 5; q++) acc[k++] += wh * J[p] * J[q];
+                        }
+                    }
+                    SY::template sumv<20>(acc, red, lane, wr);
+    #pragma unroll
+                    for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
+    #pragma unroll
+                    for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
+                }
+                wave_lds_fence();
+            }
+            if (first) {
+                first = false;
+                if (A.max_iter <= 0) break;
+                continue;
+            }
+            tries++;
+            if (accepted || tries >= 12) {
+                if (!accepted) conv = 1;
+                it++;
+                tries = 0;
+                if (conv || it >= A.max_iter) break;
+            }
         }
-        if (first) {
-            first = false;
-            if (A.max_iter <= 0) break;
-            continue;
+        // CRLB of x, y, z from the Fisher matrix at the optimum
+        double cr[3];
+    #pragma unroll
+        for (int c = 2; c < 5; c++) {
+            double Am[25], e[5];
+    #pragma unroll
+            for (int p = 0; p < 5; p++) {
+    #pragma unroll
+                for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
+                e[p] = (p == c) ? 1.0 : 0.0;
+            }
+            cr[c - 2] = solve5(Am, e) ? sqrt(e[c]) : (double)NAN;
         }
-        tries++;
-        if (accepted || tries >= 12) {
-            if (!accepted) conv = 1;
-            it++;
-            tries = 0;
-            if (conv || it >= A.max_iter) break;
+        if (lane == 0 && wr == 0) {
+    #pragma unroll
+            for (int p = 0; p < 5; p++) A.theta[(size_t)b * 5 + p] = L.th[p];
+            A.nll[b] = L.nll;
+    #pragma unroll
+            for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = cr[c];
+            A.iters[b] = it;
+            A.status[b] = (conv ? kStConverged : kStMaxIter) | (nan ? kStNaN : 0) | (nfield << 8);
         }
-    }
-    // CRLB of x, y, z from the Fisher matrix at the optimum
-    double cr[3];
-#pragma unroll
-    for (int c = 2; c < 5; c++) {
-        double Am[25], e[5];
-#pragma unroll
-        for (int p = 0; p < 5; p++) {
-#pragma unroll
-            for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
-            e[p] = (p == c) ? 1.0 : 0.0;
-        }
-        cr[c - 2] = solve5(Am, e) ? sqrt(e[c]) : (double)NAN;
-    }
-    if (lane == 0 && wr == 0) {
-#pragma unroll
-        for (int p = 0; p < 5; p++) A.theta[(size_t)b * 5 + p] = L.th[p];
-        A.nll[b] = L.nll;
-#pragma unroll
-        for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = cr[c];
-        A.iters[b] = it;
-        A.status[b] = (conv ? kStConverged : kStMaxIter) | (nan ? kStNaN : 0) | (nfield << 8);
+        if (!A.queue) break;
+        b = next_roi<SY>(A.queue, lds.q[slot], lane, wr);
     }
 }
 
@@ -612,288 +638,293 @@ __device__ __forceinline__ void fit_ref_body(const typename SY::Model& M, const 
     constexpr int RPL = SY::template RPL<R>;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int wr = w % SY::kWaves, slot = w / SY::kWaves;
-    const int b = blockIdx.x * (WPB / SY::kWaves) + slot;
-    if (b >= A.B) return;
-    float* scr = lds.S[slot];
-    auto& red = lds.red[slot];
-    float n[RPL];
-#pragma unroll
-    for (int t = 0; t < RPL; t++)
-        n[t] = SY::template valid<R>(lane, wr, t) ? A.pixels[(size_t)b * R * R + SY::template pix<R>(lane, wr, t)] : 0.f;
-    const float zoil = (float)A.zfocus;
-    double tA = A.init[(size_t)b * 5 + 0], tB = A.init[(size_t)b * 5 + 1];
-    double th[3] = {A.init[(size_t)b * 5 + 2], A.init[(size_t)b * 5 + 3], A.init[(size_t)b * 5 + 4]};
-    const double h = .0005, eps = .0001, maxShift = (double)A.xystep / 2.;  // Localization.java:29-31,115
+    // static form: one ROI per wave slot; persistent form: ROIs from the queue
+    int b = blockIdx.x * (WPB / SY::kWaves) + slot;
+    if (A.queue) b = next_roi<SY>(A.queue, lds.q[slot], lane, wr);
+    while (b < A.B) {
+        float* scr = lds.S[slot];
+        auto& red = lds.red[slot];
+        float n[RPL];
+    #pragma unroll
+        for (int t = 0; t < RPL; t++)
+            n[t] = SY::template valid<R>(lane, wr, t) ? A.pixels[(size_t)b * R * R + SY::template pix<R>(lane, wr, t)] : 0.f;
+        const float zoil = A.zoil ? A.zoil[b] : (float)A.zfocus;
+        double tA = A.init[(size_t)b * 5 + 0], tB = A.init[(size_t)b * 5 + 1];
+        double th[3] = {A.init[(size_t)b * 5 + 2], A.init[(size_t)b * 5 + 3], A.init[(size_t)b * 5 + 4]};
+        const double h = .0005, eps = .0001, maxShift = (double)A.xystep / 2.;  // Localization.java:29-31,115
 
-    // psf at the current position (cur) and the last synthesised one (tmp)
-    float cur[RPL], tmp[RPL], hi[RPL], dd[3][RPL];
-    double pc[3] = {0, 0, 0};
-    bool hc = false;
-    double req[3] = {0, 0, 0};
-    bool want = false;
+        // psf at the current position (cur) and the last synthesised one (tmp)
+        float cur[RPL], tmp[RPL], hi[RPL], dd[3][RPL];
+        double pc[3] = {0, 0, 0};
+        bool hc = false;
+        double req[3] = {0, 0, 0};
+        bool want = false;
 
-    int st = kRsInit, which = 0, k = 0, it = 0, nbIter = A.max_iter, nfield = 0;
-    int brk = 0, ok = 1, nan = 0, conv = 0, initial = 1;
-    double likelihood = 0, save = 0, l1 = 0, l3 = 0, grad = 0, prevLik = 0;
-    double oldX = 1.7976931348623157e308, oldY = oldX, oldZ = oldX;
-    auto at_theta = [&]() { return hc && pc[0] == th[0] && pc[1] == th[1] && pc[2] == th[2]; };
-    auto L = [&](const float (&ps)[RPL], double a, double bb) { return nll64<SY, R>(lane, wr, red, ps, n, a, bb); };
-    // min_amp (929-977 for A with h*1000, 997-1058 for B with h*10), on the cached psf
-    auto min_amp = [&](bool isB) {
-        const double hh = isB ? h * 10 : h * 1000;
-        double& t = isB ? tB : tA;
-        double sv = t;
-        double a1, a2, a3;
-        if (!isB) { a1 = L(cur, tA - hh, tB); a2 = L(cur, tA, tB); a3 = L(cur, tA + hh, tB); }
-        else { a1 = L(cur, tA, tB - hh); a2 = L(cur, tA, tB); a3 = L(cur, tA, tB + hh); t = sv; }
-        double gd;
-        if (fabs((a3 + a1 - 2 * a2) / (hh * hh)) == 0) gd = (a3 - a1) * (2 * hh);
-        else gd = ((a3 - a1) / (2 * hh)) / fabs((a3 + a1 - 2 * a2) / (hh * hh));
-        for (double gamma = 1; gamma > .02; gamma /= 10) {
-            t -= gamma * gd;
-            const double lk = L(cur, tA, tB);
-            if (lk < a2) { a2 = lk; sv = t; }
-            else t = sv;
-        }
-        return L(cur, tA, tB);
-    };
-
-    while (st != kRsDone) {
-        if (want) {
-            float v[RPL];
-            SY::template eval<R>(M, T, scr, lane, wr, (float)req[0], (float)req[1], (float)req[2], zoil, v);
-            normalise<SY, R>(lane, wr, red, v, tmp);
-            want = false;
-            nfield++;
-        }
-        if (st == kRsInit || st == kRsAmpA || st == kRsAmpB) {
-            if (!at_theta()) {
-                if (k == 1) {  // tmp holds psf(theta)
-#pragma unroll
-                    for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) pc[c] = th[c];
-                    hc = true;
-                    k = 0;
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) req[c] = th[c];
-                    want = true;
-                    k = 1;
-                    continue;
-                }
+        int st = kRsInit, which = 0, k = 0, it = 0, nbIter = A.max_iter, nfield = 0;
+        int brk = 0, ok = 1, nan = 0, conv = 0, initial = 1;
+        double likelihood = 0, save = 0, l1 = 0, l3 = 0, grad = 0, prevLik = 0;
+        double oldX = 1.7976931348623157e308, oldY = oldX, oldZ = oldX;
+        auto at_theta = [&]() { return hc && pc[0] == th[0] && pc[1] == th[1] && pc[2] == th[2]; };
+        auto L = [&](const float (&ps)[RPL], double a, double bb) { return nll64<SY, R>(lane, wr, red, ps, n, a, bb); };
+        // min_amp (929-977 for A with h*1000, 997-1058 for B with h*10), on the cached psf
+        auto min_amp = [&](bool isB) {
+            const double hh = isB ? h * 10 : h * 1000;
+            double& t = isB ? tB : tA;
+            double sv = t;
+            double a1, a2, a3;
+            if (!isB) { a1 = L(cur, tA - hh, tB); a2 = L(cur, tA, tB); a3 = L(cur, tA + hh, tB); }
+            else { a1 = L(cur, tA, tB - hh); a2 = L(cur, tA, tB); a3 = L(cur, tA, tB + hh); t = sv; }
+            double gd;
+            if (fabs((a3 + a1 - 2 * a2) / (hh * hh)) == 0) gd = (a3 - a1) * (2 * hh);
+            else gd = ((a3 - a1) / (2 * hh)) / fabs((a3 + a1 - 2 * a2) / (hh * hh));
+            for (double gamma = 1; gamma > .02; gamma /= 10) {
+                t -= gamma * gd;
+                const double lk = L(cur, tA, tB);
+                if (lk < a2) { a2 = lk; sv = t; }
+                else t = sv;
             }
-            if (st == kRsInit) {
-                tA = fmax(1.0, tA);
-                tB = fmax(.01, tB);
-                likelihood = L(cur, tA, tB);
-                st = kRsAmpA;
-            } else if (st == kRsAmpA) {
-                likelihood = min_amp(false);
-                tA = fmax(1.0, tA);
-                tB = fmax(.01, tB);
-                st = kRsAmpB;
-            } else {
-                likelihood = min_amp(true);
-                tA = fmax(1.0, tA);
-                tB = fmax(.01, tB);
-                if (initial) {
-                    initial = 0;
-                    it = 0;
-                    if (A.max_iter <= 0) { st = kRsFisher; k = 0; continue; }
-                    st = kRsPos; which = 0; k = 0;
-                } else {
-                    st = kRsCheck;
-                }
+            return L(cur, tA, tB);
+        };
+
+        while (st != kRsDone) {
+            if (want) {
+                float v[RPL];
+                SY::template eval<R>(M, T, scr, lane, wr, (float)req[0], (float)req[1], (float)req[2], zoil, v);
+                normalise<SY, R>(lane, wr, red, v, tmp);
+                want = false;
+                nfield++;
             }
-            continue;
-        }
-        if (st == kRsPos) {
-            // minimizeZ / minimizeX / minimizeY (672-906); which: 0 = z, 1 = x, 2 = y
-            const int ci = which == 0 ? 2 : which - 1;
-            if (k == 0) {
-                save = th[ci];
-#pragma unroll
-                for (int c = 0; c < 3; c++) req[c] = th[c];
-                req[ci] = th[ci] - h;
-                want = true;
-                k = 1;
-            } else if (k == 1) {
-                l1 = L(tmp, tA, tB);
-#pragma unroll
-                for (int c = 0; c < 3; c++) req[c] = th[c];
-                req[ci] = th[ci] + h;
-                want = true;
-                k = 2;
-            } else if (k == 2) {
-                l3 = L(tmp, tA, tB);
-                const double l2 = likelihood;
-                if (fabs((l3 + l1 - 2 * l2) / (h * h)) == 0) grad = (l3 - l1) * (2 * h);
-                else grad = ((l3 - l1) / (2 * h)) / fabs((l3 + l1 - 2 * l2) / (h * h));
-                if (grad > maxShift) grad = maxShift;
-                else if (grad < -maxShift) grad = -maxShift;
-                prevLik = likelihood;
-                th[ci] -= 1.0 * grad;
-#pragma unroll
-                for (int c = 0; c < 3; c++) req[c] = th[c];
-                want = true;
-                k = 3;
-            } else {
-                const double lik = L(tmp, tA, tB);
-                bool done = false;
-                if (lik < likelihood) {
-                    prevLik = lik;
-#pragma unroll
-                    for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) pc[c] = th[c];
-                    hc = true;
-                    done = true;
-                } else {
-                    th[ci] = save;
-                    if (k == 3) {
-                        th[ci] -= (1.0 / 10) * grad;
-#pragma unroll
+            if (st == kRsInit || st == kRsAmpA || st == kRsAmpB) {
+                if (!at_theta()) {
+                    if (k == 1) {  // tmp holds psf(theta)
+    #pragma unroll
+                        for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
+    #pragma unroll
+                        for (int c = 0; c < 3; c++) pc[c] = th[c];
+                        hc = true;
+                        k = 0;
+                    } else {
+    #pragma unroll
                         for (int c = 0; c < 3; c++) req[c] = th[c];
                         want = true;
-                        k = 4;
+                        k = 1;
+                        continue;
+                    }
+                }
+                if (st == kRsInit) {
+                    tA = fmax(1.0, tA);
+                    tB = fmax(.01, tB);
+                    likelihood = L(cur, tA, tB);
+                    st = kRsAmpA;
+                } else if (st == kRsAmpA) {
+                    likelihood = min_amp(false);
+                    tA = fmax(1.0, tA);
+                    tB = fmax(.01, tB);
+                    st = kRsAmpB;
+                } else {
+                    likelihood = min_amp(true);
+                    tA = fmax(1.0, tA);
+                    tB = fmax(.01, tB);
+                    if (initial) {
+                        initial = 0;
+                        it = 0;
+                        if (A.max_iter <= 0) { st = kRsFisher; k = 0; continue; }
+                        st = kRsPos; which = 0; k = 0;
                     } else {
-                        done = true;
+                        st = kRsCheck;
                     }
                 }
-                if (done) {
-                    likelihood = prevLik;
-                    if (which == 0) {
-                        if (th[2] < A.zmin) th[2] = A.zmin;
-                        if (th[2] > A.zmax) th[2] = A.zmax;
-                    }
-                    which++;
-                    k = 0;
-                    if (which == 3) st = kRsAmpA;
-                }
-            }
-            continue;
-        }
-        if (st == kRsCheck) {
-            const double distance = sqrt((th[0] - oldX) * (th[0] - oldX) + (th[1] - oldY) * (th[1] - oldY) +
-                                         (th[2] - oldZ) * (th[2] - oldZ));
-            int breaking = 0;
-            if (it > 4) breaking = !(tA > A.thr);
-            if (breaking) brk = 1;
-            if (distance < eps || breaking) {
-                nbIter = it + 1;
-                conv = !breaking;
-                st = kRsFisher;
-                k = 0;
                 continue;
             }
-            if (likelihood != likelihood) {
-                ok = 0;
-                nan = 1;
+            if (st == kRsPos) {
+                // minimizeZ / minimizeX / minimizeY (672-906); which: 0 = z, 1 = x, 2 = y
+                const int ci = which == 0 ? 2 : which - 1;
+                if (k == 0) {
+                    save = th[ci];
+    #pragma unroll
+                    for (int c = 0; c < 3; c++) req[c] = th[c];
+                    req[ci] = th[ci] - h;
+                    want = true;
+                    k = 1;
+                } else if (k == 1) {
+                    l1 = L(tmp, tA, tB);
+    #pragma unroll
+                    for (int c = 0; c < 3; c++) req[c] = th[c];
+                    req[ci] = th[ci] + h;
+                    want = true;
+                    k = 2;
+                } else if (k == 2) {
+                    l3 = L(tmp, tA, tB);
+                    const double l2 = likelihood;
+                    if (fabs((l3 + l1 - 2 * l2) / (h * h)) == 0) grad = (l3 - l1) * (2 * h);
+                    else grad = ((l3 - l1) / (2 * h)) / fabs((l3 + l1 - 2 * l2) / (h * h));
+                    if (grad > maxShift) grad = maxShift;
+                    else if (grad < -maxShift) grad = -maxShift;
+                    prevLik = likelihood;
+                    th[ci] -= 1.0 * grad;
+    #pragma unroll
+                    for (int c = 0; c < 3; c++) req[c] = th[c];
+                    want = true;
+                    k = 3;
+                } else {
+                    const double lik = L(tmp, tA, tB);
+                    bool done = false;
+                    if (lik < likelihood) {
+                        prevLik = lik;
+    #pragma unroll
+                        for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
+    #pragma unroll
+                        for (int c = 0; c < 3; c++) pc[c] = th[c];
+                        hc = true;
+                        done = true;
+                    } else {
+                        th[ci] = save;
+                        if (k == 3) {
+                            th[ci] -= (1.0 / 10) * grad;
+    #pragma unroll
+                            for (int c = 0; c < 3; c++) req[c] = th[c];
+                            want = true;
+                            k = 4;
+                        } else {
+                            done = true;
+                        }
+                    }
+                    if (done) {
+                        likelihood = prevLik;
+                        if (which == 0) {
+                            if (th[2] < A.zmin) th[2] = A.zmin;
+                            if (th[2] > A.zmax) th[2] = A.zmax;
+                        }
+                        which++;
+                        k = 0;
+                        if (which == 3) st = kRsAmpA;
+                    }
+                }
+                continue;
+            }
+            if (st == kRsCheck) {
+                const double distance = sqrt((th[0] - oldX) * (th[0] - oldX) + (th[1] - oldY) * (th[1] - oldY) +
+                                             (th[2] - oldZ) * (th[2] - oldZ));
+                int breaking = 0;
+                if (it > 4) breaking = !(tA > A.thr);
+                if (breaking) brk = 1;
+                if (distance < eps || breaking) {
+                    nbIter = it + 1;
+                    conv = !breaking;
+                    st = kRsFisher;
+                    k = 0;
+                    continue;
+                }
+                if (likelihood != likelihood) {
+                    ok = 0;
+                    nan = 1;
+                    st = kRsDone;
+                    continue;
+                }
+                oldX = th[0]; oldY = th[1]; oldZ = th[2];
+                it++;
+                if (it >= A.max_iter) {
+                    nbIter = A.max_iter;
+                    st = kRsFisher;
+                    k = 0;
+                } else {
+                    st = kRsPos; which = 0; k = 0;
+                }
+                continue;
+            }
+            if (st == kRsFisher) {
+                // computeFisher (317-403) with the reference's steps: probes k = 0..5
+                // at x+h, x-h, y+h, y-h, z+h, z-h; then the centre (cached psf).
+                if (k >= 1 && k <= 6) {
+                    const int q = (k - 1) / 2;
+                    if ((k - 1) % 2 == 0) {
+    #pragma unroll
+                        for (int t = 0; t < RPL; t++) hi[t] = tmp[t];
+                    } else {
+    #pragma unroll
+                        for (int t = 0; t < RPL; t++) {
+                            const double f2 = (double)hi[t] * tA + tB, f0 = (double)tmp[t] * tA + tB;
+                            const float d = (float)((f2 - f0) / (2 * fabs(h)));
+                            dd[0][t] = q == 0 ? d : dd[0][t];
+                            dd[1][t] = q == 1 ? d : dd[1][t];
+                            dd[2][t] = q == 2 ? d : dd[2][t];
+                        }
+                    }
+                }
+                if (k < 6) {
+                    const int q = k / 2;
+                    const double sgnh = (k % 2 == 0) ? h : -h;
+                    req[0] = th[0] + (q == 0 ? sgnh : 0.0);
+                    req[1] = th[1] + (q == 1 ? sgnh : 0.0);
+                    req[2] = th[2] + (q == 2 ? sgnh : 0.0);
+                    want = true;
+                    k++;
+                    continue;
+                }
+                if (!at_theta()) {
+                    if (k == 7) {
+    #pragma unroll
+                        for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
+    #pragma unroll
+                        for (int c = 0; c < 3; c++) pc[c] = th[c];
+                        hc = true;
+                    } else {
+    #pragma unroll
+                        for (int c = 0; c < 3; c++) req[c] = th[c];
+                        want = true;
+                        k = 7;
+                        continue;
+                    }
+                }
+                double acc[15];
+    #pragma unroll
+                for (int kk = 0; kk < 15; kk++) acc[kk] = 0.0;
+    #pragma unroll
+                for (int t = 0; t < RPL; t++) {
+                    if (!SY::template valid<R>(lane, wr, t)) continue;
+                    const double f1 = (double)cur[t] * tA + tB;
+                    // x, y, z, then A (the reference differentiates the model f1) and B
+                    const double J[5] = {(double)dd[0][t], (double)dd[1][t], (double)dd[2][t], f1, 1.0};
+                    int kk = 0;
+    #pragma unroll
+                    for (int p = 0; p < 5; p++)
+    #pragma unroll
+                        for (int q = p; q < 5; q++) acc[kk++] += (1 / f1) * J[p] * J[q];
+                }
+                SY::template sumv<15>(acc, red, lane, wr);
+                double F[25], inv[25];
+                int kk = 0;
+    #pragma unroll
+                for (int p = 0; p < 5; p++)
+    #pragma unroll
+                    for (int q = p; q < 5; q++) { F[p * 5 + q] = acc[kk]; F[q * 5 + p] = acc[kk]; kk++; }
+                likelihood = L(cur, tA, tB);
+                inverse_cofactor5(F, inv);
+                if (lane == 0 && wr == 0) {
+    #pragma unroll
+                    for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = sqrt(inv[c * 5 + c]);
+                }
                 st = kRsDone;
                 continue;
             }
-            oldX = th[0]; oldY = th[1]; oldZ = th[2];
-            it++;
-            if (it >= A.max_iter) {
-                nbIter = A.max_iter;
-                st = kRsFisher;
-                k = 0;
-            } else {
-                st = kRsPos; which = 0; k = 0;
-            }
-            continue;
         }
-        if (st == kRsFisher) {
-            // computeFisher (317-403) with the reference's steps: probes k = 0..5
-            // at x+h, x-h, y+h, y-h, z+h, z-h; then the centre (cached psf).
-            if (k >= 1 && k <= 6) {
-                const int q = (k - 1) / 2;
-                if ((k - 1) % 2 == 0) {
-#pragma unroll
-                    for (int t = 0; t < RPL; t++) hi[t] = tmp[t];
-                } else {
-#pragma unroll
-                    for (int t = 0; t < RPL; t++) {
-                        const double f2 = (double)hi[t] * tA + tB, f0 = (double)tmp[t] * tA + tB;
-                        const float d = (float)((f2 - f0) / (2 * fabs(h)));
-                        dd[0][t] = q == 0 ? d : dd[0][t];
-                        dd[1][t] = q == 1 ? d : dd[1][t];
-                        dd[2][t] = q == 2 ? d : dd[2][t];
-                    }
-                }
+        if (lane == 0 && wr == 0) {
+            if (!ok) {
+    #pragma unroll
+                for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = (double)NAN;
             }
-            if (k < 6) {
-                const int q = k / 2;
-                const double sgnh = (k % 2 == 0) ? h : -h;
-                req[0] = th[0] + (q == 0 ? sgnh : 0.0);
-                req[1] = th[1] + (q == 1 ? sgnh : 0.0);
-                req[2] = th[2] + (q == 2 ? sgnh : 0.0);
-                want = true;
-                k++;
-                continue;
-            }
-            if (!at_theta()) {
-                if (k == 7) {
-#pragma unroll
-                    for (int t = 0; t < RPL; t++) cur[t] = tmp[t];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) pc[c] = th[c];
-                    hc = true;
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 3; c++) req[c] = th[c];
-                    want = true;
-                    k = 7;
-                    continue;
-                }
-            }
-            double acc[15];
-#pragma unroll
-            for (int kk = 0; kk < 15; kk++) acc[kk] = 0.0;
-#pragma unroll
-            for (int t = 0; t < RPL; t++) {
-                if (!SY::template valid<R>(lane, wr, t)) continue;
-                const double f1 = (double)cur[t] * tA + tB;
-                // x, y, z, then A (the reference differentiates the model f1) and B
-                const double J[5] = {(double)dd[0][t], (double)dd[1][t], (double)dd[2][t], f1, 1.0};
-                int kk = 0;
-#pragma unroll
-                for (int p = 0; p < 5; p++)
-#pragma unroll
-                    for (int q = p; q < 5; q++) acc[kk++] += (1 / f1) * J[p] * J[q];
-            }
-            SY::template sumv<15>(acc, red, lane, wr);
-            double F[25], inv[25];
-            int kk = 0;
-#pragma unroll
-            for (int p = 0; p < 5; p++)
-#pragma unroll
-                for (int q = p; q < 5; q++) { F[p * 5 + q] = acc[kk]; F[q * 5 + p] = acc[kk]; kk++; }
-            likelihood = L(cur, tA, tB);
-            inverse_cofactor5(F, inv);
-            if (lane == 0 && wr == 0) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = sqrt(inv[c * 5 + c]);
-            }
-            st = kRsDone;
-            continue;
+            A.theta[(size_t)b * 5 + 0] = tA;
+            A.theta[(size_t)b * 5 + 1] = tB;
+            A.theta[(size_t)b * 5 + 2] = th[0];
+            A.theta[(size_t)b * 5 + 3] = th[1];
+            A.theta[(size_t)b * 5 + 4] = th[2];
+            A.nll[b] = likelihood;
+            A.iters[b] = nbIter;
+            A.status[b] = (conv ? kStConverged : 0) | (nbIter == A.max_iter && !conv ? kStMaxIter : 0) |
+                          (nan ? kStNaN : 0) | (brk ? kStLowPhoton : 0) | (nfield << 8);
         }
-    }
-    if (lane == 0 && wr == 0) {
-        if (!ok) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) A.crlb[(size_t)b * 3 + c] = (double)NAN;
-        }
-        A.theta[(size_t)b * 5 + 0] = tA;
-        A.theta[(size_t)b * 5 + 1] = tB;
-        A.theta[(size_t)b * 5 + 2] = th[0];
-        A.theta[(size_t)b * 5 + 3] = th[1];
-        A.theta[(size_t)b * 5 + 4] = th[2];
-        A.nll[b] = likelihood;
-        A.iters[b] = nbIter;
-        A.status[b] = (conv ? kStConverged : 0) | (nbIter == A.max_iter && !conv ? kStMaxIter : 0) |
-                      (nan ? kStNaN : 0) | (brk ? kStLowPhoton : 0) | (nfield << 8);
+        if (!A.queue) break;
+        b = next_roi<SY>(A.queue, lds.q[slot], lane, wr);
     }
 }
 

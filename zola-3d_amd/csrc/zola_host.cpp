@@ -232,6 +232,7 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
 
     HIP_TRY(c, hipStreamCreate(&c->stream));
     c->own_stream = true;
+    HIP_TRY(c, hipMalloc((void**)&c->d_queue, 8 * sizeof(int)));
     if (N == zola::kN2) {
         HIP_TRY(c, hipMalloc((void**)&c->d_tab128, zola::kW2 * zola::kW2 * sizeof(float4)));
         HIP_TRY(c, hipMalloc((void**)&c->d_taps128, 2 * zola::kN2 * sizeof(float)));
@@ -275,6 +276,17 @@ int zola_ctx_destroy(zola_ctx* c) {
     if (c->d_tmpl) (void)hipFree(c->d_tmpl);
     if (c->d_Zw) (void)hipFree(c->d_Zw);
     if (c->d_qbar) (void)hipFree(c->d_qbar);
+    if (c->d_queue) (void)hipFree(c->d_queue);
+    if (c->mixed_exec) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(c->mixed_exec);
+    }
+    for (int g = 0; g < 8; g++) {
+        if (c->side[g]) (void)hipStreamDestroy(c->side[g]);
+        if (c->join[g]) (void)hipEventDestroy(c->join[g]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->cap) (void)hipStreamDestroy(c->cap);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ZOLA_OK;
@@ -441,6 +453,12 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
 int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                        const double* init, double* theta, double* nll, double* crlb, int* iters,
                        int* status) {
+    return zola_fit_batch_zoil_dev(c, cfg, B, pixels, init, nullptr, theta, nll, crlb, iters, status);
+}
+
+int zola_fit_batch_zoil_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
+                            const double* init, const float* zoil, double* theta, double* nll, double* crlb,
+                            int* iters, int* status) {
     if (!c) return ZOLA_EINVAL;
     if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
     if (B == 0) return ZOLA_OK;
@@ -462,6 +480,8 @@ int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const flo
     A.noise_rel = 2e-7;
     A.pixels = pixels;
     A.init = init;
+    A.zoil = zoil;
+    A.queue = c->d_queue;
     A.theta = theta;
     A.nll = nll;
     A.crlb = crlb;
@@ -469,6 +489,92 @@ int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const flo
     A.status = status;
     if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, c->stream));
     else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, A, c->stream));
+    return ZOLA_OK;
+}
+
+int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int* R, const int* B,
+                       const float* const* pixels, const double* const* init, double* const* theta,
+                       double* const* nll, double* const* crlb, int* const* iters, int* const* status) {
+    if (!c) return ZOLA_EINVAL;
+    if (!cfg || G <= 0 || G > 8 || !R || !B || !pixels || !init || !theta || !nll || !crlb || !iters || !status)
+        return set_err(c, ZOLA_EINVAL, "bad mixed-fit arguments (1 <= G <= 8)");
+    if (cfg->mode != ZOLA_FIT_REFERENCE && cfg->mode != ZOLA_FIT_LM)
+        return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", cfg->mode);
+    if (cfg->max_iter < 0 || !(cfg->zmax >= cfg->zmin))
+        return set_err(c, ZOLA_EINVAL, "bad iteration count or z range");
+    for (int g = 0; g < G; g++) {
+        if (B[g] < 0) return set_err(c, ZOLA_EINVAL, "negative batch in group %d", g);
+        const bool ok = c->N == zola::kN ? zola_kernels_support_roi(R[g]) : (R[g] == c->R && zola_kernels_support_roi128(R[g]));
+        if (!ok) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available on the %d grid", R[g], c->N);
+        if (B[g] > 0 && (!pixels[g] || !init[g] || !theta[g] || !nll[g] || !crlb[g] || !iters[g] || !status[g]))
+            return set_err(c, ZOLA_EINVAL, "null buffer in group %d", g);
+    }
+    if (!c->fork) HIP_TRY(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+    if (!c->cap) HIP_TRY(c, hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
+    for (int g = 0; g < G; g++) {
+        if (!c->side[g]) HIP_TRY(c, hipStreamCreateWithFlags(&c->side[g], hipStreamNonBlocking));
+        if (!c->join[g]) HIP_TRY(c, hipEventCreateWithFlags(&c->join[g], hipEventDisableTiming));
+    }
+    // capture on a private stream (nothing runs during capture; the graph is
+    // launched on the context's stream, after the work already queued there):
+    // fork to one side stream per group, one fit launch each, join
+    HIP_TRY(c, hipStreamBeginCapture(c->cap, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipEventRecord(c->fork, c->cap);
+    for (int g = 0; g < G && err == hipSuccess; g++) {
+        err = hipStreamWaitEvent(c->side[g], c->fork, 0);
+        if (err != hipSuccess) break;
+        if (B[g] > 0) {
+            zola::FitArgs A{};
+            A.B = B[g];
+            A.max_iter = cfg->max_iter;
+            A.xystep = (float)c->o.xystep;
+            A.zmin = cfg->zmin;
+            A.zmax = cfg->zmax;
+            A.zfocus = cfg->zfocus;
+            A.thr = cfg->photon_threshold;
+            A.tol = cfg->tol;
+            A.noise_rel = 2e-7;
+            A.pixels = pixels[g];
+            A.init = init[g];
+            A.queue = c->d_queue + g;
+            A.theta = theta[g];
+            A.nll = nll[g];
+            A.crlb = crlb[g];
+            A.iters = iters[g];
+            A.status = status[g];
+            err = c->N == zola::kN2 ? zola_launch_fit128(c->model128, R[g], cfg->mode, A, c->side[g])
+                                    : zola_launch_fit64(c->model, R[g], cfg->mode, A, c->side[g]);
+            if (err != hipSuccess) break;
+        }
+        err = hipEventRecord(c->join[g], c->side[g]);
+        if (err == hipSuccess) err = hipStreamWaitEvent(c->cap, c->join[g], 0);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->cap, &graph);
+    if (err != hipSuccess || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_err(c, ZOLA_EHIP, "mixed-fit capture: %s", hipGetErrorString(err != hipSuccess ? err : e2));
+    }
+    // reuse the previous executable graph when only the parameters changed
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    bool updated = false;
+    if (c->mixed_exec) {
+        hipGraphExecUpdateResult res;
+        hipGraphNode_t bad = nullptr;
+        updated = hipGraphExecUpdate(c->mixed_exec, graph, &bad, &res) == hipSuccess &&
+                  res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGraphExecDestroy(c->mixed_exec);
+            c->mixed_exec = nullptr;
+        }
+    }
+    if (!updated) err = hipGraphInstantiate(&c->mixed_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) {
+        c->mixed_exec = nullptr;
+        return set_err(c, ZOLA_EHIP, "mixed-fit instantiate: %s", hipGetErrorString(err));
+    }
+    HIP_TRY(c, hipGraphLaunch(c->mixed_exec, c->stream));
     return ZOLA_OK;
 }
 

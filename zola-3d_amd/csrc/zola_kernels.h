@@ -32,3 +32,8 @@ hipError_t zola_launch_psf_forward128(const zola::Model128& M, int R, int B, con
                                       float* out, hipStream_t st);
 hipError_t zola_launch_fit128(const zola::Model128& M, int R, int mode, const zola::FitArgs& A,
                               hipStream_t st);
+
+// Zeroes n ints (a kernel, so that it can sit in a captured graph).
+hipError_t zola_launch_zero_ints(int* p, int n, hipStream_t st);
+// Blocks of a persistent launch: resident blocks per CU (occupancy) times CUs.
+int zola_persistent_blocks(const void* kernel, int threads);

@@ -67,6 +67,7 @@ hipError_t launch_psf_forward_r(const Model64& M, int B, const float* params, fl
 // ---------------------------------------------------------------------------
 #include "zola_fit64.hpp"
 #include "zola_adjoint64.hpp"
+#include "zola_kernels.h"
 
 namespace zola {
 
@@ -156,14 +157,29 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
 constexpr int kWPB_LM = 4;
 constexpr int kWPB_REF = 4;
 
+// Grid of a fit launch: one ROI per wave slot, or, with a queue (persistent
+// form), the resident blocks only (ROIs taken from the queue, zeroed first).
+template <class K>
+int fit_grid(K kernel, int threads, int roi_per_block, const FitArgs& A, hipStream_t st, hipError_t& err) {
+    const int need = (A.B + roi_per_block - 1) / roi_per_block;
+    err = hipSuccess;
+    if (!A.queue) return need;
+    const int resident = zola_persistent_blocks(reinterpret_cast<const void*>(kernel), threads);
+    err = zola_launch_zero_ints(A.queue, 1, st);
+    return resident > 0 && resident < need ? resident : need;
+}
+
 template <int R>
 hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st) {
+    hipError_t e;
     if (mode == 1) {
-        dim3 grid((A.B + kWPB_LM - 1) / kWPB_LM), block(64 * kWPB_LM);
-        hipLaunchKernelGGL((k_fit_lm64<R, kWPB_LM>), grid, block, 0, st, M, A);
+        const int g = fit_grid(k_fit_lm64<R, kWPB_LM>, 64 * kWPB_LM, kWPB_LM, A, st, e);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fit_lm64<R, kWPB_LM>), dim3(g), dim3(64 * kWPB_LM), 0, st, M, A);
     } else {
-        dim3 grid((A.B + kWPB_REF - 1) / kWPB_REF), block(64 * kWPB_REF);
-        hipLaunchKernelGGL((k_fit_ref64<R, kWPB_REF>), grid, block, 0, st, M, A);
+        const int g = fit_grid(k_fit_ref64<R, kWPB_REF>, 64 * kWPB_REF, kWPB_REF, A, st, e);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fit_ref64<R, kWPB_REF>), dim3(g), dim3(64 * kWPB_REF), 0, st, M, A);
     }
     return hipGetLastError();
 }
@@ -208,8 +224,16 @@ hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params
 // Fits: one ROI (four waves) per block.
 template <int R>
 hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st) {
-    if (mode == 1) hipLaunchKernelGGL((k_fit_lm128<R, 4>), dim3(A.B), dim3(256), 0, st, M, A);
-    else hipLaunchKernelGGL((k_fit_ref128<R, 4>), dim3(A.B), dim3(256), 0, st, M, A);
+    hipError_t e;
+    if (mode == 1) {
+        const int g = fit_grid(k_fit_lm128<R, 4>, 256, 1, A, st, e);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fit_lm128<R, 4>), dim3(g), dim3(256), 0, st, M, A);
+    } else {
+        const int g = fit_grid(k_fit_ref128<R, 4>, 256, 1, A, st, e);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fit_ref128<R, 4>), dim3(g), dim3(256), 0, st, M, A);
+    }
     return hipGetLastError();
 }
 

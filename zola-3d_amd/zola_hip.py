@@ -27,7 +27,8 @@ SYMBOLS = [
     "zola_fit_batch_dev", "zola_fit_init_dev", "zola_get_stream", "zola_calib_eval_dev",
     "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
     "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
-    "zola_detector_correlation", "zola_localize_frames",
+    "zola_detector_correlation", "zola_localize_frames", "zola_fit_mixed_dev",
+    "zola_fit_batch_zoil_dev",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -95,6 +96,10 @@ def lib():
         if hasattr(L, "zola_fit_init_dev"):
             L.zola_fit_init_dev.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                             C.c_double, vp]
+        if hasattr(L, "zola_fit_batch_zoil_dev"):
+            L.zola_fit_batch_zoil_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 8
+        if hasattr(L, "zola_fit_mixed_dev"):
+            L.zola_fit_mixed_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 9
         if hasattr(L, "zola_detector_create"):
             L.zola_detector_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.POINTER(vp)]
@@ -214,6 +219,24 @@ class Context:
                                          _ptr(theta), _ptr(nll), _ptr(crlb), _ptr(iters),
                                          _ptr(status)), "fit_batch")
         return dict(theta=theta, nll=nll, crlb=crlb, iters=iters, status=status)
+
+    def fit_zoil_dev(self, cfg, pixels, init, zoil, theta, nll, crlb, iters, status):
+        """Device form with a per-ROI focus term zoil (B floats)."""
+        self._check(lib().zola_fit_batch_zoil_dev(self._h, C.byref(cfg), pixels.shape[0], _ptr(pixels),
+                                                  _ptr(init), _ptr(zoil), _ptr(theta), _ptr(nll), _ptr(crlb),
+                                                  _ptr(iters), _ptr(status)), "fit_batch_zoil_dev")
+
+    def fit_mixed_dev(self, cfg, groups):
+        """groups: list of dicts with keys R, pixels, init, theta, nll, crlb, iters, status
+        (device tensors); one graph launch on the context's stream."""
+        G = len(groups)
+        Rs = (C.c_int * G)(*[int(g["R"]) for g in groups])
+        Bs = (C.c_int * G)(*[int(g["pixels"].shape[0]) for g in groups])
+        arr = {k: (C.c_void_p * G)(*[g[k].data_ptr() for g in groups])
+               for k in ("pixels", "init", "theta", "nll", "crlb", "iters", "status")}
+        self._check(lib().zola_fit_mixed_dev(self._h, C.byref(cfg), G, Rs, Bs, arr["pixels"], arr["init"],
+                                             arr["theta"], arr["nll"], arr["crlb"], arr["iters"],
+                                             arr["status"]), "fit_mixed_dev")
 
     def fit_dev(self, cfg, pixels, init, theta, nll, crlb, iters, status):
         self._check(lib().zola_fit_batch_dev(self._h, C.byref(cfg), pixels.shape[0],
