@@ -221,6 +221,15 @@ int  zola_detector_templates(const zola_detector* det, double* psf);
  * convolveNormalizedInFourierDomain (Predetection.java:292-496). */
 int  zola_detect_frames(zola_detector* det, int F, const float* frames, int cap, double* cand,
                         int* counts);
+/* Centre of the template z range: the depth whose window of width
+ * axial_range/2 has the smallest summed lateral CRLB for a 3000-photon
+ * emitter on background 50, found by the reference's coarse scan, halving
+ * refinement and window scan (process/cpu/SearchPSFcenter.java:37-351; the PSF
+ * is cropped to the FFT size, the Fisher matrix by the reference's finite
+ * differences).  FP64 on the host cores; called once per run by the pipeline
+ * (LocalizationPipeline.java:159-172: minZ/maxZ = centre -/+ axial_range/2). */
+int  zola_search_psf_center(zola_ctx* ctx, double zfocus, double axial_range, double* center);
+
 /* The normalised correlation maps of one frame, nZ x W x H doubles
  * (Predetection.resultCorrelation). */
 int  zola_detector_correlation(zola_detector* det, const float* frame, double* corr);

@@ -83,6 +83,8 @@ def lib():
         L.zo_calib_nll.restype = C.c_double
         L.zo_frame_localize.argtypes = [vp, vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_double, _dp, C.c_int]
+        L.zo_search_psf_center.restype = C.c_double
+        L.zo_search_psf_center.argtypes = [vp, C.c_double, C.c_double]
         L.zo_calib_nll.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]
         _lib = L
     return _lib
@@ -272,6 +274,11 @@ def frame_localize(model, det, image, zfocus, minZ, maxZ, iter_max=20, photon_th
     n = lib().zo_frame_localize(model._h, det._h, _d(image), zfocus, minZ, maxZ, iter_max,
                                 photon_thr, out, cap)
     return out[:min(n, cap)]
+
+
+def search_psf_center(model, zfocus, axial_range):
+    """SearchPSFcenter.getPosition with a model of R = N."""
+    return lib().zo_search_psf_center(model._h, zfocus, axial_range)
 
 
 def calib_nll(model, stack, dz, xs, ys, zs, Aslice, bg, zwat=0.0):

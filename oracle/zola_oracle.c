@@ -1257,6 +1257,95 @@ int zo_detector_run(zo_detector* d, const double* image, double* out, int cap, d
 }
 
 /* ------------------------------------------------------------------ */
+/* Focus centre (cpu/SearchPSFcenter.java:37-81, 86-214, 219-351)       */
+/* ------------------------------------------------------------------ */
+
+static double sc_crlb(zo_model* m, double zfocus, double z) {
+    /* computeFisher(0, 0, z, .005): photons 3000, background 50 */
+    const double h = .005, ph = 3000, bg = 50;
+    int n = m->R * m->R;
+    double* f1 = (double*)xcalloc(n, sizeof(double));
+    double* a = (double*)xcalloc(n, sizeof(double));
+    double* b = (double*)xcalloc(n, sizeof(double));
+    double* d = (double*)xcalloc((size_t)5 * n, sizeof(double));
+    zo_model_psf(m, 0, 0, zfocus, z, f1);
+    for (int k = 0; k < n; k++) f1[k] = f1[k] * ph + bg;
+    for (int p = 0; p < 3; p++) {
+        double dx = p == 0 ? h : 0, dy = p == 1 ? h : 0, dz = p == 2 ? h : 0;
+        zo_model_psf(m, dx, dy, zfocus, z + dz, a);
+        zo_model_psf(m, -dx, -dy, zfocus, z - dz, b);
+        for (int k = 0; k < n; k++) d[p * n + k] = ((a[k] * ph + bg) - (b[k] * ph + bg)) / (2 * fabs(h));
+    }
+    for (int k = 0; k < n; k++) {
+        /* the photon derivative on the model f1 with step 100 h (SearchPSFcenter.java:271-276) */
+        d[3 * n + k] = (f1[k] * (ph + h * 100) + bg - (f1[k] * (ph - h * 100) + bg)) / (2 * fabs(h));
+        d[4 * n + k] = (f1[k] * ph + (bg + h) - (f1[k] * ph + (bg - h))) / (2 * fabs(h));
+    }
+    double I[25], inv[25];
+    for (int p = 0; p < 5; p++)
+        for (int q = 0; q < 5; q++) {
+            double s = 0;
+            for (int k = 0; k < n; k++) s += (1 / f1[k]) * d[p * n + k] * d[q * n + k];
+            I[p * 5 + q] = s;
+        }
+    zo_inverse_cofactor(5, I, inv);
+    free(f1); free(a); free(b); free(d);
+    double sx = sqrt(inv[0]), sy = sqrt(inv[6]);
+    return sqrt(sx * sx + sy * sy);
+}
+
+static double sc_min_position(zo_model* m, double zfocus, double minZ, double maxZ, double stepZ) {
+    double mv = INFINITY, mp = 0;
+    for (double u = minZ; u < maxZ; u += stepZ) {
+        double c = sc_crlb(m, zfocus, u);
+        if (mv > c) { mv = c; mp = u; }
+    }
+    return mp;
+}
+
+static double sc_min_range(zo_model* m, double zfocus, double minZ, double maxZ, double stepZ, double axial) {
+    axial /= 2;
+    int cnt = 0;
+    for (double u = minZ - axial / 2; u <= maxZ + axial / 2; u += stepZ) cnt++;
+    double* crlb = (double*)xcalloc(cnt ? cnt : 1, sizeof(double));
+    cnt = 0;
+    for (double u = minZ - axial / 2; u <= maxZ + axial / 2; u += stepZ) crlb[cnt++] = sc_crlb(m, zfocus, u);
+    double mv = INFINITY, mp = 0;
+    int nb = (int)(axial / (2 * stepZ));
+    for (double u = minZ; u < maxZ; u += stepZ) {
+        double C = 0;
+        int nb2 = -(int)(axial / (2 * stepZ));
+        for (double i = -axial / 2; i <= axial / 2; i += stepZ) {
+            int k = nb + nb2;
+            if (k >= 0 && k < cnt) C += crlb[k];
+            nb2++;
+        }
+        if (mv > C) { mv = C; mp = u; }
+        nb++;
+    }
+    free(crlb);
+    return mp;
+}
+
+double zo_search_psf_center(zo_model* m, double zfocus, double axialRange) {
+    double stepZ = m->o.xystep * 5;
+    double minZ = zfocus - 2 * axialRange - fabs(zfocus);
+    double maxZ = zfocus + fabs(zfocus) + 2 * axialRange;
+    double position = sc_min_position(m, zfocus, minZ, maxZ, stepZ);
+    double old = INFINITY;
+    for (int i = 0; i < 10; i++) {
+        minZ = position - stepZ * 5;
+        maxZ = position + stepZ * 5;
+        stepZ = stepZ / 2;
+        position = sc_min_position(m, zfocus, minZ, maxZ, stepZ);
+        if (fabs(old - position) < 0.00001) break;
+        old = position;
+    }
+    return sc_min_range(m, zfocus, position - axialRange * 2, position + axialRange * 2, m->o.xystep,
+                        axialRange);
+}
+
+/* ------------------------------------------------------------------ */
 /* Frame walk (cpu/LocalizationPipeline.java:721-1073, 1099-1250)        */
 /* ------------------------------------------------------------------ */
 

@@ -20,6 +20,7 @@
  *   zo_score             process/cpu/LocalizationPipeline.java:1099-1115
  *   zo_detector_*        process/cpu/Predetection.java:63-593
  *   zo_frame_localize    process/cpu/LocalizationPipeline.java:721-1073,1099-1250
+ *   zo_search_psf_center process/cpu/SearchPSFcenter.java:37-351
  *   zo_calib_nll         process/cpu/ZernikePhaseRetrieval.java:131-140,1729-1736,1842-1879
  * Functions with no reference counterpart (analytic tangents, the adjoint,
  * the FP64 Levenberg-Marquardt optimum) are derived from the same model and
@@ -145,6 +146,11 @@ int  zo_detector_run(zo_detector* d, const double* image, double* out, int cap, 
 int  zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double zfocus,
                        double minZ, double maxZ, int iterMax, double photonThr,
                        double* out, int cap);
+
+/* Depth of smallest lateral CRLB, refined, then the best window of width
+ * axialRange/2 (SearchPSFcenter.getPosition).  m must have R = N (the
+ * reference crops the PSF to the FFT size here). */
+double zo_search_psf_center(zo_model* m, double zfocus, double axialRange);
 
 /* Calibration negative log-likelihood of one bead stack (ZernikePhaseRetrieval.Model). */
 double zo_calib_nll(zo_model* m, int S, const double* stack /* S*R*R */, const double* dz,

@@ -139,3 +139,16 @@ def test_frame_walk_lm_form_finds_same_emitters(models):
     print("LM vs reference form, max |d| nm x, y, z:", d.max(axis=0))
     assert d[:, :2].max() <= 5.0 and d[:, 2].max() <= 10.0
     det.close()
+
+
+def test_search_psf_center_matches_oracle(torch_cuda):
+    print("case: focus centre, 64 grid, astigmatic", flush=True)
+    coef = Z.astigmatic_coeffs(28)
+    ctx = Z.Context(Z.optics(size_fft=64, nwat=1.33), R, coef)
+    m = O.Model(O.optics(size_fft=64, nwat=1.33), 64, coef)
+    for zf, ax in ((0.0, 1.0), (1.5, 0.8)):
+        c = ctx.search_psf_center(zf, ax)
+        ref = O.search_psf_center(m, zf, ax)
+        print("centre", zf, ax, c, ref)
+        assert abs(c - ref) <= 1e-9
+    ctx.close()

@@ -360,3 +360,32 @@ def test_frame_walk_photon_threshold_and_border():
     # lowering the threshold accepts the dim one too
     locs2 = O.frame_localize(m, det, img, 0.0, -0.5, 0.5, photon_thr=100.0)
     assert len(locs2) == 2
+
+
+def _center_offset(axial, step):
+    # getMinRangeCRLB reports u, but its window of half-width axial/4 starts at
+    # u - axial/4 and has (int)(axial/(4 step)) steps on each side of u:
+    # the window's own centre sits at u + offset
+    a = axial / 2
+    return -a / 2 + int(a / (2 * step)) * step
+
+
+def test_psf_center_of_symmetric_psf_is_focus():
+    # an aberration-free PSF is mirror-symmetric in z (E(-z) = conj E(z)), so the
+    # CRLB window is centred on z = 0 to within half a grid step
+    m = O.Model(O.optics(size_fft=64), 64, np.zeros(15))
+    c = O.search_psf_center(m, 0.0, 1.0)
+    assert abs(c + _center_offset(1.0, 0.11)) <= 0.055 + 1e-9
+
+
+def test_psf_center_follows_defocus():
+    # a defocus term a Z4 = a (2 rho^2 - 1) moves best focus by about
+    # 4 a n / (k NA^2) (paraxial); the centre follows, antisymmetric in a
+    par = 4 * 1.518 / (2 * math.pi / 0.68 * 1.45 ** 2)
+    cs = []
+    for a in (-1.0, 1.0):
+        m = O.Model(O.optics(size_fft=64), 64, Z.coeffs_from_noll({4: a}, 15))
+        cs.append(O.search_psf_center(m, 0.0, 1.0) + _center_offset(1.0, 0.11))
+    assert cs[0] < -0.5 * par and cs[1] > 0.5 * par
+    assert abs(cs[0] + cs[1]) <= 0.11 + 1e-9
+    assert abs(cs[1] - par) < 0.35 * par

@@ -80,3 +80,5 @@ inline int stage(zola_ctx* c, size_t bytes) {
 // crop, normalised to sum 1.  Used where a result must agree with an FP64
 // computation to rounding (detector templates).
 void zola_psf_f64_host(const zola_ctx* c, double x, double y, double zoil, double zwat, double* out);
+// Same for an R x R crop of any side R <= N (blur grid pow2(min(2R, N))).
+void zola_psf_f64_host_r(const zola_ctx* c, int R, double x, double y, double zoil, double zwat, double* out);

@@ -69,7 +69,11 @@ void fft2(int n1, int n2, cd* a, int sign) {
 }  // namespace
 
 void zola_psf_f64_host(const zola_ctx* c, double x, double y, double zoil, double zwat, double* out) {
-    const int N = c->N, s2 = c->s2, R = c->R;
+    zola_psf_f64_host_r(c, c->R, x, y, zoil, zwat, out);
+}
+
+void zola_psf_f64_host_r(const zola_ctx* c, int R, double x, double y, double zoil, double zwat, double* out) {
+    const int N = c->N, s2 = ref_pow2(std::min(2 * R, N));
     const double cc = c->center, xs = c->o.xystep;
     std::vector<cd> E((size_t)N * N, cd(0, 0));
     for (int p = 0; p < c->D; p++) {

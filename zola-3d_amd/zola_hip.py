@@ -28,7 +28,7 @@ SYMBOLS = [
     "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
     "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
     "zola_detector_correlation", "zola_localize_frames", "zola_fit_mixed_dev",
-    "zola_fit_batch_zoil_dev",
+    "zola_fit_batch_zoil_dev", "zola_search_psf_center",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -96,6 +96,8 @@ def lib():
         if hasattr(L, "zola_fit_init_dev"):
             L.zola_fit_init_dev.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                             C.c_double, vp]
+        if hasattr(L, "zola_search_psf_center"):
+            L.zola_search_psf_center.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double)]
         if hasattr(L, "zola_fit_batch_zoil_dev"):
             L.zola_fit_batch_zoil_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 8
         if hasattr(L, "zola_fit_mixed_dev"):
@@ -219,6 +221,13 @@ class Context:
                                          _ptr(theta), _ptr(nll), _ptr(crlb), _ptr(iters),
                                          _ptr(status)), "fit_batch")
         return dict(theta=theta, nll=nll, crlb=crlb, iters=iters, status=status)
+
+    def search_psf_center(self, zfocus, axial_range):
+        """Centre of the template z range (SearchPSFcenter.getPosition), um."""
+        out = C.c_double()
+        self._check(lib().zola_search_psf_center(self._h, zfocus, axial_range, C.byref(out)),
+                    "search_psf_center")
+        return out.value
 
     def fit_zoil_dev(self, cfg, pixels, init, zoil, theta, nll, crlb, iters, status):
         """Device form with a per-ROI focus term zoil (B floats)."""
