@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-n128", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-frames", action="store_true")
+    p.add_argument("--no-formats", action="store_true")
     p.add_argument("--n128-batch", type=int, default=65536)
     p.add_argument("--mixed-total", type=int, default=200000)
     p.add_argument("--frames", type=int, default=64)
@@ -320,6 +321,37 @@ def frames_bench(torch, Z, stream, F):
     return out
 
 
+def formats_bench(Z, n=1_000_000):
+    """Localization table (ThunderSTORM CSV rows as the plugin writes them) for
+    n synthetic records, and calibration-file write + read of a 45-mode model."""
+    rng = np.random.default_rng(0xC5)
+    locs = np.zeros((n, Z.LOC_FIELDS))
+    locs[:, 0] = np.sort(rng.integers(0, 10000, n))
+    locs[:, 1:3] = rng.uniform(0, 50000, (n, 2))
+    locs[:, 3] = rng.uniform(-3000, 3000, n)
+    locs[:, 4] = rng.uniform(500, 2e4, n)
+    locs[:, 5] = rng.uniform(20, 100, n)
+    locs[:, 6] = rng.uniform(0, 3, n)
+    locs[:, 7:10] = rng.uniform(5, 40, (n, 3))
+    t0 = time.perf_counter()
+    text = Z.loc_csv(locs)
+    dt = time.perf_counter() - t0
+    opt = Z.optics(size_fft=128)
+    a = Z.tetrapod_coeffs(45)
+    reps = 2000
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        js = Z.calib_json_format(opt, a)
+    t_w = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Z.calib_json_parse(js, 128)
+    t_r = (time.perf_counter() - t0) / reps
+    return {"workload": f"{n} localization records -> CSV rows; 45-mode calibration file write/read",
+            "csv_rows_per_s": round(n / dt, 1), "csv_mb": round(len(text) / 1e6, 2),
+            "calib_write_us": round(t_w * 1e6, 2), "calib_read_us": round(t_r * 1e6, 2)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -472,6 +504,8 @@ def main():
         out["mixed_sizes"] = mixed_bench(torch, Z, stream, args.mixed_total)
     if not args.no_frames:
         out["frames_n128"] = frames_bench(torch, Z, stream, args.frames)
+    if not args.no_formats:
+        out["formats"] = formats_bench(Z)
 
     if args.copy_probe_mb:
         n = args.copy_probe_mb * (1 << 20) // 4

@@ -1,6 +1,7 @@
-"""The C-ABI library loads and exports every entry point include/zola_hip.h
+"""The C-ABI library loads and exports every entry point include/*.h
 declares; argument validation happens before any device call."""
 import ctypes as C
+import glob
 import os
 import re
 
@@ -12,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    text = open(os.path.join(ROOT, "include", "zola_hip.h")).read()
+    text = "".join(open(p).read() for p in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(zola_[a-z0-9_]+)\s*\(", text)))
 

@@ -28,7 +28,9 @@ SYMBOLS = [
     "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
     "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
     "zola_detector_correlation", "zola_localize_frames", "zola_fit_mixed_dev",
-    "zola_fit_batch_zoil_dev", "zola_search_psf_center",
+    "zola_fit_batch_zoil_dev", "zola_search_psf_center", "zola_java_double_string",
+    "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
+    "zola_formats_last_error",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -113,8 +115,77 @@ def lib():
             L.zola_detector_correlation.argtypes = [vp, vp, vp]
             L.zola_localize_frames.argtypes = [vp, C.POINTER(FitConfig), C.POINTER(AcceptConfig),
                                                C.c_int, vp, C.c_int, vp, vp, vp, vp]
+        L.zola_java_double_string.argtypes = [C.c_double, C.c_char_p, C.c_int]
+        L.zola_calib_json_format.restype = C.c_long
+        L.zola_calib_json_format.argtypes = [C.POINTER(Optics), vp, C.c_int, C.c_char_p, C.c_long]
+        L.zola_calib_json_parse.argtypes = [C.c_char_p, C.c_int, C.POINTER(Optics),
+                                            C.POINTER(C.c_int), vp, C.c_int, C.POINTER(C.c_int)]
+        L.zola_loc_csv_header.restype = C.c_long
+        L.zola_loc_csv_header.argtypes = [C.c_char_p, C.c_long]
+        L.zola_loc_csv_rows.restype = C.c_long
+        L.zola_loc_csv_rows.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_long]
+        L.zola_formats_last_error.restype = C.c_char_p
         _lib = L
     return _lib
+
+
+# ---- calibration file and localization table (host code, no GPU) ----------
+
+def _formats_error(rc):
+    return ZolaError(f"{rc}: {lib().zola_formats_last_error().decode()}")
+
+
+def java_double_string(v):
+    """The text Java's Double.toString gives for v."""
+    buf = C.create_string_buffer(32)
+    rc = lib().zola_java_double_string(float(v), buf, 32)
+    if rc < 0:
+        raise _formats_error(rc)
+    return buf.value.decode()
+
+
+def _text(call):
+    n = call(None, 0)
+    if n < 0:
+        raise _formats_error(n)
+    buf = C.create_string_buffer(n + 1)
+    call(buf, n + 1)
+    return buf.value.decode()
+
+
+def calib_json_format(opt, coeffs):
+    """Calibration file text of DataPhase.saveJSON for optics and Zernike slot
+    coefficients."""
+    a = np.ascontiguousarray(coeffs, dtype=np.float64)
+    return _text(lambda b, c: lib().zola_calib_json_format(C.byref(opt), _ptr(a), a.size, b, c))
+
+
+def calib_json_parse(text, size_fft):
+    """Reads calibration file text as DataPhase.loadJSON does: returns (optics,
+    coefficients, the file's own sizeFFT).  nwat is left at 0."""
+    if isinstance(text, str):
+        text = text.encode()
+    opt = Optics()
+    fsize, J = C.c_int(0), C.c_int(0)
+    L = lib()
+    rc = L.zola_calib_json_parse(text, size_fft, C.byref(opt), C.byref(fsize), None, 0, C.byref(J))
+    if rc < 0:
+        raise _formats_error(rc)
+    a = np.zeros(J.value)
+    rc = L.zola_calib_json_parse(text, size_fft, C.byref(opt), C.byref(fsize), _ptr(a), a.size,
+                                 C.byref(J))
+    if rc < 0:
+        raise _formats_error(rc)
+    return opt, a, fsize.value
+
+
+def loc_csv(locs, first_id=0, frame_base=0, header=True):
+    """Localization table text (StackLocalization.saveCSV) of records as
+    zola_localize_frames writes them."""
+    r = np.ascontiguousarray(locs, dtype=np.float64).reshape(-1, LOC_FIELDS)
+    head = _text(lambda b, c: lib().zola_loc_csv_header(b, c)) if header else ""
+    rows = _text(lambda b, c: lib().zola_loc_csv_rows(_ptr(r), r.shape[0], first_id, frame_base, b, c))
+    return head + rows
 
 
 def optics(size_fft=64, xystep=0.11, zstep=0.05, wavelength=0.68, noil=1.518, nwat=1.518,
