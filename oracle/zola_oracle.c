@@ -844,34 +844,12 @@ int zo_localize_batch(const zo_optics* o, int R, const double* coeffs, int J, in
 /* counterpart: the converged Poisson MLE used as the accuracy anchor).  */
 /* ------------------------------------------------------------------ */
 
-static int solve_spd(int n, double* A, double* b) {
-    /* Gaussian elimination with partial pivoting; A is n*n, b is n. */
-    for (int c = 0; c < n; c++) {
-        int p = c;
-        for (int r = c + 1; r < n; r++) if (fabs(A[r * n + c]) > fabs(A[p * n + c])) p = r;
-        if (A[p * n + c] == 0) return -1;
-        if (p != c) {
-            for (int k = 0; k < n; k++) { double t = A[c * n + k]; A[c * n + k] = A[p * n + k]; A[p * n + k] = t; }
-            double t = b[c]; b[c] = b[p]; b[p] = t;
-        }
-        for (int r = c + 1; r < n; r++) {
-            double f = A[r * n + c] / A[c * n + c];
-            for (int k = c; k < n; k++) A[r * n + k] -= f * A[c * n + k];
-            b[r] -= f * b[c];
-        }
-    }
-    for (int c = n - 1; c >= 0; c--) {
-        double s = b[c];
-        for (int k = c + 1; k < n; k++) s -= A[c * n + k] * b[k];
-        b[c] = s / A[c * n + c];
-    }
-    return 0;
-}
-
 /* theta order everywhere in this section: [A, B, x, y, z]. */
 
-/* Relative noise floor of the likelihood comparison (times A + npx*B);
- * 0 for the FP64 optimum; 2e-7 mirrors the FP32 GPU fit. */
+/* Likelihood resolution relative to A: a rejected trial whose predicted
+ * decrease is at most lm_noise_floor * A ends the fit (converged at the noise
+ * level).  0 for the FP64 optimum; 1e-8 mirrors the FP32 GPU fit, whose
+ * likelihood differs from FP64 by about 1e-9 A. */
 static double lm_noise_floor = 0.0;
 void zo_set_lm_noise_floor(double v) { lm_noise_floor = v; }
 static double lm_eval(zo_model* m, const double* roi, const double* th, double zfocus,
@@ -911,17 +889,93 @@ void zo_fisher_analytic(zo_model* m, const double* theta, double zfocus, double*
     free(psf); free(dpsf); free(roi);
 }
 
+/* Cholesky solve of the symmetric n x n system A x = b (A full, row-major;
+ * only the lower triangle is read).  Returns -1 unless A is positive definite. */
+static int chol_solve(int n, const double* A, double* b) {
+    double L[25];
+    for (int j = 0; j < n; j++) {
+        double s = A[j * n + j];
+        for (int k = 0; k < j; k++) s -= L[j * n + k] * L[j * n + k];
+        if (!(s > 0)) return -1;
+        L[j * n + j] = sqrt(s);
+        for (int i = j + 1; i < n; i++) {
+            double t = A[i * n + j];
+            for (int k = 0; k < j; k++) t -= L[i * n + k] * L[j * n + k];
+            L[i * n + j] = t / L[j * n + j];
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        double t = b[i];
+        for (int k = 0; k < i; k++) t -= L[i * n + k] * b[k];
+        b[i] = t / L[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double t = b[i];
+        for (int k = i + 1; k < n; k++) t -= L[k * n + i] * b[k];
+        b[i] = t / L[i * n + i];
+    }
+    return 0;
+}
+
+/* Structured secant correction of the Fisher matrix (NL2SOL's idea, Dennis,
+ * Gay & Welsch 1981, for a Poisson likelihood): the Hessian of the NLL is the
+ * Fisher matrix F plus S = sum (1 - n/mu) d2mu, which Fisher scoring drops.
+ * After an accepted step s, S is sized by min(1, |s'y|/|s'Ss|) and updated
+ * with Powell's symmetric Broyden formula so that S s = y = g+ - g - F+ s,
+ * all in the variables scaled by sqrt(diag F+). */
+static void lm_secant_update(double* S, const double* F, const double* s, const double* gnew,
+                             const double* gold) {
+    double D[5], sh[5], yh[5], Sh[25];
+    for (int p = 0; p < 5; p++) {
+        if (!(F[p * 5 + p] > 0)) return;
+        D[p] = sqrt(F[p * 5 + p]);
+    }
+    for (int p = 0; p < 5; p++) {
+        double fs = 0.0;
+        for (int q = 0; q < 5; q++) fs += F[p * 5 + q] * s[q];
+        sh[p] = D[p] * s[p];
+        yh[p] = (gnew[p] - gold[p] - fs) / D[p];
+    }
+    for (int p = 0; p < 5; p++)
+        for (int q = 0; q < 5; q++) Sh[p * 5 + q] = S[p * 5 + q] / (D[p] * D[q]);
+    double sSs = 0.0, sy = 0.0, ss = 0.0;
+    for (int p = 0; p < 5; p++) {
+        double t = 0.0;
+        for (int q = 0; q < 5; q++) t += Sh[p * 5 + q] * sh[q];
+        sSs += sh[p] * t;
+        sy += sh[p] * yh[p];
+        ss += sh[p] * sh[p];
+    }
+    if (!(ss > 0)) return;
+    const double tau = sSs != 0.0 ? fmin(1.0, fabs(sy) / fabs(sSs)) : 1.0;
+    double r[5], rs = 0.0;
+    for (int p = 0; p < 5; p++) {
+        double t = 0.0;
+        for (int q = 0; q < 5; q++) t += Sh[p * 5 + q] * sh[q];
+        r[p] = yh[p] - tau * t;
+        rs += r[p] * sh[p];
+    }
+    for (int p = 0; p < 5; p++)
+        for (int q = 0; q < 5; q++) {
+            const double v = tau * Sh[p * 5 + q] + (r[p] * sh[q] + sh[p] * r[q]) / ss -
+                             rs * sh[p] * sh[q] / (ss * ss);
+            S[p * 5 + q] = v * D[p] * D[q];
+        }
+}
+
 int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, double zmin,
               double zmax, int maxit, double tol, double* out_nll, int* out_iter,
               double* out_crlb) {
-    /* Fisher-scoring Levenberg-Marquardt with Nielsen's damping update:
+    /* Levenberg-Marquardt on the Fisher matrix plus a structured secant term S
+     * (lm_secant_update), damped with lambda * diag(F), Nielsen's update:
      * success: lambda *= max(1/3, 1-(2 rho-1)^3), nu = 2 (rho = actual/predicted
-     * decrease); failure: lambda *= nu, nu *= 2.  A trial is accepted when the
-     * likelihood rises by at most lm_noise_floor * (A + npx B). */
+     * decrease); failure or a system that is not positive definite: lambda *= nu,
+     * nu *= 2.  A trial is accepted when the likelihood does not rise; a
+     * rejected trial predicted to gain at most lm_noise_floor * A ends the fit. */
     int n = m->R * m->R;
     double* psf = (double*)xcalloc(n, sizeof(double));
     double* dpsf = (double*)xcalloc((size_t)3 * n, sizeof(double));
-    double th[5], g[5], H[25];
+    double th[5], g[5], H[25], S[25] = {0};
     memcpy(th, theta, sizeof(th));
     th[0] = fmax(th[0], 1.0);
     th[1] = fmax(th[1], 0.01);
@@ -933,13 +987,13 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
         int accepted = 0;
         for (int tries = 0; tries < 12 && !accepted; tries++) {
             double A[25], d[5], tn[5];
-            for (int p = 0; p < 25; p++) A[p] = H[p];
-            for (int p = 0; p < 5; p++) { A[p * 5 + p] *= (1.0 + lambda); d[p] = -g[p]; }
-            if (solve_spd(5, A, d) != 0) { lambda *= nu; nu *= 2; continue; }
+            for (int p = 0; p < 25; p++) A[p] = H[p] + S[p];
+            for (int p = 0; p < 5; p++) { A[p * 5 + p] += lambda * H[p * 5 + p]; d[p] = -g[p]; }
+            if (chol_solve(5, A, d) != 0) { lambda *= nu; nu *= 2; continue; }
             double pred = 0.0;
             for (int p = 0; p < 5; p++) {
                 double hd = 0.0;
-                for (int q = 0; q < 5; q++) hd += H[p * 5 + q] * d[q];
+                for (int q = 0; q < 5; q++) hd += (H[p * 5 + q] + S[p * 5 + q]) * d[q];
                 pred -= g[p] * d[p] + 0.5 * d[p] * hd;
             }
             for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
@@ -953,16 +1007,22 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             tn[1] = fmax(tn[1], 0.01);
             tn[4] = fmin(fmax(tn[4], zmin), zmax);
             double nn = lm_nll(m, roi, tn, zfocus, psf);
-            if (nn <= nll + lm_noise_floor * (th[0] + (double)n * th[1])) {
+            if (nn <= nll) {
                 double step = fmax(fabs(tn[2] - th[2]), fmax(fabs(tn[3] - th[3]), fabs(tn[4] - th[4])));
                 double rho = pred > 0 ? (nll - nn) / pred : 0.0;
                 double t3 = 2.0 * rho - 1.0;
                 lambda *= fmax(1.0 / 3.0, 1.0 - t3 * t3 * t3);
                 nu = 2.0;
+                double sv[5], gold[5];
+                for (int p = 0; p < 5; p++) sv[p] = tn[p] - th[p];
+                memcpy(gold, g, sizeof(gold));
                 memcpy(th, tn, sizeof(th));
                 accepted = 1;
                 if (step < tol) conv = 1;
                 nll = lm_eval(m, roi, th, zfocus, psf, dpsf, g, H);
+                lm_secant_update(S, H, sv, g, gold);
+            } else if (pred <= lm_noise_floor * th[0]) {
+                break;             /* the gain left is below the likelihood noise */
             } else {
                 lambda *= nu;
                 nu *= 2;
@@ -981,7 +1041,7 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             double A[25], e[5] = {0, 0, 0, 0, 0};
             memcpy(A, F, sizeof(A));
             e[c] = 1;
-            if (solve_spd(5, A, e) == 0) out_crlb[c - 2] = sqrt(e[c]);
+            if (chol_solve(5, A, e) == 0) out_crlb[c - 2] = sqrt(e[c]);
             else out_crlb[c - 2] = NAN;
         }
     }

@@ -74,7 +74,7 @@ def test_lm_fit_follows_oracle_iterations(setup):
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
     res = ctx.fit(cfg, rois, init)
     O.lib().zo_set_lm_noise_floor.argtypes = [__import__("ctypes").c_double]
-    O.lib().zo_set_lm_noise_floor(2e-7)
+    O.lib().zo_set_lm_noise_floor(1e-8)
     try:
         ref = [m.lm_fit(rois[b].astype(np.float64), init[b], zmin=-1.3, zmax=1.3, maxit=30,
                         tol=1e-4) for b in range(B)]

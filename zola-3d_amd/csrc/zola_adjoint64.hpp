@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
         // exp(-2 pi i nu hh / 64): fold into the phase as in the forward pass
         const float tw = -(float)hh * (float)(2.0 * kPi / 64.0) * (float)nu;
         float st, ct;
-        sincosf(tw, &st, &ct);
+        sincos_reduced(tw, st, ct);
         float r = pr[k] * ct - pi[k] * st;
         float i = pr[k] * st + pi[k] * ct;
         // sum of the two row halves (lane v and lane v + 32)
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
         const int wi = (nu + 16) * kW + v;
         const float q = T.phi[wi] + X * (float)nu + cY + T.kz[wi] * z - T.kzo[wi] * zoil;
         float s, c;
-        sincosf(q, &s, &c);
+        sincos_reduced(q, s, c);
         const float amp = T.amp[wi];
         const float qb = amp * (-s * r + c * i);
         if (hh == 0) {

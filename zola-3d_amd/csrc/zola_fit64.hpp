@@ -29,7 +29,7 @@ struct FitArgs {
     int max_iter;
     float xystep;
     double zmin, zmax, zfocus, thr, tol;
-    double noise_rel;         // LM: likelihood noise tolerance, relative to A + npx*B
+    double noise_rel;         // LM: likelihood resolution, relative to A
     const float* pixels;      // B*R*R
     const double* init;       // B*5 [A, B, x, y, z]
     const float* zoil;        // B per-ROI focus term z_oil (um), or null: zfocus for all
@@ -103,54 +103,45 @@ __device__ __forceinline__ double nll64(int lane, int wr, typename SY::Red& red,
     return SY::sum(acc, red, lane, wr);
 }
 
-// Gaussian elimination with partial pivoting on a 5x5 system (A destroyed).
-__device__ __forceinline__ bool solve5(double (&A)[25], double (&b)[5]) {
+// Cholesky factor Lc (lower, full storage) of the symmetric 5x5 matrix A
+// (lower triangle read); false unless A is positive definite.
+__device__ __forceinline__ bool chol5(const double (&A)[25], double (&Lc)[25]) {
+    bool ok = true;
 #pragma unroll
-    for (int c = 0; c < 5; c++) {
-        int p = c;
+    for (int j = 0; j < 5; j++) {
+        double s = A[j * 5 + j];
 #pragma unroll
-        for (int r = c + 1; r < 5; r++)
-            if (fabs(A[r * 5 + c]) > fabs(A[p * 5 + c])) p = r;
-        if (A[p * 5 + c] == 0.0) return false;
-        if (p != c) {
+        for (int k = 0; k < j; k++) s -= Lc[j * 5 + k] * Lc[j * 5 + k];
+        ok = ok && (s > 0.0);
+        Lc[j * 5 + j] = sqrt(fmax(s, 1e-300));
+        const double inv = 1.0 / Lc[j * 5 + j];
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                // swap rows c and p (p is lane-uniform; select keeps it branch free)
-                double tc = A[c * 5 + k];
-                double tp = 0.0;
+        for (int i = j + 1; i < 5; i++) {
+            double t = A[i * 5 + j];
 #pragma unroll
-                for (int r = c + 1; r < 5; r++)
-                    if (r == p) tp = A[r * 5 + k];
-                A[c * 5 + k] = tp;
-#pragma unroll
-                for (int r = c + 1; r < 5; r++)
-                    if (r == p) A[r * 5 + k] = tc;
-            }
-            double tc = b[c], tp = 0.0;
-#pragma unroll
-            for (int r = c + 1; r < 5; r++)
-                if (r == p) tp = b[r];
-            b[c] = tp;
-#pragma unroll
-            for (int r = c + 1; r < 5; r++)
-                if (r == p) b[r] = tc;
-        }
-#pragma unroll
-        for (int r = c + 1; r < 5; r++) {
-            const double f = A[r * 5 + c] / A[c * 5 + c];
-#pragma unroll
-            for (int k = c; k < 5; k++) A[r * 5 + k] -= f * A[c * 5 + k];
-            b[r] -= f * b[c];
+            for (int k = 0; k < j; k++) t -= Lc[i * 5 + k] * Lc[j * 5 + k];
+            Lc[i * 5 + j] = t * inv;
         }
     }
+    return ok;
+}
+
+// Solves (Lc Lc^T) x = b in place.
+__device__ __forceinline__ void chol5_solve(const double (&Lc)[25], double (&b)[5]) {
 #pragma unroll
-    for (int c = 4; c >= 0; c--) {
-        double s = b[c];
+    for (int i = 0; i < 5; i++) {
+        double t = b[i];
 #pragma unroll
-        for (int k = c + 1; k < 5; k++) s -= A[c * 5 + k] * b[k];
-        b[c] = s / A[c * 5 + c];
+        for (int k = 0; k < i; k++) t -= Lc[i * 5 + k] * b[k];
+        b[i] = t / Lc[i * 5 + i];
     }
-    return true;
+#pragma unroll
+    for (int i = 4; i >= 0; i--) {
+        double t = b[i];
+#pragma unroll
+        for (int k = i + 1; k < 5; k++) t -= Lc[k * 5 + i] * b[k];
+        b[i] = t / Lc[i * 5 + i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -305,18 +296,74 @@ struct Synth128 {
 };
 
 struct LmState {
-    double H[15];   // normal matrix, packed upper triangle
+    double H[15];   // Fisher matrix, packed upper triangle
+    double S[15];   // secant estimate of the rest of the Hessian, packed upper
     double gr[5];   // gradient
     double th[5];   // current point
     double tn[5];   // trial point
     double nll, lambda, nu, pred;
 };
 
-__device__ __forceinline__ double hget(const double (&H)[15], int p, int q) {
+__device__ __forceinline__ constexpr int hidx(int p, int q) {
     // packed upper triangle, row-major
-    if (p > q) { int t = p; p = q; q = t; }
-    const int off = p * 5 - (p * (p - 1)) / 2;
-    return H[off + (q - p)];
+    return p > q ? hidx(q, p) : p * 5 - (p * (p - 1)) / 2 + (q - p);
+}
+__device__ __forceinline__ double hget(const double (&H)[15], int p, int q) { return H[hidx(p, q)]; }
+
+// Structured secant correction (NL2SOL's idea, Dennis, Gay & Welsch 1981, for
+// a Poisson likelihood): the NLL Hessian is the Fisher matrix F plus
+// S = sum (1 - n/mu) d2mu, which Fisher scoring drops.  After an accepted
+// step s = tn - th, S is sized by min(1, |s'y|/|s'Ss|) and updated with
+// Powell's symmetric Broyden formula so that S s = y = g+ - g - F+ s, in the
+// variables scaled by sqrt(diag F+).  acc = [g+ (5), F+ (packed 15)].
+__device__ __forceinline__ void lm_secant_update(LmState& L, const double (&acc)[20]) {
+    double D[5], iD[5], sh[5], yh[5];
+    bool ok = true;
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+        const double f = acc[5 + hidx(p, p)];
+        ok = ok && (f > 0.0);
+        D[p] = sqrt(fmax(f, 1e-300));
+        iD[p] = 1.0 / D[p];
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+        const double sp = L.tn[p] - L.th[p];
+        double fs = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) fs += acc[5 + hidx(p, q)] * (L.tn[q] - L.th[q]);
+        sh[p] = D[p] * sp;
+        yh[p] = (acc[p] - L.gr[p] - fs) * iD[p];
+    }
+    double Ssh[5], sSs = 0.0, sy = 0.0, ss = 0.0;
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) t += L.S[hidx(p, q)] * iD[p] * iD[q] * sh[q];
+        Ssh[p] = t;
+        sSs += sh[p] * t;
+        sy += sh[p] * yh[p];
+        ss += sh[p] * sh[p];
+    }
+    if (!(ss > 0.0)) return;
+    const double tau = sSs != 0.0 ? fmin(1.0, fabs(sy) / fabs(sSs)) : 1.0;
+    double r[5], rs = 0.0;
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+        r[p] = yh[p] - tau * Ssh[p];
+        rs += r[p] * sh[p];
+    }
+    const double iss = 1.0 / ss;
+#pragma unroll
+    for (int p = 0; p < 5; p++)
+#pragma unroll
+        for (int q = p; q < 5; q++) {
+            const double v = tau * L.S[hidx(p, q)] * iD[p] * iD[q] + (r[p] * sh[q] + sh[p] * r[q]) * iss -
+                             rs * sh[p] * sh[q] * iss * iss;
+            L.S[hidx(p, q)] = v * D[p] * D[q];
+        }
 }
 
 // One LDS struct fixes the layout: shared tables first (low addresses, so lane
@@ -376,11 +423,15 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         const float* pix = A.pixels + (size_t)b * R * R;
         const float zoil = A.zoil ? A.zoil[b] : (float)A.zfocus;
         // Wave-uniform FP64 state lives in LDS between uses; registers are kept
-        // for the transforms.  Damping follows Nielsen's rule: on success
+        // for the transforms.  The step solves (F + S + lambda diag F) d = -g by
+        // Cholesky (F Fisher matrix, S its secant correction); a system that is
+        // not positive definite counts as a failed trial.  Damping follows
+        // Nielsen's rule: on success
         // lambda *= max(1/3, 1-(2 rho-1)^3) with rho the ratio of actual to
         // predicted decrease; on failure lambda *= nu, nu *= 2.  A trial is
-        // accepted when the likelihood does not rise by more than the FP32
-        // synthesis noise (noise_rel * (A + npx B)).
+        // accepted when the likelihood does not rise; a rejected trial whose
+        // predicted gain is below the FP32 synthesis noise of the likelihood
+        // (noise_rel * A; measured about 1e-9 A) ends the fit as converged.
     #pragma unroll
         for (int p = 0; p < 5; p++) L.th[p] = A.init[(size_t)b * 5 + p];
         L.th[0] = fmax(L.th[0], 1.0);
@@ -388,6 +439,8 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
     #pragma unroll
         for (int p = 0; p < 5; p++) L.tn[p] = L.th[p];
         L.nll = 0.0;
+    #pragma unroll
+        for (int k = 0; k < 15; k++) L.S[k] = 0.0;
         L.lambda = 1e-3;
         L.nu = 2.0;
         L.pred = 0.0;
@@ -396,27 +449,28 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         int it = 0, tries = 0, conv = 0, nan = 0, nfield = 0;
         bool first = true;
         while (true) {
-            bool eval_now = true;
+            bool eval_now = true, stop_noise = false;
             if (!first) {
-                double Am[25], d[5];
+                double Am[25], Lc[25], d[5];
     #pragma unroll
                 for (int p = 0; p < 5; p++) {
     #pragma unroll
-                    for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
-                    Am[p * 5 + p] *= (1.0 + L.lambda);
+                    for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q) + hget(L.S, p, q);
+                    Am[p * 5 + p] += L.lambda * hget(L.H, p, p);
                     d[p] = -L.gr[p];
                 }
-                if (!solve5(Am, d)) {
+                if (!chol5(Am, Lc)) {
                     L.lambda *= L.nu;
                     L.nu *= 2;
                     eval_now = false;
                 } else {
+                    chol5_solve(Lc, d);
                     double pred = 0.0;
     #pragma unroll
                     for (int p = 0; p < 5; p++) {
                         double hd = 0.0;
     #pragma unroll
-                        for (int q = 0; q < 5; q++) hd += hget(L.H, p, q) * d[q];
+                        for (int q = 0; q < 5; q++) hd += (hget(L.H, p, q) + hget(L.S, p, q)) * d[q];
                         pred -= L.gr[p] * d[p] + 0.5 * d[p] * hd;
                     }
                     L.pred = pred;
@@ -457,9 +511,10 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                 const double tA = L.tn[0], tB = L.tn[1];
                 const double nn = nll64<SY, R>(lane, wr, red, psf, n, tA, tB);
                 if (nn != nn) nan = 1;
-                const double noise = A.noise_rel * (L.th[0] + (double)(R * R) * L.th[1]);
-                accepted = first || nn <= L.nll + noise;
+                accepted = first || nn <= L.nll;
                 if (!accepted) {
+                    // a gain predicted below the likelihood noise cannot be resolved
+                    if (L.pred <= A.noise_rel * L.th[0]) stop_noise = true;
                     L.lambda *= L.nu;
                     L.nu *= 2;
                 } else {
@@ -472,8 +527,6 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                         L.nu = 2.0;
                         if (step < A.tol) conv = 1;
                     }
-    #pragma unroll
-                    for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
                     L.nll = nn;
                     // d psf = (sgn dv - psf * sum(sgn dv)) / S for the three tangents
                     float ss[3] = {0.f, 0.f, 0.f};
@@ -513,6 +566,9 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                         }
                     }
                     SY::template sumv<20>(acc, red, lane, wr);
+                    if (!first) lm_secant_update(L, acc);
+    #pragma unroll
+                    for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
     #pragma unroll
                     for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
     #pragma unroll
@@ -526,7 +582,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                 continue;
             }
             tries++;
-            if (accepted || tries >= 12) {
+            if (accepted || tries >= 12 || stop_noise) {
                 if (!accepted) conv = 1;
                 it++;
                 tries = 0;
@@ -535,16 +591,21 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         }
         // CRLB of x, y, z from the Fisher matrix at the optimum
         double cr[3];
+        {
+            double Am[25], Lc[25];
     #pragma unroll
-        for (int c = 2; c < 5; c++) {
-            double Am[25], e[5];
-    #pragma unroll
-            for (int p = 0; p < 5; p++) {
+            for (int p = 0; p < 5; p++)
     #pragma unroll
                 for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
-                e[p] = (p == c) ? 1.0 : 0.0;
+            const bool pd = chol5(Am, Lc);
+    #pragma unroll
+            for (int c = 2; c < 5; c++) {
+                double e[5];
+    #pragma unroll
+                for (int p = 0; p < 5; p++) e[p] = (p == c) ? 1.0 : 0.0;
+                chol5_solve(Lc, e);
+                cr[c - 2] = pd ? sqrt(e[c]) : (double)NAN;
             }
-            cr[c - 2] = solve5(Am, e) ? sqrt(e[c]) : (double)NAN;
         }
         if (lane == 0 && wr == 0) {
     #pragma unroll

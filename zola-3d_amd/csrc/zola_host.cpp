@@ -477,7 +477,7 @@ int zola_fit_batch_zoil_dev(zola_ctx* c, const zola_fit_config* cfg, int B, cons
     A.zfocus = cfg->zfocus;
     A.thr = cfg->photon_threshold;
     A.tol = cfg->tol;
-    A.noise_rel = 2e-7;
+    A.noise_rel = 1e-8;
     A.pixels = pixels;
     A.init = init;
     A.zoil = zoil;
@@ -533,7 +533,7 @@ int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int
             A.zfocus = cfg->zfocus;
             A.thr = cfg->photon_threshold;
             A.tol = cfg->tol;
-            A.noise_rel = 2e-7;
+            A.noise_rel = 1e-8;
             A.pixels = pixels[g];
             A.init = init[g];
             A.queue = c->d_queue + g;

@@ -106,7 +106,7 @@ __device__ __forceinline__ void stage1_64(const BlockTables64& T, WaveScratch64&
         const float kzv = T.kz[wi];
         const float q = T.phi[wi] + Xh * (float)nu + cY + kzv * z - T.kzo[wi] * zoil;
         float s, c;
-        sincosf(q, &s, &c);
+        sincos_reduced(q, s, c);
         const float a = T.amp[wi];
         const float pr = a * c, pi = a * s;
         const float f = fx * ((float)nu + 0.5f) + fy + fz * kzv;
