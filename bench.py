@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--no-formats", action="store_true")
     p.add_argument("--n128-batch", type=int, default=65536)
     p.add_argument("--mixed-total", type=int, default=200000)
-    p.add_argument("--frames", type=int, default=64)
+    p.add_argument("--frames", type=int, default=256)
     p.add_argument("--cpu-sample", type=int, default=0, help="ROIs for the CPU baseline (0: auto)")
     p.add_argument("--copy-probe-mb", type=int, default=0,
                    help="also run one device copy of this size (reference for HBM counters)")

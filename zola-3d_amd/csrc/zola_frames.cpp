@@ -18,10 +18,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/zola_hip.h"
@@ -170,7 +172,9 @@ void free_fit(zola_detector* d) {
 int batch_frames(const zola_detector* d, int F) {
     const zola::DetGeom& g = d->g;
     const double per = (double)g.nZ * g.W * g.H * 8 + (double)g.W * g.H * (4 + 24) + (double)g.FX * g.FY * 16;
-    int fb = (int)std::min(64.0, std::max(1.0, (double)(2u << 30) / per));
+    // up to 256 frames in flight (8 GiB of correlation maps and spectra): more
+    // frames per batch give the fit rounds more ROIs and the walk more threads
+    int fb = (int)std::min(256.0, std::max(1.0, 8.0 * (1u << 30) / per));
     return std::max(1, std::min(fb, F));
 }
 
@@ -428,6 +432,7 @@ struct FrameRun {
     std::vector<double> ab;            // n x 3 (A, B, max patch)
     std::vector<uint8_t> ab_ok;        // n
     std::vector<double> locs;          // records
+    std::vector<int> q;                // candidates queued for the next fit round
 };
 
 struct Pipe {
@@ -451,8 +456,9 @@ inline bool in_bounds(const Pipe& P, int ci, int cj) {
 bool patch_touches(const Pipe& P, const std::vector<uint8_t>& m, int ci, int cj) {
     for (int a = -P.half; a < P.R - P.half; a++) {
         const uint8_t* row = m.data() + (size_t)(ci + a) * P.H + cj - P.half;
-        for (int b = 0; b < P.R; b++)
-            if (row[b]) return true;
+        uint8_t any = 0;  // OR over the row without an early exit: vectorises
+        for (int b = 0; b < P.R; b++) any |= row[b];
+        if (any) return true;
     }
     return false;
 }
@@ -495,11 +501,10 @@ const double* get_ab(const Pipe& P, FrameRun& fr, int k) {
 
 inline bool snr_gate(const Pipe& P, const double* ab) { return (ab[2] / ab[1] > P.acc->snr_min) && (ab[1] > 0); }
 
-void queue_fit(Pipe& P, FrameRun& fr, int run, int k) {
+void queue_fit(Pipe&, FrameRun& fr, int, int k) {
     if (fr.queued[k] || fr.res_idx[k] >= 0) return;
     fr.queued[k] = 1;
-    P.q_frame.push_back(run);
-    P.q_k.push_back(k);
+    fr.q.push_back(k);
 }
 
 // The accept branch of LocalizationThread.run (LocalizationPipeline.java:857-948)
@@ -585,6 +590,12 @@ struct Unc {
         return false;
     }
 };
+
+// Host threads for the walk replay: the cores this process may use, at most 16.
+int host_threads() {
+    const unsigned n = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(n, 16u));
+}
 
 // Look-ahead after the walk stopped at k0: queue the fits that the coming
 // rounds will need, as far as the known mask can tell.
@@ -801,9 +812,26 @@ extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         P.results.clear();
         for (;;) {
+            // Frames are independent: their walks replay on host threads; the
+            // queued fits are then gathered in frame order.
+            std::atomic<int> next{0};
+            auto work = [&]() {
+                for (int f; (f = next.fetch_add(1)) < fb;) replay(P, runs[f], f);
+            };
+            const int nt = std::max(1, std::min(fb, host_threads()));
+            std::vector<std::thread> pool;
+            for (int t = 1; t < nt; t++) pool.emplace_back(work);
+            work();
+            for (auto& t : pool) t.join();
             P.q_frame.clear();
             P.q_k.clear();
-            for (int f = 0; f < fb; f++) replay(P, runs[f], f);
+            for (int f = 0; f < fb; f++) {
+                for (int k : runs[f].q) {
+                    P.q_frame.push_back(f);
+                    P.q_k.push_back(k);
+                }
+                runs[f].q.clear();
+            }
             if (P.q_k.empty()) break;
             rc = fit_queue(d, P, runs);
             if (rc) return rc;
