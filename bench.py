@@ -237,6 +237,12 @@ def mixed_bench(torch, Z, stream, total):
                            status=torch.empty(Bg, dtype=torch.int32, device="cuda")))
         ctxs.append(c)
     ctx = ctxs[1]
+    # autotuner: per ROI side, the faster of one wave and four waves per ROI
+    layouts = {}
+    for Rg in sizes:
+        lay, t_wave, t_quad = ctx.autotune_fit(Rg, Z.FIT_LM, B=8192)
+        layouts[str(Rg)] = {"layout": "quad" if lay == Z.LAYOUT_QUAD else "wave",
+                            "ms_wave": round(t_wave, 3), "ms_quad": round(t_quad, 3)}
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
     ctx.fit_mixed_dev(cfg, groups)
     e = _events(torch)
@@ -258,7 +264,7 @@ def mixed_bench(torch, Z, stream, total):
     return {"workload": f"{B} ROIs mixed 40/30/20/10% of 17/21/31/41 px, 64x64 pupil, photons 200..2e4, "
                         "one graph launch (one kernel node per size)",
             "rois": B, "graph_ms": round(ms, 3), "value": round(B / (ms / 1e3), 1), "unit": "localizations/s",
-            "groups": per}
+            "groups": per, "autotune_8192_rois": layouts}
 
 
 def frames_bench(torch, Z, stream, F):

@@ -106,6 +106,16 @@ int  zola_psf_backward_dev(zola_ctx* ctx, int B, const float* params, const floa
  * transforms) the fit computed for that ROI: status >> 8. */
 #define ZOLA_ST_NFIELD_SHIFT 8
 
+/* Work layout of the 64-grid fit kernels (no counterpart in the reference,
+ * whose GPU batch shape is fixed by the 100-ROI slots of
+ * process/gpu/Modelmany_double_.java:271-387):
+ * ZOLA_LAYOUT_WAVE one wave per ROI; ZOLA_LAYOUT_QUAD four waves (one block)
+ * per ROI, ceil(R/4) ROI rows per lane; ZOLA_LAYOUT_AUTO (the default) one
+ * wave up to R = 21, four above. */
+#define ZOLA_LAYOUT_AUTO 0
+#define ZOLA_LAYOUT_WAVE 1
+#define ZOLA_LAYOUT_QUAD 2
+
 typedef struct zola_fit_config {
     int    mode;             /* ZOLA_FIT_*                                    */
     int    max_iter;         /* iterMax (reference default 20)                */
@@ -135,6 +145,19 @@ int  zola_fit_batch_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
 int  zola_fit_batch_zoil_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
                              const float* pixels, const double* init, const float* zoil,
                              double* theta, double* nll, double* crlb, int* iters, int* status);
+
+/* Sets the work layout (ZOLA_LAYOUT_*) of the 64-grid fits of ROI side R in
+ * fit mode `mode` for this context.  Results of the two layouts agree to FP32
+ * rounding of the blur sums (a different summation order). */
+int  zola_set_fit_layout(zola_ctx* ctx, int R, int mode, int layout);
+
+/* Autotuner of the fit layout: fits a synthetic batch of B ROIs of side R
+ * (noise-free model images, started off the truth) with each layout on the
+ * context's stream, keeps the faster one for (R, mode) and returns it in
+ * *layout, with the two kernel times in ms[0] (one wave per ROI) and ms[1]
+ * (four waves per ROI).  Synchronises the stream; not for use under graph
+ * capture. */
+int  zola_autotune_fit(zola_ctx* ctx, int R, int mode, int B, int* layout, double* ms);
 
 /* Fits G groups of ROIs of different sides R[g] (device pointers, one set per
  * group; 1 <= G <= 8) in one graph launch on the context's stream: one kernel

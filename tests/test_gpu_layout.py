@@ -1,0 +1,94 @@
+"""64-grid fits in both work layouts (one wave per ROI, four waves per ROI) at
+every instantiated ROI side, against the FP64 oracle:
+
+* LM form: the FP64 Levenberg-Marquardt optimum (0.5 nm x/y, 1 nm z, 1e-5
+  relative NLL);
+* reference form: the FP64 restatement of cpu/Localization.localize from the
+  same start (0.5 nm x/y, 1 nm z where both converge);
+* the autotuner returns a valid layout and the fits that follow use it.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import simdata
+import zola_hip as Z
+
+pytestmark = pytest.mark.gpu
+
+B = 12
+ZMIN_T, ZSTEP_T, NZ = -1.0, 0.1, 21
+_cache = {}
+
+
+def setup(R):
+    if R not in _cache:
+        a = Z.astigmatic_coeffs(28)
+        m = O.Model(O.optics(), R, a)
+        rois, truth = simdata.isolated_rois(m, B, seed=300 + R)
+        tmpl = simdata.templates(m, ZMIN_T, ZSTEP_T, NZ)
+        init = simdata.init_restated(rois, tmpl, ZMIN_T, ZSTEP_T)
+        opt = [m.lm_fit(rois[b].astype(np.float64), init[b], zmin=-1.3, zmax=1.3, maxit=500, tol=1e-10)
+               for b in range(B)]
+        _cache[R] = (m, a, rois, init, opt)
+    return _cache[R]
+
+
+@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD])
+@pytest.mark.parametrize("R", [17, 21, 31, 41])
+def test_lm_layout_matches_fp64_optimum(R, layout):
+    m, a, rois, init, opt = setup(R)
+    print(f"R {R} layout {layout}", flush=True)
+    ctx = Z.Context(Z.optics(), R, a)
+    ctx.set_fit_layout(R, Z.FIT_LM, layout)
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    res = ctx.fit(cfg, rois, init)
+    ctx.close()
+    th = np.array([r["theta"] for r in opt])
+    nll = np.array([r["nll"] for r in opt])
+    d = (res["theta"][:, 2:] - th[:, 2:]) * 1000.0
+    print("max |dx|,|dy|,|dz| nm", np.abs(d).max(axis=0), "iters", res["iters"])
+    assert ((res["status"] & Z.ST_CONVERGED) != 0).all()
+    assert np.abs(d[:, :2]).max() <= 0.5
+    assert np.abs(d[:, 2]).max() <= 1.0
+    assert np.all(np.abs(res["nll"] - nll) <= 1e-5 * np.abs(nll))
+    np.testing.assert_allclose(res["crlb"], np.array([r["crlb"] for r in opt]), rtol=1e-3)
+
+
+@pytest.mark.parametrize("R", [31, 41])
+def test_reference_form_quad_layout_matches_restatement(R):
+    m, a, rois, init, opt = setup(R)
+    ctx = Z.Context(Z.optics(), R, a)
+    ctx.set_fit_layout(R, Z.FIT_REFERENCE, Z.LAYOUT_QUAD)
+    cfg = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=-1.3, zmax=1.3, photon_threshold=0.0)
+    res = ctx.fit(cfg, rois, init)
+    ctx.close()
+    O.lib().zo_set_psf_cache(1)
+    try:
+        ref = [m.localize(rois[b].astype(np.float64), init[b], minZ=-1.3, maxZ=1.3, iterMax=20) for b in range(B)]
+    finally:
+        O.lib().zo_set_psf_cache(0)
+    th_ref = np.array([r["theta"] for r in ref])
+    it_ref = np.array([r["iters"] for r in ref])
+    d = (res["theta"][:, 2:] - th_ref[:, 2:]) * 1000.0
+    both = (it_ref < 20) & (res["iters"] < 20)
+    print("ref-form |d| nm max:", np.abs(d[both]).max(axis=0), "iters gpu", res["iters"], "cpu", it_ref)
+    assert both.sum() >= B // 2
+    assert np.abs(d[both][:, :2]).max() <= 0.5
+    assert np.abs(d[both][:, 2]).max() <= 1.0
+
+
+def test_autotune_picks_a_layout_and_fits_follow_it():
+    m, a, rois, init, opt = setup(31)
+    ctx = Z.Context(Z.optics(), 31, a)
+    lay, t_wave, t_quad = ctx.autotune_fit(31, Z.FIT_LM, B=2048)
+    print("autotune R 31:", lay, t_wave, t_quad)
+    assert lay in (Z.LAYOUT_WAVE, Z.LAYOUT_QUAD)
+    assert t_wave > 0 and t_quad > 0
+    assert (lay == Z.LAYOUT_QUAD) == (t_quad < t_wave)
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    got = ctx.fit(cfg, rois, init)
+    ctx.set_fit_layout(31, Z.FIT_LM, lay)
+    again = ctx.fit(cfg, rois, init)
+    ctx.close()
+    np.testing.assert_array_equal(got["theta"], again["theta"])

@@ -41,7 +41,13 @@ struct zola_ctx {
     hipStream_t side[8] = {};
     hipStream_t cap = nullptr;    // private capture stream (the context stream may be the legacy default)
     hipEvent_t fork = nullptr, join[8] = {};
+    // work layout of the 64-grid fits per ROI side (17, 21, 31, 41) and mode
+    // (reference form, LM): ZOLA_LAYOUT_*
+    int layout[4][2] = {};
 };
+
+// Four waves per ROI (1) or one wave per ROI (0) for a 64-grid fit.
+int zola_fit_quad(const zola_ctx* c, int R, int mode);
 
 inline int set_err(zola_ctx* c, int code, const char* fmt, ...) {
     if (c) {

@@ -14,6 +14,8 @@ hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_
 template <int R>
 hipError_t launch_adjoint_r(const Model64& M, const AdjArgs& A, hipStream_t st);
 template <int R>
+hipError_t launch_fit64q_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st);
+template <int R>
 hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params, float* out, hipStream_t st);
 template <int R>
 hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st);
@@ -70,7 +72,12 @@ hipError_t zola_launch_psf_forward64(const Model64& M, int R, int B, const float
 }
 
 
-hipError_t zola_launch_fit64(const Model64& M, int R, int mode, const FitArgs& A, hipStream_t st) {
+hipError_t zola_launch_fit64(const Model64& M, int R, int mode, int quad, const FitArgs& A, hipStream_t st) {
+    if (quad) {
+#define CALL(RR) launch_fit64q_r<RR>(M, mode, A, st)
+        ZOLA_R_DISPATCH(R, CALL)
+#undef CALL
+    }
 #define CALL(RR) launch_fit_r<RR>(M, mode, A, st)
     ZOLA_R_DISPATCH(R, CALL)
 #undef CALL

@@ -134,6 +134,11 @@ __device__ __forceinline__ void sincos_reduced(float q, float& s, float& c) {
     c = ((qd + 1) & 2) ? -ca : ca;
 }
 
+// Value of v in lane l, broadcast to the wave (l wave-uniform).
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -21,6 +21,7 @@
 #pragma once
 #include "zola_psf64.hpp"
 #include "zola_psf128.hpp"
+#include "zola_psf64q.hpp"
 
 namespace zola {
 
@@ -224,18 +225,15 @@ struct Synth64 {
     }
 };
 
-// Four waves per ROI (zola_psf128.hpp); lane = ROI column j, wave wr owns
-// rows wr*RQ + t.  Every reduction is summed over the four waves through LDS
-// in a fixed order, so all waves see identical values and take identical
-// branches (the block barriers inside the evaluations stay matched).
-struct Synth128 {
-    using Model = Model128;
-    using Shared = NoShared;
+// Four waves per ROI (zola_psf128.hpp, zola_psf64q.hpp); lane = ROI column
+// j, wave wr owns rows wr*RQ + t (RQ = ceil(R/4)).  Every reduction is summed
+// over the four waves through LDS in a fixed order, so all waves see identical
+// values and take identical branches (the block barriers inside the
+// evaluations stay matched).
+struct QuadWaves {
     static constexpr int kWaves = 4;
-    static constexpr int kLmFloats = 2 * (int)(sizeof(Buf128) / sizeof(float));
-    static constexpr int kRefFloats = kLmFloats;
     template <int R>
-    static constexpr int RPL = Roi128<R>::RQ;
+    static constexpr int RPL = (R + 3) / 4;
     template <int R>
     __device__ static bool valid(int lane, int wr, int t) { return lane < R && wr * RPL<R> + t < R; }
     template <int R>
@@ -280,6 +278,13 @@ struct Synth128 {
         sumv<1>(a, r, lane, wr);
         return a[0];
     }
+};
+
+struct Synth128 : QuadWaves {
+    using Model = Model128;
+    using Shared = NoShared;
+    static constexpr int kLmFloats = 2 * (int)(sizeof(Buf128) / sizeof(float));
+    static constexpr int kRefFloats = kLmFloats;
     __device__ static void load(Shared&, const Model&) {}
     template <int R>
     __device__ static void eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
@@ -292,6 +297,27 @@ struct Synth128 {
                                    float z, float zoil, float (&vB)[Roi128<R>::RQ], float (&vX)[Roi128<R>::RQ],
                                    float (&vY)[Roi128<R>::RQ], float (&vZ)[Roi128<R>::RQ]) {
         eval128<R, true>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
+    }
+};
+
+// The 64 grid by four waves per ROI: stage-1 tables in LDS as for one wave.
+struct Synth64q : QuadWaves {
+    using Model = Model64;
+    using Shared = BlockTables64;
+    static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
+    static constexpr int kRefFloats = (int)(sizeof(WaveScratch64) / sizeof(float));
+    __device__ static void load(Shared& T, const Model& M) { load_tables64(T, M); }
+    template <int R>
+    __device__ static void eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
+                                float z, float zoil, float (&v)[Roi64q<R>::RQ]) {
+        float d0[Roi64q<R>::RQ], d1[Roi64q<R>::RQ], d2[Roi64q<R>::RQ];
+        eval64q<R, false>(M, T, scr, lane, wr, x, y, z, zoil, v, d0, d1, d2);
+    }
+    template <int R>
+    __device__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
+                                   float z, float zoil, float (&vB)[Roi64q<R>::RQ], float (&vX)[Roi64q<R>::RQ],
+                                   float (&vY)[Roi64q<R>::RQ], float (&vZ)[Roi64q<R>::RQ]) {
+        eval64q<R, true>(M, T, scr, lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
     }
 };
 
@@ -631,6 +657,12 @@ template <int R, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fit_lm128(Model128 M, FitArgs A) {
     __shared__ LmLds<Synth128, WPB> lds;
     fit_lm_body<Synth128, R, WPB>(M, A, lds);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_fit_lm64q(Model64 M, FitArgs A) {
+    __shared__ LmLds<Synth64q, 4> lds;
+    fit_lm_body<Synth64q, R, 4>(M, A, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -999,6 +1031,12 @@ template <int R, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fit_ref128(Model128 M, FitArgs A) {
     __shared__ RefLds<Synth128, WPB> lds;
     fit_ref_body<Synth128, R, WPB>(M, A, lds);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_fit_ref64q(Model64 M, FitArgs A) {
+    __shared__ RefLds<Synth64q, 4> lds;
+    fit_ref_body<Synth64q, R, 4>(M, A, lds);
 }
 
 }  // namespace zola

@@ -134,6 +134,18 @@ static int upload_tables(zola_ctx* c) {
     return ZOLA_OK;
 }
 
+namespace {
+int roi_slot(int R) { return R == 17 ? 0 : R == 21 ? 1 : R == 31 ? 2 : R == 41 ? 3 : -1; }
+}  // namespace
+
+int zola_fit_quad(const zola_ctx* c, int R, int mode) {
+    const int i = roi_slot(R);
+    const int l = i >= 0 ? c->layout[i][mode == ZOLA_FIT_LM ? 1 : 0] : ZOLA_LAYOUT_AUTO;
+    if (l == ZOLA_LAYOUT_WAVE) return 0;
+    if (l == ZOLA_LAYOUT_QUAD) return 1;
+    return R >= 31 ? 1 : 0;  // one wave holds ceil(R/3) or more ROI rows per lane above 21
+}
+
 extern "C" {
 
 int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zola_ctx** out) {
@@ -488,7 +500,120 @@ int zola_fit_batch_zoil_dev(zola_ctx* c, const zola_fit_config* cfg, int B, cons
     A.iters = iters;
     A.status = status;
     if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, c->stream));
-    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, A, c->stream));
+    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, zola_fit_quad(c, c->R, cfg->mode), A, c->stream));
+    return ZOLA_OK;
+}
+
+int zola_set_fit_layout(zola_ctx* c, int R, int mode, int layout) {
+    if (!c) return ZOLA_EINVAL;
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "fit layouts apply to the 64 grid");
+    const int i = roi_slot(R);
+    if (i < 0) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available", R);
+    if (mode != ZOLA_FIT_REFERENCE && mode != ZOLA_FIT_LM) return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", mode);
+    if (layout < ZOLA_LAYOUT_AUTO || layout > ZOLA_LAYOUT_QUAD) return set_err(c, ZOLA_EINVAL, "unknown layout %d", layout);
+    c->layout[i][mode == ZOLA_FIT_LM ? 1 : 0] = layout;
+    return ZOLA_OK;
+}
+
+int zola_autotune_fit(zola_ctx* c, int R, int mode, int B, int* layout, double* ms) {
+    if (!c) return ZOLA_EINVAL;
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "fit layouts apply to the 64 grid");
+    if (roi_slot(R) < 0) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available", R);
+    if (mode != ZOLA_FIT_REFERENCE && mode != ZOLA_FIT_LM) return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", mode);
+    if (B < 64 || B > (1 << 20)) return set_err(c, ZOLA_EINVAL, "autotune batch %d outside [64, 2^20]", B);
+    // Synthetic batch: noise-free model images A psf + bg at spread (x, y, z),
+    // started off the truth, so that both layouts run typical iteration counts.
+    const size_t R2 = (size_t)R * R;
+    std::vector<float> par((size_t)B * 4);
+    std::vector<double> init((size_t)B * 5);
+    uint32_t st = 0x9E3779B9u;
+    auto uni = [&]() {
+        st = st * 1664525u + 1013904223u;
+        return (double)(st >> 8) / (double)(1u << 24);
+    };
+    for (int b = 0; b < B; b++) {
+        const double x = (2 * uni() - 1) * c->o.xystep, y = (2 * uni() - 1) * c->o.xystep, z = 2 * uni() - 1;
+        par[b * 4 + 0] = (float)x;
+        par[b * 4 + 1] = (float)y;
+        par[b * 4 + 2] = (float)z;
+        par[b * 4 + 3] = 0.f;
+        init[b * 5 + 0] = 1500.0;
+        init[b * 5 + 1] = 15.0;
+        init[b * 5 + 2] = x + 0.3 * c->o.xystep;
+        init[b * 5 + 3] = y - 0.3 * c->o.xystep;
+        init[b * 5 + 4] = z + 0.05;
+    }
+    float *d_par = nullptr, *d_px = nullptr;
+    double *d_init = nullptr, *d_th = nullptr, *d_nll = nullptr, *d_cr = nullptr;
+    int *d_it = nullptr, *d_st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ZOLA_OK;
+    double t[2] = {0, 0};
+    auto run = [&]() -> int {
+        HIP_TRY(c, hipMalloc((void**)&d_par, par.size() * sizeof(float)));
+        HIP_TRY(c, hipMalloc((void**)&d_px, (size_t)B * R2 * sizeof(float)));
+        HIP_TRY(c, hipMalloc((void**)&d_init, init.size() * sizeof(double)));
+        HIP_TRY(c, hipMalloc((void**)&d_th, (size_t)B * 5 * sizeof(double)));
+        HIP_TRY(c, hipMalloc((void**)&d_nll, (size_t)B * sizeof(double)));
+        HIP_TRY(c, hipMalloc((void**)&d_cr, (size_t)B * 3 * sizeof(double)));
+        HIP_TRY(c, hipMalloc((void**)&d_it, (size_t)B * sizeof(int)));
+        HIP_TRY(c, hipMalloc((void**)&d_st, (size_t)B * sizeof(int)));
+        HIP_TRY(c, hipEventCreate(&e0));
+        HIP_TRY(c, hipEventCreate(&e1));
+        HIP_TRY(c, hipMemcpyAsync(d_par, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(d_init, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, zola_launch_psf_forward64(c->model, R, B, d_par, d_px, c->stream));
+        std::vector<float> px((size_t)B * R2);
+        HIP_TRY(c, hipMemcpyAsync(px.data(), d_px, px.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < px.size(); k++) px[k] = 1000.f * px[k] + 10.f;
+        HIP_TRY(c, hipMemcpyAsync(d_px, px.data(), px.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        zola::FitArgs A{};
+        A.B = B;
+        A.max_iter = 30;
+        A.xystep = (float)c->o.xystep;
+        A.zmin = -1.3;
+        A.zmax = 1.3;
+        A.tol = 1e-4;
+        A.noise_rel = 1e-8;
+        A.pixels = d_px;
+        A.init = d_init;
+        A.queue = c->d_queue;
+        A.theta = d_th;
+        A.nll = d_nll;
+        A.crlb = d_cr;
+        A.iters = d_it;
+        A.status = d_st;
+        for (int q = 0; q < 2; q++) {
+            HIP_TRY(c, zola_launch_fit64(c->model, R, mode, q, A, c->stream));  // warm-up
+            float best = 1e30f;
+            for (int rep = 0; rep < 2; rep++) {
+                HIP_TRY(c, hipEventRecord(e0, c->stream));
+                HIP_TRY(c, zola_launch_fit64(c->model, R, mode, q, A, c->stream));
+                HIP_TRY(c, hipEventRecord(e1, c->stream));
+                HIP_TRY(c, hipEventSynchronize(e1));
+                float m = 0.f;
+                HIP_TRY(c, hipEventElapsedTime(&m, e0, e1));
+                best = std::min(best, m);
+            }
+            t[q] = best;
+        }
+        return ZOLA_OK;
+    };
+    rc = run();
+    for (void* p : {(void*)d_par, (void*)d_px, (void*)d_init, (void*)d_th, (void*)d_nll, (void*)d_cr, (void*)d_it,
+                    (void*)d_st})
+        if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    const int choice = t[1] < t[0] ? ZOLA_LAYOUT_QUAD : ZOLA_LAYOUT_WAVE;
+    c->layout[roi_slot(R)][mode == ZOLA_FIT_LM ? 1 : 0] = choice;
+    if (layout) *layout = choice;
+    if (ms) {
+        ms[0] = t[0];
+        ms[1] = t[1];
+    }
     return ZOLA_OK;
 }
 
@@ -543,7 +668,8 @@ int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int
             A.iters = iters[g];
             A.status = status[g];
             err = c->N == zola::kN2 ? zola_launch_fit128(c->model128, R[g], cfg->mode, A, c->side[g])
-                                    : zola_launch_fit64(c->model, R[g], cfg->mode, A, c->side[g]);
+                                    : zola_launch_fit64(c->model, R[g], cfg->mode, zola_fit_quad(c, R[g], cfg->mode),
+                                                        A, c->side[g]);
             if (err != hipSuccess) break;
         }
         err = hipEventRecord(c->join[g], c->side[g]);

@@ -1,0 +1,6 @@
+// Kernel instantiations for ROI side 21, 64 grid, four waves per ROI.
+#include "zola_kernels_impl.hpp"
+
+namespace zola {
+template hipError_t launch_fit64q_r<21>(const Model64&, int, const FitArgs&, hipStream_t);
+}  // namespace zola

@@ -14,7 +14,8 @@ bool zola_kernels_support_roi(int R);
 hipError_t zola_launch_psf_forward64(const zola::Model64& M, int R, int B, const float* params,
                                      float* out, hipStream_t st);
 
-hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, const zola::FitArgs& A,
+// quad: 0 one wave per ROI (zola_psf64.hpp), 1 four waves per ROI (zola_psf64q.hpp)
+hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, int quad, const zola::FitArgs& A,
                              hipStream_t st);
 
 hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,

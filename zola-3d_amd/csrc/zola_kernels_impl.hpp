@@ -184,6 +184,22 @@ hipError_t launch_fit_r(const Model64& M, int mode, const FitArgs& A, hipStream_
     return hipGetLastError();
 }
 
+// N = 64 with four waves (one block) per ROI, see zola_psf64q.hpp.
+template <int R>
+hipError_t launch_fit64q_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st) {
+    hipError_t e;
+    if (mode == 1) {
+        const int g = fit_grid(k_fit_lm64q<R>, 256, 1, A, st, e);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fit_lm64q<R>), dim3(g), dim3(256), 0, st, M, A);
+    } else {
+        const int g = fit_grid(k_fit_ref64q<R>, 256, 1, A, st, e);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fit_ref64q<R>), dim3(g), dim3(256), 0, st, M, A);
+    }
+    return hipGetLastError();
+}
+
 // N = 128: four waves per PSF (one block), see zola_psf128.hpp.
 template <int R>
 __global__ __launch_bounds__(256) void k_psf_forward128(Model128 M, int B, const float* __restrict__ params,

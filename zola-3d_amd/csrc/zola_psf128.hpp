@@ -59,10 +59,6 @@ struct Roi128 {
 
 __device__ __forceinline__ constexpr int slot64_n(int j) { return j < 32 ? j : j - 64; }
 
-__device__ __forceinline__ float bcast(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
 // Tap d (0..127) of a kernel held as two VGPRs (lane l holds taps l and l+64).
 template <int D>
 __device__ __forceinline__ float tap(const float (&k)[2]) {

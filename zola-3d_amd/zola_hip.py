@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libzola_hip.so")
 
 FIT_REFERENCE = 0
 FIT_LM = 1
+LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_QUAD = 0, 1, 2
 ST_CONVERGED, ST_MAXITER, ST_NAN, ST_LOWPHOTON = 1, 2, 4, 8
 
 SYMBOLS = [
@@ -28,7 +29,8 @@ SYMBOLS = [
     "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
     "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
     "zola_detector_correlation", "zola_localize_frames", "zola_fit_mixed_dev",
-    "zola_fit_batch_zoil_dev", "zola_search_psf_center", "zola_java_double_string",
+    "zola_fit_batch_zoil_dev", "zola_search_psf_center", "zola_set_fit_layout", "zola_autotune_fit",
+    "zola_java_double_string",
     "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
     "zola_formats_last_error",
 ]
@@ -115,6 +117,8 @@ def lib():
             L.zola_detector_correlation.argtypes = [vp, vp, vp]
             L.zola_localize_frames.argtypes = [vp, C.POINTER(FitConfig), C.POINTER(AcceptConfig),
                                                C.c_int, vp, C.c_int, vp, vp, vp, vp]
+        L.zola_set_fit_layout.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.zola_autotune_fit.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), vp]
         L.zola_java_double_string.argtypes = [C.c_double, C.c_char_p, C.c_int]
         L.zola_calib_json_format.restype = C.c_long
         L.zola_calib_json_format.argtypes = [C.POINTER(Optics), vp, C.c_int, C.c_char_p, C.c_long]
@@ -292,6 +296,18 @@ class Context:
                                          _ptr(theta), _ptr(nll), _ptr(crlb), _ptr(iters),
                                          _ptr(status)), "fit_batch")
         return dict(theta=theta, nll=nll, crlb=crlb, iters=iters, status=status)
+
+    def set_fit_layout(self, R, mode, layout):
+        """Work layout of the 64-grid fits of side R: LAYOUT_AUTO / WAVE / QUAD."""
+        self._check(lib().zola_set_fit_layout(self._h, R, mode, layout), "set_fit_layout")
+
+    def autotune_fit(self, R, mode, B=8192):
+        """Times both layouts on a synthetic batch, keeps the faster; returns
+        (layout, ms one wave per ROI, ms four waves per ROI)."""
+        lay = C.c_int(0)
+        ms = np.zeros(2)
+        self._check(lib().zola_autotune_fit(self._h, R, mode, B, C.byref(lay), _ptr(ms)), "autotune_fit")
+        return lay.value, float(ms[0]), float(ms[1])
 
     def search_psf_center(self, zfocus, axial_range):
         """Centre of the template z range (SearchPSFcenter.getPosition), um."""
