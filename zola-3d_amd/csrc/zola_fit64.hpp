@@ -163,8 +163,7 @@ __device__ __forceinline__ void lm_eval64(const Model64& M, const BlockTables64&
                                           float (&vZ)[RoiLayout<R>::RPL]) {
     WaveScratch64& SE = *reinterpret_cast<WaveScratch64*>(scr);
     WaveScratch64& SZ = *reinterpret_cast<WaveScratch64*>(scr + 2 * kN * kS1);
-    stage1_64(T, SE, lane, 0, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
-    stage1_64(T, SZ, lane, 3, M.kappa * x, M.kappa * y, z, zoil, M.kappa);
+    stage1_64_ez(T, SE, SZ, lane, M.kappa * x, M.kappa * y, z, zoil);
     wave_lds_fence();
     {
         float acc[3][R];

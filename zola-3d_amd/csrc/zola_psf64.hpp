@@ -122,6 +122,48 @@ __device__ __forceinline__ void stage1_64(const BlockTables64& T, WaveScratch64&
     }
 }
 
+// Stage 1 of E and dE/dz together (dE/dz = IDFT(i kz P)): the pupil phases and
+// table reads are shared, two transforms.  Same outputs as stage1_64 with
+// field 0 into SE and field 3 into SZ.
+__device__ __forceinline__ void stage1_64_ez(const BlockTables64& T, WaveScratch64& SE, WaveScratch64& SZ,
+                                             int lane, float X, float Y, float z, float zoil) {
+    const int v = lane & 31;
+    const int h = lane >> 5;
+    const float nv = (float)(v - 16);
+    const float Xh = X + (float)h * (float)(2.0 * kPi / 64.0);
+    const float cY = Y * (nv + 0.5f) + 0.5f * X;
+    float re[32], im[32], zr[32], zi[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+        const int nu = slot_n(j);
+        const int wi = (nu + 16) * kW + v;
+        const float kzv = T.kz[wi];
+        const float q = T.phi[wi] + Xh * (float)nu + cY + kzv * z - T.kzo[wi] * zoil;
+        float sn, cs;
+        sincos_reduced(q, sn, cs);
+        const float a = T.amp[wi];
+        const float pr = a * cs, pi = a * sn;
+        re[j] = pr;
+        im[j] = pi;
+        zr[j] = -kzv * pi;
+        zi[j] = kzv * pr;
+    }
+    fft_dif<32, +1>(re, im);
+#pragma unroll
+    for (int m = 0; m < 32; m++) {
+        const int a = 2 * m + h;
+        SE.re[a * kS1 + v] = re[fidx<32>(m)];
+        SE.im[a * kS1 + v] = im[fidx<32>(m)];
+    }
+    fft_dif<32, +1>(zr, zi);
+#pragma unroll
+    for (int m = 0; m < 32; m++) {
+        const int a = 2 * m + h;
+        SZ.re[a * kS1 + v] = zr[fidx<32>(m)];
+        SZ.im[a * kS1 + v] = zi[fidx<32>(m)];
+    }
+}
+
 // Stage 2 half hp for row a = lane: returns E(a, 2m'+hp) in natural order.
 template <int HP>
 __device__ __forceinline__ void stage2_half64(const WaveScratch64& S, int lane, float (&re)[32],
