@@ -159,6 +159,23 @@ int  zola_set_fit_layout(zola_ctx* ctx, int R, int mode, int layout);
  * capture. */
 int  zola_autotune_fit(zola_ctx* ctx, int R, int mode, int B, int* layout, double* ms);
 
+/* As zola_fit_batch_dev for an sCMOS camera: varg holds B x R x R floats, the
+ * per-pixel var/gain^2 of each ROI (utils/SCMOScamera.java:95), added to the
+ * model mu = A psf + B as in Localization.computeLikelihood
+ * (process/cpu/Localization.java:1131-1132); pixels are the converted
+ * (adu - offset)/gain + var/gain^2 (process/cpu/LocalizationPipeline.java:633).
+ * The reference form keeps the reference's Fisher weights 1/(A psf + B)
+ * (computeFisher, 317-403, has no sCMOS term); the LM form uses the full mu. */
+int  zola_fit_batch_scmos_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
+                              const float* pixels, const float* varg, const double* init,
+                              double* theta, double* nll, double* crlb, int* iters, int* status);
+
+/* General form of the three above: zoil and varg may each be NULL. */
+int  zola_fit_batch_ext_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
+                            const float* pixels, const double* init, const float* zoil,
+                            const float* varg, double* theta, double* nll, double* crlb,
+                            int* iters, int* status);
+
 /* Fits G groups of ROIs of different sides R[g] (device pointers, one set per
  * group; 1 <= G <= 8) in one graph launch on the context's stream: one kernel
  * node per group, the nodes independent so the groups run concurrently.  On the

@@ -60,6 +60,7 @@ def lib():
                                   C.c_double, _dp, C.POINTER(C.c_double), C.POINTER(C.c_int),
                                   _dp, C.POINTER(C.c_int)]
         L.zo_set_psf_cache.argtypes = [C.c_int]
+        L.zo_set_scmos.argtypes = [C.c_void_p]
         L.zo_localize_batch.argtypes = [C.POINTER(Optics), C.c_int, C.c_void_p, C.c_int, C.c_int,
                                         _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_double, _dp, _dp, _ip, _ip, C.c_int]
@@ -193,6 +194,20 @@ class Model:
         F = np.zeros((5, 5))
         lib().zo_fisher_analytic(self._h, _d(theta), zfocus, F)
         return F
+
+
+_scmos_keep = None
+
+
+def set_scmos(varg):
+    """sCMOS map (R x R, var/gain^2) added to mu by localize and lm_fit, or None."""
+    global _scmos_keep
+    if varg is None:
+        _scmos_keep = None
+        lib().zo_set_scmos(None)
+    else:
+        _scmos_keep = np.ascontiguousarray(varg, dtype=np.float64)
+        lib().zo_set_scmos(_scmos_keep.ctypes.data)
 
 
 def fft2d(re, im, sign):

@@ -597,13 +597,18 @@ typedef struct {
     int iterMax;
 } loc_t;
 
+/* sCMOS term var/gain^2 per ROI pixel (R x R, utils/SCMOScamera.java:95) added
+ * to the model of both fit forms, or NULL (Localization.java:1131-1132). */
+static const double* scmos_map = NULL;
+void zo_set_scmos(const double* varg) { scmos_map = varg; }
+
 /* Localization.computeLikelihood (1125-1143) */
 static double L(loc_t* s, double x, double y, double z, double a, double b) {
     int R = s->m->R;
     zo_model_psf(s->m, x, y, s->zfocus, z, s->psf);
     double lik = 0;
     for (int k = 0; k < R * R; k++) {
-        s->model[k] = s->psf[k] * a + b;
+        s->model[k] = s->psf[k] * a + b + (scmos_map ? scmos_map[k] : 0.0);
         lik += s->model[k] - s->roi[k] * log(s->model[k]);
     }
     return lik;
@@ -860,7 +865,7 @@ static double lm_eval(zo_model* m, const double* roi, const double* th, double z
     if (g) for (int i = 0; i < 5; i++) g[i] = 0;
     if (H) for (int i = 0; i < 25; i++) H[i] = 0;
     for (int k = 0; k < n; k++) {
-        double mu = th[0] * psf[k] + th[1];
+        double mu = th[0] * psf[k] + th[1] + (scmos_map ? scmos_map[k] : 0.0);
         nll += mu - roi[k] * log(mu);
         double J[5] = {psf[k], 1.0, th[0] * dpsf[k], th[0] * dpsf[n + k], th[0] * dpsf[2 * n + k]};
         double wg = 1.0 - roi[k] / mu, wh = 1.0 / mu;
@@ -875,7 +880,12 @@ static double lm_eval(zo_model* m, const double* roi, const double* th, double z
 static double lm_nll(zo_model* m, const double* roi, const double* th, double zfocus, double* psf) {
     int n = m->R * m->R;
     zo_model_psf(m, th[2], th[3], zfocus, th[4], psf);
-    return zo_nll(n, psf, roi, th[0], th[1]);
+    double lik = 0;
+    for (int k = 0; k < n; k++) {
+        double mu = psf[k] * th[0] + th[1] + (scmos_map ? scmos_map[k] : 0.0);
+        lik += mu - roi[k] * log(mu);
+    }
+    return lik;
 }
 
 void zo_fisher_analytic(zo_model* m, const double* theta, double zfocus, double* fisher) {

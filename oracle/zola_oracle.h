@@ -113,6 +113,9 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
               double* out_crlb);
 
 void zo_set_lm_noise_floor(double v);
+/* sCMOS map (R x R, var/gain^2 per pixel) added to mu by zo_localize and
+ * zo_lm_fit; NULL to clear.  The pointer is kept, not copied. */
+void zo_set_scmos(const double* varg);
 
 /* Fisher matrix sum(J J^T / mu) by analytic tangents, 5x5 in [A,B,x,y,z] order. */
 void zo_fisher_analytic(zo_model* m, const double* theta, double zfocus, double* fisher);

@@ -389,3 +389,57 @@ def test_psf_center_follows_defocus():
     assert cs[0] < -0.5 * par and cs[1] > 0.5 * par
     assert abs(cs[0] + cs[1]) <= 0.11 + 1e-9
     assert abs(cs[1] - par) < 0.35 * par
+
+
+def test_scmos_constant_map_is_a_background_shift():
+    """A constant sCMOS map c adds c to every mu: the fit moves B by -c and
+    leaves x, y, z and A where they are (both fit forms)."""
+    a = Z.astigmatic_coeffs(28)
+    m = O.Model(O.optics(), 21, a)
+    rng = np.random.default_rng(41)
+    roi = rng.poisson(1500 * m.psf(0.03, -0.02, 0.0, 0.2) + 30.0).astype(np.float64)
+    start = np.array([1400.0, 28.0, 0.0, 0.0, 0.25])
+    c = 7.0
+    base = m.lm_fit(roi, start, maxit=200, tol=1e-10)
+    O.set_scmos(np.full((21, 21), c))
+    try:
+        shifted = m.lm_fit(roi, start - np.array([0, c, 0, 0, 0]), maxit=200, tol=1e-10)
+        ref_s = m.localize(roi, start - np.array([0, c, 0, 0, 0]), iterMax=20)
+    finally:
+        O.set_scmos(None)
+    ref_0 = m.localize(roi, start, iterMax=20)
+    np.testing.assert_allclose(shifted["theta"][2:], base["theta"][2:], atol=1e-7)
+    assert abs(shifted["theta"][1] - (base["theta"][1] - c)) < 1e-6
+    assert abs(shifted["theta"][0] - base["theta"][0]) < 1e-4
+    assert abs(shifted["nll"] - base["nll"]) < 1e-6 * abs(base["nll"])
+    np.testing.assert_allclose(ref_s["theta"][2:], ref_0["theta"][2:], atol=1e-6)
+    assert abs(ref_s["theta"][1] - (ref_0["theta"][1] - c)) < 1e-5
+
+
+def test_scmos_map_enters_the_likelihood_per_pixel():
+    """lm_fit with a map v optimises sum(mu + v - n ln(mu + v)): its gradient
+    vanishes at the returned optimum (checked by central differences on the
+    likelihood written out here)."""
+    a = Z.astigmatic_coeffs(28)
+    m = O.Model(O.optics(), 21, a)
+    rng = np.random.default_rng(42)
+    v = rng.uniform(0.5, 6.0, (21, 21))
+    roi = rng.poisson(2000 * m.psf(-0.02, 0.04, 0.0, -0.3) + 20.0 + v).astype(np.float64)
+    O.set_scmos(v)
+    try:
+        r = m.lm_fit(roi, np.array([1800.0, 22.0, 0.0, 0.0, -0.25]), maxit=300, tol=1e-11)
+    finally:
+        O.set_scmos(None)
+    th = r["theta"]
+
+    def nll(t):
+        mu = t[0] * m.psf(t[2], t[3], 0.0, t[4]) + t[1] + v
+        return float(np.sum(mu - roi * np.log(mu)))
+
+    for p, h in ((0, 0.5), (1, 1e-3), (2, 1e-5), (3, 1e-5), (4, 1e-5)):
+        e = np.zeros(5)
+        e[p] = h
+        g = (nll(th + e) - nll(th - e)) / (2 * h)
+        curv = (nll(th + e) - 2 * nll(th) + nll(th - e)) / h**2
+        assert abs(g) < 1e-3 * max(1.0, curv * h), (p, g, curv)
+    assert abs(r["nll"] - nll(th)) < 1e-9 * abs(r["nll"])

@@ -34,6 +34,7 @@ struct FitArgs {
     const float* pixels;      // B*R*R
     const double* init;       // B*5 [A, B, x, y, z]
     const float* zoil;        // B per-ROI focus term z_oil (um), or null: zfocus for all
+    const float* varg;        // B*R*R sCMOS term var/gain^2 added to mu per pixel, or null
     int* queue;               // persistent form: ROI counter (zeroed before launch), or null
     double* theta;            // B*5
     double* nll;              // B
@@ -87,17 +88,26 @@ __device__ __forceinline__ float normalise(int lane, int wr, typename SY::Red& r
     return s;
 }
 
-// Poisson NLL without ln n! (Localization.computeLikelihood, 1125-1143), FP64.
+// sCMOS term of a pixel: var/gain^2 of the ROI's map (vroi), 0 without one.
+template <class SY, int R>
+__device__ __forceinline__ double scmos_term(const float* vroi, int lane, int wr, int t) {
+    return vroi ? (double)vroi[SY::template pix<R>(lane, wr, t)] : 0.0;
+}
+
+// Poisson NLL without ln n! (Localization.computeLikelihood, 1125-1143), FP64;
+// mu = A psf + B, plus the sCMOS term var/gain^2 of the pixel when vroi is
+// given (1131-1132).
 template <class SY, int R>
 __device__ __forceinline__ double nll64(int lane, int wr, typename SY::Red& red,
                                         const float (&psf)[SY::template RPL<R>],
-                                        const float (&n)[SY::template RPL<R>], double A, double Bg) {
+                                        const float (&n)[SY::template RPL<R>], double A, double Bg,
+                                        const float* vroi) {
     constexpr int RPL = SY::template RPL<R>;
     double acc = 0.0;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
         if (SY::template valid<R>(lane, wr, t)) {
-            const double mu = (double)psf[t] * A + Bg;
+            const double mu = (double)psf[t] * A + Bg + scmos_term<SY, R>(vroi, lane, wr, t);
             acc += mu - (double)n[t] * log(mu);
         }
     }
@@ -446,6 +456,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         auto& red = lds.red[slot];
         LmState& L = lds.L[w];
         const float* pix = A.pixels + (size_t)b * R * R;
+        const float* vroi = A.varg ? A.varg + (size_t)b * R * R : nullptr;
         const float zoil = A.zoil ? A.zoil[b] : (float)A.zfocus;
         // Wave-uniform FP64 state lives in LDS between uses; registers are kept
         // for the transforms.  The step solves (F + S + lambda diag F) d = -g by
@@ -534,7 +545,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     n[t] = SY::template valid<R>(lane, wr, t) ? pix[SY::template pix<R>(lane, wr, t)] : 0.f;
                 const float Sn = normalise<SY, R>(lane, wr, red, vB, psf);
                 const double tA = L.tn[0], tB = L.tn[1];
-                const double nn = nll64<SY, R>(lane, wr, red, psf, n, tA, tB);
+                const double nn = nll64<SY, R>(lane, wr, red, psf, n, tA, tB, vroi);
                 if (nn != nn) nan = 1;
                 accepted = first || nn <= L.nll;
                 if (!accepted) {
@@ -576,7 +587,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
     #pragma unroll
                     for (int t = 0; t < RPL; t++) {
                         if (!SY::template valid<R>(lane, wr, t)) continue;
-                        const double mu = (double)psf[t] * tA + tB;
+                        const double mu = (double)psf[t] * tA + tB + scmos_term<SY, R>(vroi, lane, wr, t);
                         const double J[5] = {(double)psf[t], 1.0,
                                              tA * (double)((vX[t] - psf[t] * sx) * inv),
                                              tA * (double)((vY[t] - psf[t] * sy) * inv),
@@ -757,7 +768,10 @@ __device__ __forceinline__ void fit_ref_body(const typename SY::Model& M, const 
         double likelihood = 0, save = 0, l1 = 0, l3 = 0, grad = 0, prevLik = 0;
         double oldX = 1.7976931348623157e308, oldY = oldX, oldZ = oldX;
         auto at_theta = [&]() { return hc && pc[0] == th[0] && pc[1] == th[1] && pc[2] == th[2]; };
-        auto L = [&](const float (&ps)[RPL], double a, double bb) { return nll64<SY, R>(lane, wr, red, ps, n, a, bb); };
+        const float* vroi = A.varg ? A.varg + (size_t)b * R * R : nullptr;
+        auto L = [&](const float (&ps)[RPL], double a, double bb) {
+            return nll64<SY, R>(lane, wr, red, ps, n, a, bb, vroi);
+        };
         // min_amp (929-977 for A with h*1000, 997-1058 for B with h*10), on the cached psf
         auto min_amp = [&](bool isB) {
             const double hh = isB ? h * 10 : h * 1000;
@@ -974,6 +988,7 @@ __device__ __forceinline__ void fit_ref_body(const typename SY::Model& M, const 
     #pragma unroll
                 for (int t = 0; t < RPL; t++) {
                     if (!SY::template valid<R>(lane, wr, t)) continue;
+                    // the reference's weights 1/f1 carry no sCMOS term (computeFisher, 317-403)
                     const double f1 = (double)cur[t] * tA + tB;
                     // x, y, z, then A (the reference differentiates the model f1) and B
                     const double J[5] = {(double)dd[0][t], (double)dd[1][t], (double)dd[2][t], f1, 1.0};

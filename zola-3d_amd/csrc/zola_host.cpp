@@ -471,6 +471,19 @@ int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const flo
 int zola_fit_batch_zoil_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                             const double* init, const float* zoil, double* theta, double* nll, double* crlb,
                             int* iters, int* status) {
+    return zola_fit_batch_ext_dev(c, cfg, B, pixels, init, zoil, nullptr, theta, nll, crlb, iters, status);
+}
+
+int zola_fit_batch_scmos_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
+                             const float* varg, const double* init, double* theta, double* nll, double* crlb,
+                             int* iters, int* status) {
+    if (c && !varg) return set_err(c, ZOLA_EINVAL, "null sCMOS map");
+    return zola_fit_batch_ext_dev(c, cfg, B, pixels, init, nullptr, varg, theta, nll, crlb, iters, status);
+}
+
+int zola_fit_batch_ext_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels, const double* init,
+                           const float* zoil, const float* varg, double* theta, double* nll, double* crlb,
+                           int* iters, int* status) {
     if (!c) return ZOLA_EINVAL;
     if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
     if (B == 0) return ZOLA_OK;
@@ -493,6 +506,7 @@ int zola_fit_batch_zoil_dev(zola_ctx* c, const zola_fit_config* cfg, int B, cons
     A.pixels = pixels;
     A.init = init;
     A.zoil = zoil;
+    A.varg = varg;
     A.queue = c->d_queue;
     A.theta = theta;
     A.nll = nll;

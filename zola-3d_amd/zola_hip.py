@@ -29,7 +29,8 @@ SYMBOLS = [
     "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
     "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
     "zola_detector_correlation", "zola_localize_frames", "zola_fit_mixed_dev",
-    "zola_fit_batch_zoil_dev", "zola_search_psf_center", "zola_set_fit_layout", "zola_autotune_fit",
+    "zola_fit_batch_zoil_dev", "zola_fit_batch_scmos_dev", "zola_fit_batch_ext_dev",
+    "zola_search_psf_center", "zola_set_fit_layout", "zola_autotune_fit",
     "zola_java_double_string",
     "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
     "zola_formats_last_error",
@@ -104,6 +105,8 @@ def lib():
             L.zola_search_psf_center.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double)]
         if hasattr(L, "zola_fit_batch_zoil_dev"):
             L.zola_fit_batch_zoil_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 8
+            L.zola_fit_batch_scmos_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 8
+            L.zola_fit_batch_ext_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 9
         if hasattr(L, "zola_fit_mixed_dev"):
             L.zola_fit_mixed_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 9
         if hasattr(L, "zola_detector_create"):
@@ -321,6 +324,15 @@ class Context:
         self._check(lib().zola_fit_batch_zoil_dev(self._h, C.byref(cfg), pixels.shape[0], _ptr(pixels),
                                                   _ptr(init), _ptr(zoil), _ptr(theta), _ptr(nll), _ptr(crlb),
                                                   _ptr(iters), _ptr(status)), "fit_batch_zoil_dev")
+
+    def fit_ext_dev(self, cfg, pixels, init, theta, nll, crlb, iters, status, zoil=None, varg=None):
+        """Device form with an optional per-ROI focus term zoil (B floats) and an
+        optional sCMOS map varg (B x R x R floats, var/gain^2 added to mu)."""
+        self._check(lib().zola_fit_batch_ext_dev(self._h, C.byref(cfg), pixels.shape[0], _ptr(pixels), _ptr(init),
+                                                 _ptr(zoil) if zoil is not None else None,
+                                                 _ptr(varg) if varg is not None else None, _ptr(theta),
+                                                 _ptr(nll), _ptr(crlb), _ptr(iters), _ptr(status)),
+                    "fit_batch_ext_dev")
 
     def fit_mixed_dev(self, cfg, groups):
         """groups: list of dicts with keys R, pixels, init, theta, nll, crlb, iters, status
