@@ -63,8 +63,13 @@ def test_lm_fit_matches_fp64_optimum(setup):
     assert np.abs(d[:, :2]).max() <= 0.5
     assert np.abs(d[:, 2]).max() <= 1.0
     assert np.all(np.abs(res["nll"] - nll_ref) <= 1e-5 * np.abs(nll_ref))
-    crlb_ref = np.array([r["crlb"] for r in ref])
+    # CRLB: the oracle's analytic Fisher matrix at the GPU's own optimum (the
+    # two optima differ by up to the step tolerance, and the CRLB of the flat
+    # defocused ROIs moves by ~1e-3 over that distance)
+    crlb_ref = np.array([np.sqrt(np.diag(np.linalg.inv(m.fisher(res["theta"][b])))[2:]) for b in range(B)])
     np.testing.assert_allclose(res["crlb"], crlb_ref, rtol=1e-3)
+    crlb_opt = np.array([r["crlb"] for r in ref])
+    np.testing.assert_allclose(res["crlb"], crlb_opt, rtol=5e-3)
 
 
 def test_lm_fit_follows_oracle_iterations(setup):

@@ -52,7 +52,10 @@ def test_lm_layout_matches_fp64_optimum(R, layout):
     assert np.abs(d[:, :2]).max() <= 0.5
     assert np.abs(d[:, 2]).max() <= 1.0
     assert np.all(np.abs(res["nll"] - nll) <= 1e-5 * np.abs(nll))
-    np.testing.assert_allclose(res["crlb"], np.array([r["crlb"] for r in opt]), rtol=1e-3)
+    # CRLB: the oracle's analytic Fisher matrix at the GPU's own optimum
+    crlb_ref = np.array([np.sqrt(np.diag(np.linalg.inv(m.fisher(res["theta"][b])))[2:]) for b in range(B)])
+    np.testing.assert_allclose(res["crlb"], crlb_ref, rtol=1e-3)
+    np.testing.assert_allclose(res["crlb"], np.array([r["crlb"] for r in opt]), rtol=5e-3)
 
 
 @pytest.mark.parametrize("R", [31, 41])

@@ -485,7 +485,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         int it = 0, tries = 0, conv = 0, nan = 0, nfield = 0;
         bool first = true;
         while (true) {
-            bool eval_now = true, stop_noise = false;
+            bool eval_now = true, stop_noise = false, pre_conv = false;
             if (!first) {
                 double Am[25], Lc[25], d[5];
     #pragma unroll
@@ -501,6 +501,14 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     eval_now = false;
                 } else {
                     chol5_solve(Lc, d);
+                    if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < A.tol) {
+                        // the step from the point just accepted is below tol:
+                        // converged there, without evaluating the step
+                        pre_conv = true;
+                        eval_now = false;
+                    }
+                }
+                if (eval_now) {
                     double pred = 0.0;
     #pragma unroll
                     for (int p = 0; p < 5; p++) {
@@ -618,7 +626,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                 continue;
             }
             tries++;
-            if (accepted || tries >= 12 || stop_noise) {
+            if (accepted || pre_conv || tries >= 12 || stop_noise) {
                 if (!accepted) conv = 1;
                 it++;
                 tries = 0;
