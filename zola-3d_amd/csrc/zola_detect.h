@@ -35,9 +35,10 @@ hipError_t zola_launch_frame_spectrum(const zola::DetGeom& g, int F, const float
 hipError_t zola_launch_template_spectrum(const zola::DetGeom& g, const double* tmpl, double2* tspec,
                                          const double2* tw, hipStream_t st);
 
-// Normalised correlation maps of templates [r0, r0 + nr) for F frames:
-// corr[f][r][ii][iii] (F x nZ x W x H).  work: F x nr x W x FY double2.
-hipError_t zola_launch_correlation(const zola::DetGeom& g, int F, int r0, int nr, const double2* spec,
+// Normalised correlation maps of templates [r0, min(r0 + 2 np, nZ)) for F
+// frames, two templates per complex inverse transform:
+// corr[f][r][ii][iii] (F x nZ x W x H).  work: F x np x W x FY double2.
+hipError_t zola_launch_correlation(const zola::DetGeom& g, int F, int r0, int np, const double2* spec,
                                    const double2* tspec, const double* var, const double* var_psf,
                                    double2* work, double* corr, const double2* tw, hipStream_t st);
 

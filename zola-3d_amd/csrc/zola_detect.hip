@@ -141,55 +141,65 @@ struct TemplateRowIO {
 // Inverse column pass for templates r0..r0+nr-1: line = (f * nr + rr) * FY + y.
 // Input F_I conj(F_T) written as in Predetection.java:449-450; only the W rows
 // that the crop keeps are stored: x -> ii = (x + W/2) % FX < W.
+// The frame and the templates are real, so each product F_I conj(F_T) is
+// Hermitian and its inverse transform real: two templates ra, rb = ra + 1 share
+// one complex inverse transform of F_I conj(F_Ta) + i F_I conj(F_Tb), whose real
+// and imaginary parts are the two correlations.  Line = (f * np + pp) * FY + y
+// for template pair pp (templates r0 + 2 pp and r0 + 2 pp + 1 < nZ).
 struct CorrColIO {
     DetGeom g;
-    int r0, nr;
+    int r0, np;
     const double2* spec;
     const double2* tspec;
     double2* work;
     __device__ double2 load(int line, int k) const {
-        const int y = line % g.FY, fr = line / g.FY;
-        const int f = fr / nr, r = r0 + fr % nr;
+        const int y = line % g.FY, fp = line / g.FY;
+        const int f = fp / np, ra = r0 + 2 * (fp % np), rb = ra + 1;
         const double2 a = spec[((size_t)f * g.FX + k) * g.FY + y];
-        const double2 d = tspec[((size_t)r * g.FX + k) * g.FY + y];
-        return make_double2(a.y * d.y + a.x * d.x, a.y * d.x - a.x * d.y);
+        const double2 da = tspec[((size_t)ra * g.FX + k) * g.FY + y];
+        const double2 xa = make_double2(a.y * da.y + a.x * da.x, a.y * da.x - a.x * da.y);
+        if (rb >= g.nZ) return xa;
+        const double2 db = tspec[((size_t)rb * g.FX + k) * g.FY + y];
+        const double2 xb = make_double2(a.y * db.y + a.x * db.x, a.y * db.x - a.x * db.y);
+        return make_double2(xa.x - xb.y, xa.y + xb.x);
     }
     __device__ void store(int line, int k, double2 v) const {
         const int ii = (k + g.W / 2) % g.FX;
         if (ii >= g.W) return;
-        const int y = line % g.FY, fr = line / g.FY;
-        work[((size_t)fr * g.W + ii) * g.FY + y] = v;
+        const int y = line % g.FY, fp = line / g.FY;
+        work[((size_t)fp * g.W + ii) * g.FY + y] = v;
     }
 };
 
-// Inverse row pass: line = (f * nr + rr) * W + ii; keeps iii = (y + H/2) % FY < H
-// and normalises as Predetection.java:463-476.
+// Inverse row pass: line = (f * np + pp) * W + ii; keeps iii = (y + H/2) % FY < H
+// and normalises as Predetection.java:463-476, the real part for template
+// r0 + 2 pp and the imaginary part for r0 + 2 pp + 1.
 struct CorrRowIO {
     DetGeom g;
-    int r0, nr;
+    int r0, np;
     const double2* work;
     const double* var;
     const double* var_psf;
     double* corr;
     __device__ double2 load(int line, int k) const { return work[(size_t)line * g.FY + k]; }
-    __device__ void store(int line, int k, double2 v) const {
-        const int iii = (k + g.H / 2) % g.FY;
-        if (iii >= g.H) return;
-        const int ii = line % g.W, fr = line / g.W;
-        const int f = fr / nr, r = r0 + fr % nr;
-        const double vp = var_psf[r];
-        const double vi = var[((size_t)f * g.W + ii) * g.H + iii];
-        const double prod = vp * vi;
-        const double sq = sqrt(prod);
-        double out;
-        if (sq > 0) out = v.x / sq;
-        else if (vp <= 0) out = -1.0;
-        else if (vi <= 0) out = -2.0;
+    __device__ static double normalised(double c, double vp, double vi) {
+        const double sq = sqrt(vp * vi);
+        if (sq > 0) return c / sq;
+        if (vp <= 0) return -1.0;
+        if (vi <= 0) return -2.0;
         // var is NaN where the reference's norm is 0 (its size mix-up zeroes it on
         // the same border pixels of every frame): there it keeps the array's
         // previous value, which stays the initial 0.0
-        else out = 0.0;
-        corr[(((size_t)f * g.nZ + r) * g.W + ii) * g.H + iii] = out;
+        return 0.0;
+    }
+    __device__ void store(int line, int k, double2 v) const {
+        const int iii = (k + g.H / 2) % g.FY;
+        if (iii >= g.H) return;
+        const int ii = line % g.W, fp = line / g.W;
+        const int f = fp / np, ra = r0 + 2 * (fp % np), rb = ra + 1;
+        const double vi = var[((size_t)f * g.W + ii) * g.H + iii];
+        corr[(((size_t)f * g.nZ + ra) * g.W + ii) * g.H + iii] = normalised(v.x, var_psf[ra], vi);
+        if (rb < g.nZ) corr[(((size_t)f * g.nZ + rb) * g.W + ii) * g.H + iii] = normalised(v.y, var_psf[rb], vi);
     }
 };
 
@@ -409,14 +419,14 @@ hipError_t zola_launch_template_spectrum(const DetGeom& g, const double* tmpl, d
     return launch_lines<ColInPlaceIO, true>(g.FX, cio, g.nZ * g.FY, -1.0, tw, st);
 }
 
-hipError_t zola_launch_correlation(const DetGeom& g, int F, int r0, int nr, const double2* spec,
+hipError_t zola_launch_correlation(const DetGeom& g, int F, int r0, int np, const double2* spec,
                                    const double2* tspec, const double* var, const double* var_psf,
                                    double2* work, double* corr, const double2* tw, hipStream_t st) {
-    CorrColIO cio{g, r0, nr, spec, tspec, work};
-    hipError_t e = launch_lines<CorrColIO, true>(g.FX, cio, F * nr * g.FY, +1.0, tw, st);
+    CorrColIO cio{g, r0, np, spec, tspec, work};
+    hipError_t e = launch_lines<CorrColIO, true>(g.FX, cio, F * np * g.FY, +1.0, tw, st);
     if (e != hipSuccess) return e;
-    CorrRowIO rio{g, r0, nr, work, var, var_psf, corr};
-    return launch_lines<CorrRowIO, false>(g.FY, rio, F * nr * g.W, +1.0, tw, st);
+    CorrRowIO rio{g, r0, np, work, var, var_psf, corr};
+    return launch_lines<CorrRowIO, false>(g.FY, rio, F * np * g.W, +1.0, tw, st);
 }
 
 hipError_t zola_launch_local_max(const DetGeom& g, int F, const double* corr, int cap, DetCand* cand,

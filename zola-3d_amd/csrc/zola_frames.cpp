@@ -126,7 +126,7 @@ struct zola_detector {
     double* d_varpsf = nullptr;
     double* d_norm = nullptr;
     // per-batch buffers (grow-only)
-    int fb = 0, nr = 0;
+    int fb = 0, nr = 0;  // nr: template pairs per correlation pass
     float* d_frames = nullptr;
     double *d_sum1 = nullptr, *d_sq1 = nullptr, *d_var = nullptr, *d_corr = nullptr;
     double2 *d_spec = nullptr, *d_work = nullptr;
@@ -185,7 +185,7 @@ int ensure_batch(zola_detector* d, int fb) {
     const zola::DetGeom& g = d->g;
     const size_t WH = (size_t)g.W * g.H;
     int nr = (int)std::max<size_t>(1, (size_t)(512u << 20) / ((size_t)fb * g.W * g.FY * 16));
-    d->nr = std::min(nr, g.nZ);
+    d->nr = std::min(nr, (g.nZ + 1) / 2);  // template pairs per correlation pass
     HIP_TRY(c, hipMalloc((void**)&d->d_frames, fb * WH * sizeof(float)));
     HIP_TRY(c, hipMalloc((void**)&d->d_sum1, fb * WH * sizeof(double)));
     HIP_TRY(c, hipMalloc((void**)&d->d_sq1, fb * WH * sizeof(double)));
@@ -218,9 +218,9 @@ int run_correlation(zola_detector* d, int fb) {
     hipStream_t st = c->stream;
     HIP_TRY(c, zola_launch_window_stats(g, fb, d->d_frames, d->d_sum1, d->d_sq1, d->d_norm, d->d_var, st));
     HIP_TRY(c, zola_launch_frame_spectrum(g, fb, d->d_frames, d->d_spec, d->d_tw, st));
-    for (int r0 = 0; r0 < g.nZ; r0 += d->nr) {
-        const int nr = std::min(d->nr, g.nZ - r0);
-        HIP_TRY(c, zola_launch_correlation(g, fb, r0, nr, d->d_spec, d->d_tspec, d->d_var, d->d_varpsf,
+    for (int r0 = 0; r0 < g.nZ; r0 += 2 * d->nr) {
+        const int np = std::min(d->nr, (g.nZ - r0 + 1) / 2);
+        HIP_TRY(c, zola_launch_correlation(g, fb, r0, np, d->d_spec, d->d_tspec, d->d_var, d->d_varpsf,
                                            d->d_work, d->d_corr, d->d_tw, st));
     }
     return ZOLA_OK;
