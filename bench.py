@@ -73,7 +73,7 @@ def synth(ctx, torch, B, seed):
     return px, truth
 
 
-def calibration_bench(torch, Z, stream):
+def calibration_bench(torch, Z, stream, cpu=True):
     """PSF calibration from 5 bead z-stacks of 41 slices (50 nm, -1..+1 um), 31x31 px,
     2e5 photons/slice, background 10, 45 Zernike modes ~ N(0, 0.15 rad) with
     Noll 5 = 0.8 rad; all 42 non-registration coefficients plus per-bead
@@ -126,7 +126,12 @@ def calibration_bench(torch, Z, stream):
     t0 = time.perf_counter()
     res = ctx.calib_fit(host, dz, start, np.zeros(J), fit_mask=mask, max_iter=500, gtol=1e-3)
     dt = time.perf_counter() - t0
-    return {
+    # algorithmic work of one evaluation: a forward and an adjoint pupil
+    # transform per PSF plus the Zernike projection, 2 D J FLOP per PSF
+    D = ctx.D
+    fft = 5 * 64 * 64 * np.log2(64 * 64)
+    W = nb * S * (2 * fft + 2 * D * J)
+    out = {
         "workload": "5 beads x 41 slices x 31x31 px, 45 Zernike modes, L-BFGS on adjoint "
                     "gradients (forward+adjoint per evaluation), zero start",
         "iterations": res["iters"], "evaluations": res["evals"],
@@ -134,7 +139,30 @@ def calibration_bench(torch, Z, stream):
         "evals_per_s": round(res["evals"] / dt, 2),
         "eval_kernel_ms": round(eval_ms, 4),
         "max_coeff_err_rad": round(float(np.abs(res["coeffs"] - truth)[3:].max()), 5),
+        "roofline": {"bound": "valu", "achieved": round(W / (eval_ms * 1e-3) / 1e12, 4), "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": round(W / (eval_ms * 1e-3) / 1e12 / 157.3, 5),
+                     "algorithmic_flops_per_eval": float(W)},
     }
+    if cpu:
+        # the reference CPU calibration re-evaluates the whole stack about 6 times
+        # per fitted coefficient per iteration (finite differences and the gamma
+        # line search, ZernikePhaseRetrieval.java:1091-1187): 6 x 42 evaluations
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        m = O.Model(O.optics(), Rc, np.zeros(J))
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for b in range(nb):
+                O.calib_nll(m, host[b].astype(np.float64), dz, np.full(S, beads[b, 0]), np.full(S, beads[b, 1]),
+                            np.full(S, beads[b, 2]), np.full(S, 2e5), np.full(Rc * Rc, 10.0))
+        t_eval = (time.perf_counter() - t0) / reps
+        per_iter = 6 * (J - 3)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / (per_iter * t_eval), 5), "unit": "iterations/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} full-stack NLL evaluations (FP64 C restatement, 5 beads x 41 slices) on one "
+                      f"core, {t_eval * 1e3:.1f} ms each, x {per_iter} evaluations per reference iteration"}
+    return out
 
 
 def _events(torch, n=2):
@@ -503,7 +531,7 @@ def main():
         out["batched_vs_plain"] = round(lm_rate_kernel / (Bp / ((pms + np.mean(init_ms) * Bp / B) / 1e3)), 3)
 
     if not args.no_calib:
-        out["calibration"] = calibration_bench(torch, Z, stream)
+        out["calibration"] = calibration_bench(torch, Z, stream, cpu=not args.no_cpu)
     if not args.no_n128:
         out["depth_n128"] = depth128_bench(torch, Z, stream, args.n128_batch)
     if not args.no_mixed:
