@@ -1,5 +1,6 @@
 // zola_detect.h -- launchers of the frame detector and frame-pipeline kernels.
 #pragma once
+#include <cstdint>
 #include <hip/hip_runtime.h>
 
 namespace zola {
@@ -56,6 +57,11 @@ hipError_t zola_launch_rank_sort(int F, int cap, int max_count, const zola::DetC
 // frames[frame][ci + a - R/2][cj + bb - R/2].
 hipError_t zola_launch_gather_rois(int W, int H, int R, int B, const int* roi, const float* frames,
                                    float* pixels, hipStream_t st);
+
+// Model statistics of B fitted ROIs for the acceptance walk (see zola_detect.hip):
+// max(mu - B), the chi2 score and the bit mask of (mu - B) > frac * max.
+hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* px, const double* th, double frac,
+                                   double* mx, double* score, uint32_t* bits, hipStream_t st);
 
 // FP64 twiddle table W_4096^a = (cos, sin)(2 pi a / 4096), a = 0..4095.
 void zola_twiddle_table(double2* host4096);

@@ -360,10 +360,10 @@ __global__ __launch_bounds__(256) void k_rank_sort(int cap, const DetCand* __res
 // ---------------------------------------------------------------- ROI gather
 __global__ void k_gather_rois(int W, int H, int R, int B, const int* __restrict__ roi,
                               const float* __restrict__ frames, float* __restrict__ pixels) {
-    const int b = blockIdx.y;
+    const int b = blockIdx.x;
     if (b >= B) return;
     const int f = roi[b * 3 + 0], ci = roi[b * 3 + 1], cj = roi[b * 3 + 2];
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < R * R; e += gridDim.x * blockDim.x) {
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
         const int a = e / R, bb = e % R;
         const int i = ci + a - R / 2, jj = cj + bb - R / 2;
         float v = 0.f;
@@ -448,8 +448,56 @@ hipError_t zola_launch_rank_sort(int F, int cap, int max_count, const DetCand* c
 hipError_t zola_launch_gather_rois(int W, int H, int R, int B, const int* roi, const float* frames,
                                    float* pixels, hipStream_t st) {
     if (B <= 0) return hipSuccess;
-    dim3 grid((R * R + 255) / 256, B);
-    hipLaunchKernelGGL(k_gather_rois, grid, dim3(256), 0, st, W, H, R, B, roi, frames, pixels);
+    hipLaunchKernelGGL(k_gather_rois, dim3(B), dim3(256), 0, st, W, H, R, B, roi, frames, pixels);
+    return hipGetLastError();
+}
+
+// Statistics of the fitted model mu = A psf + B that the acceptance walk uses
+// (LocalizationPipeline.java:862-873, 924-938 and scoreCompute, 1099-1115), one
+// ROI per thread, in FP64 with the host's operation order and no contraction,
+// so that they equal the host's values bit for bit:
+//   mx[b]    = max(0, max_p (mu_p - B))
+//   score[b] = sum_p (n_p - mu_p)^2 / mu_p / R^2, in pixel order
+//   bits[b]  = bit p set where (mu_p - B) > frac * mx[b]   (ceil(R^2/32) words)
+__global__ void k_model_stats(int B, int R, const float* __restrict__ psf, const float* __restrict__ px,
+                              const double* __restrict__ th, double frac, double* __restrict__ mx,
+                              double* __restrict__ score, uint32_t* __restrict__ bits) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int n = R * R, nw = (n + 31) / 32;
+    const double A = th[(size_t)b * 5 + 0], Bg = th[(size_t)b * 5 + 1];
+    const float* ps = psf + (size_t)b * n;
+    const float* pp = px + (size_t)b * n;
+    double m = 0.0;
+    for (int p = 0; p < n; p++) {
+        const double mu = (double)ps[p] * A + Bg;
+        if (m < (mu - Bg)) m = mu - Bg;
+    }
+    const double thr = m * frac;
+    double sum = 0.0, cnt = 0.0;
+    for (int w = 0; w < nw; w++) {
+        uint32_t word = 0;
+        for (int t = 0; t < 32; t++) {
+            const int p = w * 32 + t;
+            if (p >= n) break;
+            const double mu = (double)ps[p] * A + Bg;
+            const double v = (double)pp[p];
+            sum += (v - mu) * (v - mu) / mu;
+            cnt++;
+            if ((mu - Bg) > thr) word |= 1u << t;
+        }
+        bits[(size_t)b * nw + w] = word;
+    }
+    mx[b] = m;
+    score[b] = sum / cnt;
+}
+
+hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* px, const double* th, double frac,
+                                   double* mx, double* score, uint32_t* bits, hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_model_stats, dim3((B + 63) / 64), dim3(64), 0, st, B, R, psf, px, th, frac, mx, score,
+                       bits);
     return hipGetLastError();
 }
 

@@ -140,6 +140,8 @@ struct zola_detector {
     double *d_init = nullptr, *d_th = nullptr, *d_nll = nullptr, *d_cr = nullptr;
     int *d_it = nullptr, *d_st = nullptr;
     float *d_px = nullptr, *d_par = nullptr, *d_psf = nullptr;
+    double* d_mstat = nullptr;    // per fit: max(mu - B), chi2 score
+    uint32_t* d_mbits = nullptr;  // per fit: model mask bits
 };
 
 namespace {
@@ -165,6 +167,7 @@ void free_cand(zola_detector* d) {
 void free_fit(zola_detector* d) {
     dfree(d->d_roi); dfree(d->d_init); dfree(d->d_th); dfree(d->d_nll); dfree(d->d_cr);
     dfree(d->d_it); dfree(d->d_st); dfree(d->d_px); dfree(d->d_par); dfree(d->d_psf);
+    dfree(d->d_mstat); dfree(d->d_mbits);
     d->fit_cap = 0;
 }
 
@@ -172,9 +175,10 @@ void free_fit(zola_detector* d) {
 int batch_frames(const zola_detector* d, int F) {
     const zola::DetGeom& g = d->g;
     const double per = (double)g.nZ * g.W * g.H * 8 + (double)g.W * g.H * (4 + 24) + (double)g.FX * g.FY * 16;
-    // up to 256 frames in flight (8 GiB of correlation maps and spectra): more
-    // frames per batch give the fit rounds more ROIs and the walk more threads
-    int fb = (int)std::min(256.0, std::max(1.0, 8.0 * (1u << 30) / per));
+    // up to 2048 frames in flight (at most 64 GiB of correlation maps and
+    // spectra, a fraction of the 288 GB of HBM): more frames per batch give each
+    // fit round more ROIs and the walk more threads
+    int fb = (int)std::min(2048.0, std::max(1.0, 64.0 * (1u << 30) / per));
     return std::max(1, std::min(fb, F));
 }
 
@@ -413,10 +417,12 @@ int zola_detect_frames(zola_detector* d, int F, const float* frames, int cap, do
 // ------------------------------------------------------------------ frame pipeline
 namespace {
 
+// A fit and the statistics of its model that the walk uses (k_model_stats):
+// max(mu - B), the chi2 score and the model mask bits (P.bits, nw words).
 struct FitRes {
     double th[5], crlb[3], nll;
     int status;
-    std::vector<float> psf;
+    double mmax, score;
 };
 
 struct FrameRun {
@@ -441,6 +447,8 @@ struct Pipe {
     const zola_accept_config* acc;
     int W, H, R, half, foot;
     std::vector<FitRes> results;
+    std::vector<uint32_t> bits;        // nw words per result
+    int nw = 0;
     std::vector<int> q_frame, q_k;     // fit queue (run index, candidate)
 };
 
@@ -514,13 +522,6 @@ bool apply_fit(const Pipe& P, FrameRun& fr, int k, const FitRes& res, std::vecto
     if ((res.status & ZOLA_ST_NAN) || lowph) return false;
     const int ci = cand_i(fr, k), cj = cand_j(fr, k);
     const double A = res.th[0], Bg = res.th[1], x = res.th[2], y = res.th[3], z = res.th[4];
-    const int n = P.R * P.R;
-    double thrm = 0;
-    for (int p = 0; p < n; p++) {
-        const double mu = (double)res.psf[p] * A + Bg;
-        if (thrm < (mu - Bg)) thrm = mu - Bg;
-    }
-    thrm *= P.acc->model_mask_frac;
     const double dx = P.d->g.xystep;
     if (!(A > P.acc->photon_threshold)) return false;
     if (!(std::fabs(x) < P.acc->max_shift_px * dx && std::fabs(y) < P.acc->max_shift_px * dx)) return false;
@@ -528,21 +529,13 @@ bool apply_fit(const Pipe& P, FrameRun& fr, int k, const FitRes& res, std::vecto
     if (!(px >= 0 && px < P.W && py >= 0 && py < P.H && !mask[(size_t)px * P.H + py])) return false;
     if (!(z > P.acc->min_z && z < P.acc->max_z)) return false;
     if (emit) {
-        double sum = 0, cnt = 0;
-        for (int a = 0; a < P.R; a++)
-            for (int b = 0; b < P.R; b++) {
-                const double v = fr.img[(size_t)(ci + a - P.half) * P.H + cj + b - P.half];
-                const double mu = (double)res.psf[a * P.R + b] * A + Bg;
-                sum += (v - mu) * (v - mu) / mu;
-                cnt++;
-            }
         const double rec[ZOLA_LOC_FIELDS] = {(double)fr.f,
                                              1000 * (ci * dx - x),
                                              1000 * (cj * dx - y),
                                              1000 * z,
                                              A,
                                              Bg,
-                                             sum / cnt,
+                                             res.score,
                                              1000 * res.crlb[0],
                                              1000 * res.crlb[1],
                                              1000 * res.crlb[2],
@@ -550,10 +543,12 @@ bool apply_fit(const Pipe& P, FrameRun& fr, int k, const FitRes& res, std::vecto
                                              (double)k};
         fr.locs.insert(fr.locs.end(), rec, rec + ZOLA_LOC_FIELDS);
     }
+    // pixels of the model above 0.2 of its peak (mu - B > 0.2 max(mu - B))
+    const uint32_t* bits = P.bits.data() + (size_t)fr.res_idx[k] * P.nw;
     for (int a = 0; a < P.R; a++)
         for (int b = 0; b < P.R; b++) {
-            const double mu = (double)res.psf[a * P.R + b] * A + Bg;
-            if ((mu - Bg) > thrm) mask[(size_t)(ci + a - P.half) * P.H + cj + b - P.half] = 1;
+            const int p = a * P.R + b;
+            if ((bits[p >> 5] >> (p & 31)) & 1u) mask[(size_t)(ci + a - P.half) * P.H + cj + b - P.half] = 1;
         }
     return true;
 }
@@ -689,6 +684,8 @@ int ensure_fit(zola_detector* d, int B) {
     HIP_TRY(c, hipMalloc((void**)&d->d_px, (size_t)B * R2 * sizeof(float)));
     HIP_TRY(c, hipMalloc((void**)&d->d_par, (size_t)B * 4 * sizeof(float)));
     HIP_TRY(c, hipMalloc((void**)&d->d_psf, (size_t)B * R2 * sizeof(float)));
+    HIP_TRY(c, hipMalloc((void**)&d->d_mstat, (size_t)B * 2 * sizeof(double)));
+    HIP_TRY(c, hipMalloc((void**)&d->d_mbits, (size_t)B * ((R2 + 31) / 32) * sizeof(uint32_t)));
     d->fit_cap = B;
     return ZOLA_OK;
 }
@@ -741,8 +738,17 @@ int fit_queue(zola_detector* d, Pipe& P, std::vector<FrameRun>& runs) {
     HIP_TRY(c, hipMemcpyAsync(d->d_par, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice, st));
     rc = zola_psf_forward_dev(c, B, d->d_par, d->d_psf);
     if (rc) return rc;
-    std::vector<float> psf((size_t)B * R2);
-    HIP_TRY(c, hipMemcpyAsync(psf.data(), d->d_psf, psf.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    // only the model statistics the walk uses come back, not the PSFs
+    const int nw = (int)((R2 + 31) / 32);
+    HIP_TRY(c, zola_launch_model_stats(B, P.R, d->d_psf, d->d_px, d->d_th, P.acc->model_mask_frac, d->d_mstat,
+                                       d->d_mstat + B, d->d_mbits, st));
+    std::vector<double> mstat((size_t)B * 2);
+    const size_t base = P.bits.size();
+    P.nw = nw;
+    P.bits.resize(base + (size_t)B * nw);
+    HIP_TRY(c, hipMemcpyAsync(mstat.data(), d->d_mstat, mstat.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(c, hipMemcpyAsync(P.bits.data() + base, d->d_mbits, (size_t)B * nw * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, st));
     HIP_TRY(c, hipStreamSynchronize(st));
     for (int b = 0; b < B; b++) {
         FitRes r;
@@ -750,11 +756,12 @@ int fit_queue(zola_detector* d, Pipe& P, std::vector<FrameRun>& runs) {
         std::memcpy(r.crlb, &cr[b * 3], 3 * sizeof(double));
         r.nll = nll[b];
         r.status = stt[b];
-        r.psf.assign(psf.begin() + (size_t)b * R2, psf.begin() + (size_t)(b + 1) * R2);
+        r.mmax = mstat[b];
+        r.score = mstat[B + b];
         FrameRun& fr = runs[P.q_frame[b]];
         fr.res_idx[P.q_k[b]] = (int)P.results.size();
         fr.queued[P.q_k[b]] = 0;
-        P.results.push_back(std::move(r));
+        P.results.push_back(r);
     }
     return ZOLA_OK;
 }
@@ -811,6 +818,7 @@ extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit
         }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         P.results.clear();
+        P.bits.clear();
         for (;;) {
             // Frames are independent: their walks replay on host threads; the
             // queued fits are then gathered in frame order.
