@@ -309,6 +309,19 @@ int  zola_localize_frames(zola_detector* det, const zola_fit_config* fit,
                           const zola_accept_config* acc, int F, const float* frames, int cap,
                           double* locs, int* n_locs, int* per_frame, int* rounds);
 
+/* As zola_localize_frames for an sCMOS camera (the isSCMOS branch of
+ * LocalizationThread.run, LocalizationPipeline.java:597-634, 797-850,
+ * 1168-1211): adu the raw frames (F x W x H), offset, gain and variance the
+ * camera maps (W x H, utils/SCMOScamera.java:78-96).  Detection runs on
+ * (adu - offset)/gain + variance[0]/gain[0]^2 clamped at 0; the walk on
+ * (adu - offset)/gain + variance/gain^2 per pixel, with the map patch in
+ * getAandB (patch - map), in the fit's model and in the accepted model's
+ * mask and score. */
+int  zola_localize_frames_scmos(zola_detector* det, const zola_fit_config* fit,
+                                const zola_accept_config* acc, int F, const float* adu,
+                                const float* offset, const float* gain, const float* variance, int cap,
+                                double* locs, int* n_locs, int* per_frame, int* rounds);
+
 #ifdef __cplusplus
 }
 #endif

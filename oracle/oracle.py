@@ -84,6 +84,8 @@ def lib():
         L.zo_calib_nll.restype = C.c_double
         L.zo_frame_localize.argtypes = [vp, vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_double, _dp, C.c_int]
+        L.zo_frame_localize_scmos.argtypes = [vp, vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                              C.c_int, C.c_double, _dp, C.c_int]
         L.zo_search_psf_center.restype = C.c_double
         L.zo_search_psf_center.argtypes = [vp, C.c_double, C.c_double]
         L.zo_calib_nll.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]
@@ -288,6 +290,16 @@ def frame_localize(model, det, image, zfocus, minZ, maxZ, iter_max=20, photon_th
     out = np.zeros((cap, 12))
     n = lib().zo_frame_localize(model._h, det._h, _d(image), zfocus, minZ, maxZ, iter_max,
                                 photon_thr, out, cap)
+    return out[:min(n, cap)]
+
+
+def frame_localize_scmos(model, det, det_image, image, varg, zfocus, minZ, maxZ, iter_max=20,
+                         photon_thr=500.0, cap=10000):
+    """The walk of an sCMOS frame: candidates from det_image, patches from image,
+    var/gain^2 map varg (W x H)."""
+    out = np.zeros((cap, 12))
+    n = lib().zo_frame_localize_scmos(model._h, det._h, _d(det_image), _d(image), _d(varg), zfocus, minZ, maxZ,
+                                      iter_max, photon_thr, out, cap)
     return out[:min(n, cap)]
 
 

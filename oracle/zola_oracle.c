@@ -1426,22 +1426,62 @@ double zo_search_psf_center(zo_model* m, double zfocus, double axialRange) {
 /* Frame walk (cpu/LocalizationPipeline.java:721-1073, 1099-1250)        */
 /* ------------------------------------------------------------------ */
 
+/* getAandB of the sCMOS branch (1168-1211): the sums and the maximum are of
+ * patch - scmos. */
+static void get_a_and_b_scmos(int npx, const double* patch, const double* psf, const double* sc, double thr,
+                              double* res) {
+    double maxPatch = -INFINITY, maxi = -INFINITY;
+    for (int k = 0; k < npx; k++) {
+        if (psf[k] > maxi) maxi = psf[k];
+        if (patch[k] - sc[k] > maxPatch) maxPatch = patch[k] - sc[k];
+    }
+    double threshold = maxi * thr;
+    double countB = 0, B = 0, A = 0, countA = 0;
+    for (int k = 0; k < npx; k++) {
+        if (psf[k] < threshold) { B += patch[k] - sc[k]; countB++; }
+        else { A += patch[k] - sc[k]; countA++; }
+    }
+    if (countB != 0) B /= countB; else B = 0;
+    A -= countA * B;
+    res[0] = A; res[1] = B; res[2] = maxPatch;
+}
+
+static int frame_walk(zo_model* m, zo_detector* d, const double* det_image, const double* image,
+                      const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
+                      double photonThr, double* out, int cap);
+
 int zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double zfocus,
                       double minZ, double maxZ, int iterMax, double photonThr,
                       double* out, int cap) {
+    return frame_walk(m, d, image, image, NULL, zfocus, minZ, maxZ, iterMax, photonThr, out, cap);
+}
+
+int zo_frame_localize_scmos(zo_model* m, zo_detector* d, const double* det_image, const double* image,
+                            const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
+                            double photonThr, double* out, int cap) {
+    return frame_walk(m, d, det_image, image, varg, zfocus, minZ, maxZ, iterMax, photonThr, out, cap);
+}
+
+/* The walk on `image`, candidates from `det_image`; with an sCMOS map varg
+ * (W x H, var/gain^2), the patch's map enters getAandB and the fit's model
+ * (LocalizationPipeline.java:797-850). */
+static int frame_walk(zo_model* m, zo_detector* d, const double* det_image, const double* image,
+                      const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
+                      double photonThr, double* out, int cap) {
     int W = d->W, H = d->H, R = m->R, n = R * R, h = R / 2;
     double dx = m->o.xystep;
     int ccap = 1 << 16;
     double* cand = (double*)xcalloc((size_t)ccap * 7, sizeof(double));
-    int nc = zo_detector_run(d, image, cand, ccap, NULL);
+    int nc = zo_detector_run(d, det_image, cand, ccap, NULL);
     if (nc > ccap) {
         free(cand);
         ccap = nc;
         cand = (double*)xcalloc((size_t)ccap * 7, sizeof(double));
-        nc = zo_detector_run(d, image, cand, ccap, NULL);
+        nc = zo_detector_run(d, det_image, cand, ccap, NULL);
     }
     unsigned char* mask = (unsigned char*)xcalloc((size_t)W * H, 1);
     double* patch = (double*)xcalloc(n, sizeof(double));
+    double* pvar = (double*)xcalloc(n, sizeof(double));
     double* model = (double*)xcalloc(n, sizeof(double));
     int ptr = 0, nloc = 0;
     for (;;) {
@@ -1464,18 +1504,22 @@ int zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double z
             for (int b = 0; b < R; b++) {
                 if (mask[(ci + a - h) * H + cj + b - h]) { already = 1; break; }
                 patch[a * R + b] = image[(ci + a - h) * H + cj + b - h];
+                if (varg) pvar[a * R + b] = varg[(ci + a - h) * H + cj + b - h];
             }
         if (already) continue;
         double ab[3];
-        zo_get_a_and_b(n, patch, d->psf + (long)r * n, .1, ab);
+        if (varg) get_a_and_b_scmos(n, patch, d->psf + (long)r * n, pvar, .1, ab);
+        else zo_get_a_and_b(n, patch, d->psf + (long)r * n, .1, ab);
         int treated = 0;
         if ((ab[2] / ab[1] > 1) && (ab[1] > 0)) {
             double theta[5] = {ab[0], ab[1], -cand[k * 7 + 5], -cand[k * 7 + 6],
                                d->range[r] + cand[k * 7 + 4]};
             double crlb[3], lik;
             int it, nanf;
+            if (varg) zo_set_scmos(pvar);
             int ok = zo_localize(m, patch, theta, zfocus, minZ - .3, maxZ + .3, iterMax, 0.0, crlb,
                                  &lik, &it, model, &nanf);
+            if (varg) zo_set_scmos(NULL);
             if (ok) {
                 double B = theta[1], thr = 0;
                 for (int p = 0; p < n; p++)
@@ -1514,7 +1558,7 @@ int zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double z
                 for (int b = 0; b < R; b++)
                     if (patch[a * R + b] > ab[2] * .5) mask[(ci + a - h) * H + cj + b - h] = 1;
     }
-    free(cand); free(mask); free(patch); free(model);
+    free(cand); free(mask); free(patch); free(pvar); free(model);
     return nloc;
 }
 

@@ -150,6 +150,15 @@ int  zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double 
                        double minZ, double maxZ, int iterMax, double photonThr,
                        double* out, int cap);
 
+/* The walk with an sCMOS camera: candidates from det_image (converted with the
+ * constant var/gain^2 of pixel 0 and clamped at 0), patches from image
+ * ((adu - offset)/gain + var/gain^2 per pixel), varg the W x H map var/gain^2
+ * used by getAandB and the fit (LocalizationPipeline.java:597-633, 797-850,
+ * 1168-1211). */
+int  zo_frame_localize_scmos(zo_model* m, zo_detector* d, const double* det_image, const double* image,
+                             const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
+                             double photonThr, double* out, int cap);
+
 /* Depth of smallest lateral CRLB, refined, then the best window of width
  * axialRange/2 (SearchPSFcenter.getPosition).  m must have R = N (the
  * reference crops the PSF to the FFT size here). */

@@ -152,3 +152,51 @@ def test_search_psf_center_matches_oracle(torch_cuda):
         print("centre", zf, ax, c, ref)
         assert abs(c - ref) <= 1e-9
     ctx.close()
+
+
+def _scmos_camera(frames, seed):
+    """Raw frames of an sCMOS camera and its maps (offset, gain ADU/e-, read
+    variance ADU^2), and the frames the pipeline derives from them (the
+    library's conversion: FP64, then float32)."""
+    rng = np.random.default_rng(seed)
+    F, W, H = frames.shape
+    off = rng.uniform(90, 110, (W, H)).astype(np.float32)
+    gain = rng.uniform(1.8, 2.4, (W, H)).astype(np.float32)
+    var = rng.uniform(4.0, 60.0, (W, H)).astype(np.float32)
+    read = rng.normal(0.0, 1.0, frames.shape) * np.sqrt(var.astype(np.float64))
+    adu = np.round(off + gain * frames.astype(np.float64) + read).astype(np.float32)
+    e = (adu.astype(np.float64) - off.astype(np.float64)) / gain.astype(np.float64)
+    vmap = (var.astype(np.float64) / (gain.astype(np.float64) ** 2)).astype(np.float32)
+    v0 = float(var[0, 0]) / (float(gain[0, 0]) ** 2)
+    det = np.maximum(e + v0, 0.0).astype(np.float32)
+    img = (e + vmap.astype(np.float64)).astype(np.float32)
+    return adu, off, gain, var, det, img, vmap
+
+
+def test_frame_walk_scmos_matches_oracle(models):
+    ctx, om = models
+    print("case: sCMOS frame walk, reference fit form, 3 frames 96x96", flush=True)
+    frames = _frames(96, 96, 3, seed=41, n=6)
+    adu, off, gain, var, det_f, img_f, vmap = _scmos_camera(frames, 42)
+    zlim = (-0.7, 0.7)
+    det = Z.Detector(ctx, 96, 96, zlim[0], zlim[1], 0.1)
+    fit = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=zlim[0] - 0.3, zmax=zlim[1] + 0.3,
+                         photon_threshold=0.0)
+    acc = det.accept_config(photon_threshold=500.0, min_z=zlim[0], max_z=zlim[1])
+    locs, per, rounds = det.localize_scmos(adu, off, gain, var, fit, acc)
+    recs = []
+    for f in range(3):
+        od = O.Detector(om, 96, 96, zlim[0], zlim[1], 0.1)
+        r = O.frame_localize_scmos(om, od, det_f[f].astype(np.float64), img_f[f].astype(np.float64),
+                                   vmap.astype(np.float64), 0.0, zlim[0], zlim[1])
+        r[:, 0] = f
+        recs.append(r)
+    ref = np.concatenate(recs)
+    print("records gpu", len(locs), "oracle", len(ref), "fit rounds", rounds)
+    assert len(locs) == len(ref) and len(ref) >= 12
+    np.testing.assert_array_equal(locs[:, [0, 11]], ref[:, [0, 11]])
+    d = np.abs(locs[:, 1:4] - ref[:, 1:4])
+    print("max |d| nm x, y, z:", d.max(axis=0))
+    assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
+    np.testing.assert_allclose(locs[:, 4:6], ref[:, 4:6], rtol=1e-3)
+    det.close()

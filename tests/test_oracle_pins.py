@@ -443,3 +443,32 @@ def test_scmos_map_enters_the_likelihood_per_pixel():
         curv = (nll(th + e) - 2 * nll(th) + nll(th - e)) / h**2
         assert abs(g) < 1e-3 * max(1.0, curv * h), (p, g, curv)
     assert abs(r["nll"] - nll(th)) < 1e-9 * abs(r["nll"])
+
+
+def test_scmos_frame_walk_with_a_zero_map_is_the_plain_walk():
+    # var/gain^2 = 0, offset 0, gain 1: the sCMOS walk is the plain walk exactly
+    m = _frame_model()
+    em = [(30.3, 25.6, 0.2, 3000), (60.8, 70.1, -0.3, 2500), (25.2, 70.4, 0.05, 4000)]
+    img = simdata.render_frame(m, 96, 96, em, 20.0, seed=9).astype(np.float64)
+    det = O.Detector(m, 96, 96, -0.7, 0.7, 0.1, thr=0.1)
+    a = O.frame_localize(m, det, img, 0.0, -0.7, 0.7)
+    b = O.frame_localize_scmos(m, det, img, img, np.zeros_like(img), 0.0, -0.7, 0.7)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_scmos_frame_walk_subtracts_the_map_in_the_fit():
+    # emitters on a frame with a strong per-pixel variance term: the sCMOS walk
+    # (map given) recovers photons and background of the photo-electrons; the
+    # plain walk on the same frame absorbs the variance term into B
+    m = _frame_model()
+    em = [(30.3, 25.6, 0.2, 3000), (60.8, 70.1, -0.3, 2500)]
+    rng = np.random.default_rng(11)
+    v = rng.uniform(2.0, 12.0, (96, 96))
+    img = simdata.render_frame(m, 96, 96, em, 20.0, seed=10).astype(np.float64) + rng.poisson(v)
+    det = O.Detector(m, 96, 96, -0.7, 0.7, 0.1, thr=0.1)
+    det_img = np.maximum(img - v + v[0, 0], 0.0)
+    locs = O.frame_localize_scmos(m, det, det_img, img, v, 0.0, -0.7, 0.7)
+    assert len(locs) == 2
+    for k, dxy, dz, ra in _match(locs, em):
+        assert dxy < 40 and dz < 80 and 0.85 < ra < 1.15
+    assert np.all(np.abs(locs[:, 5] - 20.0) < 2.0)   # background without the variance term

@@ -60,8 +60,13 @@ hipError_t zola_launch_gather_rois(int W, int H, int R, int B, const int* roi, c
 
 // Model statistics of B fitted ROIs for the acceptance walk (see zola_detect.hip):
 // max(mu - B), the chi2 score and the bit mask of (mu - B) > frac * max.
-hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* px, const double* th, double frac,
-                                   double* mx, double* score, uint32_t* bits, hipStream_t st);
+hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* px, const float* vpx,
+                                   const double* th, double frac, double* mx, double* score, uint32_t* bits,
+                                   hipStream_t st);
+
+// R x R patches of one W x H map at the ROI centres (roi as for gather_rois).
+hipError_t zola_launch_gather_map(int W, int H, int R, int B, const int* roi, const float* map, float* out,
+                                  hipStream_t st);
 
 // FP64 twiddle table W_4096^a = (cos, sin)(2 pi a / 4096), a = 0..4095.
 void zola_twiddle_table(double2* host4096);

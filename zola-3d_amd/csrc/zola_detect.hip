@@ -452,7 +452,8 @@ hipError_t zola_launch_gather_rois(int W, int H, int R, int B, const int* roi, c
     return hipGetLastError();
 }
 
-// Statistics of the fitted model mu = A psf + B that the acceptance walk uses
+// Statistics of the fitted model mu = A psf + B (+ the sCMOS term of the pixel
+// when vpx is given, as in Localization.computeLikelihood) that the acceptance walk uses
 // (LocalizationPipeline.java:862-873, 924-938 and scoreCompute, 1099-1115), one
 // ROI per thread, in FP64 with the host's operation order and no contraction,
 // so that they equal the host's values bit for bit:
@@ -460,8 +461,8 @@ hipError_t zola_launch_gather_rois(int W, int H, int R, int B, const int* roi, c
 //   score[b] = sum_p (n_p - mu_p)^2 / mu_p / R^2, in pixel order
 //   bits[b]  = bit p set where (mu_p - B) > frac * mx[b]   (ceil(R^2/32) words)
 __global__ void k_model_stats(int B, int R, const float* __restrict__ psf, const float* __restrict__ px,
-                              const double* __restrict__ th, double frac, double* __restrict__ mx,
-                              double* __restrict__ score, uint32_t* __restrict__ bits) {
+                              const float* __restrict__ vpx, const double* __restrict__ th, double frac,
+                              double* __restrict__ mx, double* __restrict__ score, uint32_t* __restrict__ bits) {
 #pragma clang fp contract(off)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
@@ -469,9 +470,10 @@ __global__ void k_model_stats(int B, int R, const float* __restrict__ psf, const
     const double A = th[(size_t)b * 5 + 0], Bg = th[(size_t)b * 5 + 1];
     const float* ps = psf + (size_t)b * n;
     const float* pp = px + (size_t)b * n;
+    const float* pv = vpx ? vpx + (size_t)b * n : nullptr;
     double m = 0.0;
     for (int p = 0; p < n; p++) {
-        const double mu = (double)ps[p] * A + Bg;
+        const double mu = pv ? (double)ps[p] * A + Bg + (double)pv[p] : (double)ps[p] * A + Bg;
         if (m < (mu - Bg)) m = mu - Bg;
     }
     const double thr = m * frac;
@@ -481,7 +483,7 @@ __global__ void k_model_stats(int B, int R, const float* __restrict__ psf, const
         for (int t = 0; t < 32; t++) {
             const int p = w * 32 + t;
             if (p >= n) break;
-            const double mu = (double)ps[p] * A + Bg;
+            const double mu = pv ? (double)ps[p] * A + Bg + (double)pv[p] : (double)ps[p] * A + Bg;
             const double v = (double)pp[p];
             sum += (v - mu) * (v - mu) / mu;
             cnt++;
@@ -493,11 +495,31 @@ __global__ void k_model_stats(int B, int R, const float* __restrict__ psf, const
     score[b] = sum / cnt;
 }
 
-hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* px, const double* th, double frac,
-                                   double* mx, double* score, uint32_t* bits, hipStream_t st) {
+hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* px, const float* vpx,
+                                   const double* th, double frac, double* mx, double* score, uint32_t* bits,
+                                   hipStream_t st) {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_model_stats, dim3((B + 63) / 64), dim3(64), 0, st, B, R, psf, px, th, frac, mx, score,
-                       bits);
+    hipLaunchKernelGGL(k_model_stats, dim3((B + 63) / 64), dim3(64), 0, st, B, R, psf, px, vpx, th, frac, mx,
+                       score, bits);
+    return hipGetLastError();
+}
+
+// Patches of one W x H map (the sCMOS var/gain^2 map) at the ROI centres.
+__global__ void k_gather_map(int W, int H, int R, int B, const int* __restrict__ roi, const float* __restrict__ map,
+                             float* __restrict__ out) {
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const int ci = roi[b * 3 + 1], cj = roi[b * 3 + 2];
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const int i = ci + e / R - R / 2, jj = cj + e % R - R / 2;
+        out[(size_t)b * R * R + e] = (i >= 0 && i < W && jj >= 0 && jj < H) ? map[(size_t)i * H + jj] : 0.f;
+    }
+}
+
+hipError_t zola_launch_gather_map(int W, int H, int R, int B, const int* roi, const float* map, float* out,
+                                  hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_map, dim3(B), dim3(256), 0, st, W, H, R, B, roi, map, out);
     return hipGetLastError();
 }
 

@@ -142,6 +142,11 @@ struct zola_detector {
     float *d_px = nullptr, *d_par = nullptr, *d_psf = nullptr;
     double* d_mstat = nullptr;    // per fit: max(mu - B), chi2 score
     uint32_t* d_mbits = nullptr;  // per fit: model mask bits
+    // sCMOS walk: frames of the walk (the detection frames differ), map, map patches
+    float* d_locf = nullptr;
+    int locf_fb = 0;
+    float* d_vmap = nullptr;
+    float* d_vpx = nullptr;
 };
 
 namespace {
@@ -167,7 +172,7 @@ void free_cand(zola_detector* d) {
 void free_fit(zola_detector* d) {
     dfree(d->d_roi); dfree(d->d_init); dfree(d->d_th); dfree(d->d_nll); dfree(d->d_cr);
     dfree(d->d_it); dfree(d->d_st); dfree(d->d_px); dfree(d->d_par); dfree(d->d_psf);
-    dfree(d->d_mstat); dfree(d->d_mbits);
+    dfree(d->d_mstat); dfree(d->d_mbits); dfree(d->d_vpx);
     d->fit_cap = 0;
 }
 
@@ -343,6 +348,8 @@ int zola_detector_destroy(zola_detector* d) {
     dfree(d->d_tspec);
     dfree(d->d_varpsf);
     dfree(d->d_norm);
+    dfree(d->d_locf);
+    dfree(d->d_vmap);
     delete d;
     return ZOLA_OK;
 }
@@ -449,6 +456,7 @@ struct Pipe {
     std::vector<FitRes> results;
     std::vector<uint32_t> bits;        // nw words per result
     int nw = 0;
+    const float* vmap = nullptr;       // sCMOS: W x H var/gain^2 (host), or null
     std::vector<int> q_frame, q_k;     // fit queue (run index, candidate)
 };
 
@@ -477,9 +485,14 @@ const double* get_ab(const Pipe& P, FrameRun& fr, int k) {
         const int n = P.R * P.R;
         const double* psf = P.d->psf.data() + (size_t)r * n;
         double maxPatch = -INFINITY, maxi = -INFINITY;
+        // with an sCMOS map the sums and the maximum are of patch - map (1168-1211)
+        auto val = [&](int a, int b) {
+            const size_t q = (size_t)(ci + a - P.half) * P.H + cj + b - P.half;
+            return P.vmap ? (double)fr.img[q] - (double)P.vmap[q] : (double)fr.img[q];
+        };
         for (int a = 0; a < P.R; a++)
             for (int b = 0; b < P.R; b++) {
-                const double v = fr.img[(size_t)(ci + a - P.half) * P.H + cj + b - P.half];
+                const double v = val(a, b);
                 if (psf[a * P.R + b] > maxi) maxi = psf[a * P.R + b];
                 if (v > maxPatch) maxPatch = v;
             }
@@ -487,7 +500,7 @@ const double* get_ab(const Pipe& P, FrameRun& fr, int k) {
         double cB = 0, Bs = 0, As = 0, cA = 0;
         for (int a = 0; a < P.R; a++)
             for (int b = 0; b < P.R; b++) {
-                const double v = fr.img[(size_t)(ci + a - P.half) * P.H + cj + b - P.half];
+                const double v = val(a, b);
                 if (psf[a * P.R + b] < thr) {
                     Bs += v;
                     cB++;
@@ -686,6 +699,7 @@ int ensure_fit(zola_detector* d, int B) {
     HIP_TRY(c, hipMalloc((void**)&d->d_psf, (size_t)B * R2 * sizeof(float)));
     HIP_TRY(c, hipMalloc((void**)&d->d_mstat, (size_t)B * 2 * sizeof(double)));
     HIP_TRY(c, hipMalloc((void**)&d->d_mbits, (size_t)B * ((R2 + 31) / 32) * sizeof(uint32_t)));
+    HIP_TRY(c, hipMalloc((void**)&d->d_vpx, (size_t)B * R2 * sizeof(float)));
     d->fit_cap = B;
     return ZOLA_OK;
 }
@@ -717,8 +731,10 @@ int fit_queue(zola_detector* d, Pipe& P, std::vector<FrameRun>& runs) {
     }
     HIP_TRY(c, hipMemcpyAsync(d->d_roi, roi.data(), roi.size() * sizeof(int), hipMemcpyHostToDevice, st));
     HIP_TRY(c, hipMemcpyAsync(d->d_init, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_TRY(c, zola_launch_gather_rois(P.W, P.H, P.R, B, d->d_roi, d->d_frames, d->d_px, st));
-    rc = zola_fit_batch_dev(c, P.fit, B, d->d_px, d->d_init, d->d_th, d->d_nll, d->d_cr, d->d_it, d->d_st);
+    HIP_TRY(c, zola_launch_gather_rois(P.W, P.H, P.R, B, d->d_roi, P.vmap ? d->d_locf : d->d_frames, d->d_px, st));
+    if (P.vmap) HIP_TRY(c, zola_launch_gather_map(P.W, P.H, P.R, B, d->d_roi, d->d_vmap, d->d_vpx, st));
+    rc = zola_fit_batch_ext_dev(c, P.fit, B, d->d_px, d->d_init, nullptr, P.vmap ? d->d_vpx : nullptr, d->d_th,
+                                d->d_nll, d->d_cr, d->d_it, d->d_st);
     if (rc) return rc;
     std::vector<double> th((size_t)B * 5), nll(B), cr((size_t)B * 3);
     std::vector<int> stt(B);
@@ -740,7 +756,8 @@ int fit_queue(zola_detector* d, Pipe& P, std::vector<FrameRun>& runs) {
     if (rc) return rc;
     // only the model statistics the walk uses come back, not the PSFs
     const int nw = (int)((R2 + 31) / 32);
-    HIP_TRY(c, zola_launch_model_stats(B, P.R, d->d_psf, d->d_px, d->d_th, P.acc->model_mask_frac, d->d_mstat,
+    HIP_TRY(c, zola_launch_model_stats(B, P.R, d->d_psf, d->d_px, P.vmap ? d->d_vpx : nullptr, d->d_th,
+                                       P.acc->model_mask_frac, d->d_mstat,
                                        d->d_mstat + B, d->d_mbits, st));
     std::vector<double> mstat((size_t)B * 2);
     const size_t base = P.bits.size();
@@ -768,13 +785,15 @@ int fit_queue(zola_detector* d, Pipe& P, std::vector<FrameRun>& runs) {
 
 }  // namespace
 
-extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit, const zola_accept_config* acc,
-                                    int F, const float* frames, int cap, double* locs, int* n_locs,
-                                    int* per_frame, int* rounds) {
-    if (!d) return ZOLA_EINVAL;
+namespace {
+
+// The pipeline on detection frames det (uploaded for the detector) and walk
+// frames img (patches, fits; the same array without an sCMOS map), vmap the
+// sCMOS var/gain^2 map or null.
+int localize_impl(zola_detector* d, const zola_fit_config* fit, const zola_accept_config* acc, int F,
+                  const float* det, const float* img, const float* vmap, int cap, double* locs, int* n_locs,
+                  int* per_frame, int* rounds) {
     zola_ctx* c = d->ctx;
-    if (!fit || !acc || F < 0 || cap < 0 || !n_locs || (F > 0 && !frames) || (cap > 0 && !locs))
-        return set_err(c, ZOLA_EINVAL, "bad frame-pipeline arguments");
     *n_locs = 0;
     if (rounds) *rounds = 0;
     if (F == 0) return ZOLA_OK;
@@ -783,7 +802,17 @@ extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit
     const int fbmax = batch_frames(d, F);
     int rc = ensure_batch(d, fbmax);
     if (rc) return rc;
+    if (vmap) {
+        if (d->locf_fb < fbmax) {
+            dfree(d->d_locf);
+            HIP_TRY(c, hipMalloc((void**)&d->d_locf, (size_t)fbmax * WH * sizeof(float)));
+            d->locf_fb = fbmax;
+        }
+        if (!d->d_vmap) HIP_TRY(c, hipMalloc((void**)&d->d_vmap, WH * sizeof(float)));
+        HIP_TRY(c, hipMemcpyAsync(d->d_vmap, vmap, WH * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
     Pipe P;
+    P.vmap = vmap;
     P.d = d;
     P.fit = fit;
     P.acc = acc;
@@ -796,15 +825,18 @@ extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit
     std::vector<int> cnt;
     for (int f0 = 0; f0 < F; f0 += fbmax) {
         const int fb = std::min(fbmax, F - f0);
-        HIP_TRY(c, hipMemcpyAsync(d->d_frames, frames + f0 * WH, fb * WH * sizeof(float), hipMemcpyHostToDevice,
+        HIP_TRY(c, hipMemcpyAsync(d->d_frames, det + f0 * WH, fb * WH * sizeof(float), hipMemcpyHostToDevice,
                                   c->stream));
+        if (vmap)
+            HIP_TRY(c, hipMemcpyAsync(d->d_locf, img + f0 * WH, fb * WH * sizeof(float), hipMemcpyHostToDevice,
+                                      c->stream));
         rc = run_detect(d, fb, cnt);
         if (rc) return rc;
         std::vector<FrameRun> runs(fb);
         for (int f = 0; f < fb; f++) {
             FrameRun& fr = runs[f];
             fr.f = f0 + f;
-            fr.img = frames + (size_t)(f0 + f) * WH;
+            fr.img = img + (size_t)(f0 + f) * WH;
             fr.n = cnt[f];
             fr.cand.resize((size_t)fr.n * 7);
             if (fr.n > 0)
@@ -857,4 +889,51 @@ extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit
     }
     *n_locs = (int)total;
     return ZOLA_OK;
+}
+
+}  // namespace
+
+extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit, const zola_accept_config* acc,
+                                    int F, const float* frames, int cap, double* locs, int* n_locs,
+                                    int* per_frame, int* rounds) {
+    if (!d) return ZOLA_EINVAL;
+    if (!fit || !acc || F < 0 || cap < 0 || !n_locs || (F > 0 && !frames) || (cap > 0 && !locs))
+        return set_err(d->ctx, ZOLA_EINVAL, "bad frame-pipeline arguments");
+    return localize_impl(d, fit, acc, F, frames, frames, nullptr, cap, locs, n_locs, per_frame, rounds);
+}
+
+extern "C" int zola_localize_frames_scmos(zola_detector* d, const zola_fit_config* fit,
+                                          const zola_accept_config* acc, int F, const float* adu,
+                                          const float* offset, const float* gain, const float* variance, int cap,
+                                          double* locs, int* n_locs, int* per_frame, int* rounds) {
+    if (!d) return ZOLA_EINVAL;
+    zola_ctx* c = d->ctx;
+    if (!fit || !acc || F < 0 || cap < 0 || !n_locs || (F > 0 && (!adu || !offset || !gain || !variance)) ||
+        (cap > 0 && !locs))
+        return set_err(c, ZOLA_EINVAL, "bad sCMOS frame-pipeline arguments");
+    const size_t WH = (size_t)d->g.W * d->g.H;
+    // vargain = variance / gain^2 (SCMOScamera.java:95); detection frames use the
+    // constant vargain of pixel 0 and are clamped at 0, the walk's frames use the
+    // pixel's own and are not (LocalizationPipeline.java:597-603, 628-634)
+    std::vector<float> vmap(WH);
+    for (size_t q = 0; q < WH; q++) vmap[q] = (float)((double)variance[q] / ((double)gain[q] * (double)gain[q]));
+    const double v0 = (double)variance[0] / ((double)gain[0] * (double)gain[0]);
+    std::vector<float> det((size_t)F * WH), img((size_t)F * WH);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int f; (f = next.fetch_add(1)) < F;)
+            for (size_t q = 0; q < WH; q++) {
+                const size_t k = (size_t)f * WH + q;
+                const double e = ((double)adu[k] - (double)offset[q]) / (double)gain[q];
+                const double dv = e + v0;
+                det[k] = (float)(dv < 0 ? 0.0 : dv);
+                img[k] = (float)(e + (double)vmap[q]);
+            }
+    };
+    const int nt = std::max(1, std::min(F, host_threads()));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return localize_impl(d, fit, acc, F, det.data(), img.data(), vmap.data(), cap, locs, n_locs, per_frame, rounds);
 }

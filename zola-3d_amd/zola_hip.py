@@ -28,7 +28,7 @@ SYMBOLS = [
     "zola_fit_batch_dev", "zola_fit_init_dev", "zola_get_stream", "zola_calib_eval_dev",
     "zola_calib_fit", "zola_detector_create", "zola_detector_destroy", "zola_detector_info",
     "zola_detector_range", "zola_detector_templates", "zola_detect_frames",
-    "zola_detector_correlation", "zola_localize_frames", "zola_fit_mixed_dev",
+    "zola_detector_correlation", "zola_localize_frames", "zola_localize_frames_scmos", "zola_fit_mixed_dev",
     "zola_fit_batch_zoil_dev", "zola_fit_batch_scmos_dev", "zola_fit_batch_ext_dev",
     "zola_search_psf_center", "zola_set_fit_layout", "zola_autotune_fit",
     "zola_java_double_string",
@@ -120,6 +120,8 @@ def lib():
             L.zola_detector_correlation.argtypes = [vp, vp, vp]
             L.zola_localize_frames.argtypes = [vp, C.POINTER(FitConfig), C.POINTER(AcceptConfig),
                                                C.c_int, vp, C.c_int, vp, vp, vp, vp]
+            L.zola_localize_frames_scmos.argtypes = [vp, C.POINTER(FitConfig), C.POINTER(AcceptConfig),
+                                                     C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
         L.zola_set_fit_layout.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.zola_autotune_fit.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), vp]
         L.zola_java_double_string.argtypes = [C.c_double, C.c_char_p, C.c_int]
@@ -456,6 +458,22 @@ class Detector:
         self._check(lib().zola_localize_frames(self._h, C.byref(fit_cfg), C.byref(acc_cfg), F, _ptr(fr),
                                                cap, _ptr(locs), C.byref(n), _ptr(per), C.byref(rounds)),
                     "zola_localize_frames")
+        return locs[:min(n.value, cap)].copy(), per, rounds.value
+
+    def localize_scmos(self, adu, offset, gain, variance, fit_cfg, acc_cfg, cap=1000000):
+        """Localizations of raw sCMOS frames adu (F, W, H) with the camera maps
+        offset, gain, variance (W, H)."""
+        fr = np.ascontiguousarray(adu, dtype=np.float32).reshape(-1, self.W, self.H)
+        maps = [np.ascontiguousarray(x, dtype=np.float32).reshape(self.W, self.H) for x in (offset, gain, variance)]
+        F = fr.shape[0]
+        locs = np.zeros((cap, LOC_FIELDS))
+        n = C.c_int()
+        per = np.zeros(F, dtype=np.int32)
+        rounds = C.c_int()
+        self._check(lib().zola_localize_frames_scmos(self._h, C.byref(fit_cfg), C.byref(acc_cfg), F, _ptr(fr),
+                                                     _ptr(maps[0]), _ptr(maps[1]), _ptr(maps[2]), cap, _ptr(locs),
+                                                     C.byref(n), _ptr(per), C.byref(rounds)),
+                    "zola_localize_frames_scmos")
         return locs[:min(n.value, cap)].copy(), per, rounds.value
 
 
