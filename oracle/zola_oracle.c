@@ -1000,8 +1000,10 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             for (int p = 0; p < 25; p++) A[p] = H[p] + S[p];
             for (int p = 0; p < 5; p++) { A[p * 5 + p] += lambda * H[p * 5 + p]; d[p] = -g[p]; }
             if (chol_solve(5, A, d) != 0) { lambda *= nu; nu *= 2; continue; }
-            if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < tol) {
-                /* the step from the point just accepted is below tol: converged
+            if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < tol &&
+                fabs(d[0]) * sqrt(H[0]) < 0.01 && fabs(d[1]) * sqrt(H[6]) < 0.01) {
+                /* the step from the point just accepted is below tol in x, y, z
+                 * and below 1% of the statistical precision in A and B: converged
                  * there, without evaluating the step */
                 conv = 1;
                 accepted = 1;

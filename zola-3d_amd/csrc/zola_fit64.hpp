@@ -501,9 +501,11 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     eval_now = false;
                 } else {
                     chol5_solve(Lc, d);
-                    if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < A.tol) {
-                        // the step from the point just accepted is below tol:
-                        // converged there, without evaluating the step
+                    if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < A.tol &&
+                        fabs(d[0]) * sqrt(hget(L.H, 0, 0)) < 0.01 && fabs(d[1]) * sqrt(hget(L.H, 1, 1)) < 0.01) {
+                        // the step from the point just accepted is below tol in x,
+                        // y, z and below 1% of the statistical precision in A and
+                        // B: converged there, without evaluating the step
                         pre_conv = true;
                         eval_now = false;
                     }
