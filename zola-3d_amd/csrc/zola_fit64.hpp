@@ -334,6 +334,7 @@ struct LmState {
     double H[15];   // Fisher matrix, packed upper triangle
     double S[15];   // secant estimate of the rest of the Hessian, packed upper
     double gr[5];   // gradient
+    double gp[5];   // gradient at the previous point (secant update)
     double th[5];   // current point
     double tn[5];   // trial point
     double nll, lambda, nu, pred;
@@ -341,7 +342,8 @@ struct LmState {
 
 __device__ __forceinline__ constexpr int hidx(int p, int q) {
     // packed upper triangle, row-major
-    return p > q ? hidx(q, p) : p * 5 - (p * (p - 1)) / 2 + (q - p);
+    const int lo = p < q ? p : q, hi = p < q ? q : p;
+    return lo * 5 - (lo * (lo - 1)) / 2 + (hi - lo);
 }
 __device__ __forceinline__ double hget(const double (&H)[15], int p, int q) { return H[hidx(p, q)]; }
 
@@ -350,13 +352,15 @@ __device__ __forceinline__ double hget(const double (&H)[15], int p, int q) { re
 // S = sum (1 - n/mu) d2mu, which Fisher scoring drops.  After an accepted
 // step s = tn - th, S is sized by min(1, |s'y|/|s'Ss|) and updated with
 // Powell's symmetric Broyden formula so that S s = y = g+ - g - F+ s, in the
-// variables scaled by sqrt(diag F+).  acc = [g+ (5), F+ (packed 15)].
-__device__ __forceinline__ void lm_secant_update(LmState& L, const double (&acc)[20]) {
+// variables scaled by sqrt(diag F+).  Reads g+ and F+ from L.gr and L.H, g from
+// L.gp, the step from L.tn - L.th (all in LDS: no register array is indexed
+// at run time).
+__device__ __forceinline__ void lm_secant_update(LmState& L) {
     double D[5], iD[5], sh[5], yh[5];
     bool ok = true;
 #pragma unroll
     for (int p = 0; p < 5; p++) {
-        const double f = acc[5 + hidx(p, p)];
+        const double f = L.H[hidx(p, p)];
         ok = ok && (f > 0.0);
         D[p] = sqrt(fmax(f, 1e-300));
         iD[p] = 1.0 / D[p];
@@ -367,9 +371,9 @@ __device__ __forceinline__ void lm_secant_update(LmState& L, const double (&acc)
         const double sp = L.tn[p] - L.th[p];
         double fs = 0.0;
 #pragma unroll
-        for (int q = 0; q < 5; q++) fs += acc[5 + hidx(p, q)] * (L.tn[q] - L.th[q]);
+        for (int q = 0; q < 5; q++) fs += L.H[hidx(p, q)] * (L.tn[q] - L.th[q]);
         sh[p] = D[p] * sp;
-        yh[p] = (acc[p] - L.gr[p] - fs) * iD[p];
+        yh[p] = (L.gr[p] - L.gp[p] - fs) * iD[p];
     }
     double Ssh[5], sSs = 0.0, sy = 0.0, ss = 0.0;
 #pragma unroll
@@ -612,13 +616,17 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                         }
                     }
                     SY::template sumv<20>(acc, red, lane, wr);
-                    if (!first) lm_secant_update(L, acc);
     #pragma unroll
-                    for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
-    #pragma unroll
-                    for (int p = 0; p < 5; p++) L.gr[p] = acc[p];
+                    for (int p = 0; p < 5; p++) {
+                        L.gp[p] = L.gr[p];
+                        L.gr[p] = acc[p];
+                    }
     #pragma unroll
                     for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
+                    wave_lds_fence();
+                    if (!first) lm_secant_update(L);
+    #pragma unroll
+                    for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
                 }
                 wave_lds_fence();
             }
