@@ -258,8 +258,31 @@ __device__ __forceinline__ void column_blur64(const BlockTables64& T, WaveScratc
     wave_lds_fence();  // the scratch is reused by the next stage 1
 }
 
+// Row blur of one half of a grid row for the LM channels: acc[0] += h1 (x) I,
+// acc[1] += hd (x) I, acc[2] += h1 (x) Iz, with I and Iz at columns 2m + HP.
+template <int R, int HP>
+__device__ __forceinline__ void row_blur_half_lm(float h1reg, float hdreg, const float (&im)[32],
+                                                 const float (&iz)[32], float (&acc)[3][R]) {
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        const float s1 = bcast(h1reg, c), sd = bcast(hdreg, c);
+        static_for<0, 32>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            constexpr int jj = (c + R / 2 + 2 * m + HP) & 63;
+            if constexpr (jj < R) {
+                acc[0][jj] += s1 * im[m];
+                acc[1][jj] += sd * im[m];
+                acc[2][jj] += s1 * iz[m];
+            }
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
 // Fused stage 2 of E (from SE) and dE/dz (from SZ) for row a = lane:
 //   T0 = h1 (x) I,  T1 = hd (x) I  (row direction),  Tz = h1 (x) 2 Re(conj(E) dE/dz).
+// |E|^2 is formed before the dE/dz transform so that E's and dE/dz's spectra
+// and the three images are not all live at once.
 template <int R>
 __device__ __forceinline__ void stage2_EZ64(float h1reg, float hdreg, const WaveScratch64& SE,
                                             const WaveScratch64& SZ, int lane,
@@ -272,18 +295,20 @@ __device__ __forceinline__ void stage2_EZ64(float h1reg, float hdreg, const Wave
         constexpr int HP = decltype(hpc)::value;
         float er[32], ei[32];
         stage2_half64<HP>(SE, lane, er, ei);
-        float zr[32], zi[32];
-        stage2_half64<HP>(SZ, lane, zr, zi);
-        float img[3][32];
+        float im[32], iz[32];
         static_for<0, 32>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             constexpr int k = fidx<32>(m);
-            img[0][m] = er[k] * er[k] + ei[k] * ei[k];
-            img[1][m] = img[0][m];
-            img[2][m] = 2.f * (er[k] * zr[k] + ei[k] * zi[k]);
+            im[m] = er[k] * er[k] + ei[k] * ei[k];
         });
-        const float hr[3] = {h1reg, hdreg, h1reg};
-        row_blur_half<R, HP, 3>(hr, img, acc);
+        float zr[32], zi[32];
+        stage2_half64<HP>(SZ, lane, zr, zi);
+        static_for<0, 32>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            constexpr int k = fidx<32>(m);
+            iz[m] = 2.f * (er[k] * zr[k] + ei[k] * zi[k]);
+        });
+        row_blur_half_lm<R, HP>(h1reg, hdreg, im, iz, acc);
     });
 }
 
