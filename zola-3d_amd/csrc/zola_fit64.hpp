@@ -49,7 +49,7 @@ template <int R>
 struct Px {
     static constexpr int G = RoiLayout<R>::G;
     static constexpr int RPL = RoiLayout<R>::RPL;
-    __device__ static bool valid(int lane, int t) {
+    __device__ __forceinline__ static bool valid(int lane, int t) {
         const int g = lane / R;
         return g < G && g * RPL + t < R;
     }
@@ -200,32 +200,32 @@ struct Synth64 {
     static constexpr int RPL = RoiLayout<R>::RPL;
     // lane = (ROI column j, row group g), rows g*RPL + t
     template <int R>
-    __device__ static bool valid(int lane, int, int t) { return Px<R>::valid(lane, t); }
+    __device__ __forceinline__ static bool valid(int lane, int, int t) { return Px<R>::valid(lane, t); }
     template <int R>
-    __device__ static int pix(int lane, int, int t) { return ((lane / R) * RPL<R> + t) * R + lane % R; }
+    __device__ __forceinline__ static int pix(int lane, int, int t) { return ((lane / R) * RPL<R> + t) * R + lane % R; }
     struct Red {};
-    __device__ static float sum(float v, Red&, int, int) { return wave_sum(v); }
-    __device__ static double sum(double v, Red&, int, int) { return wave_sum(v); }
+    __device__ __forceinline__ static float sum(float v, Red&, int, int) { return wave_sum(v); }
+    __device__ __forceinline__ static double sum(double v, Red&, int, int) { return wave_sum(v); }
     template <int K>
-    __device__ static void sumv(double (&v)[K], Red&, int, int) {
+    __device__ __forceinline__ static void sumv(double (&v)[K], Red&, int, int) {
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
     }
     template <int K>
-    __device__ static void sumv(float (&v)[K], Red&, int, int) {
+    __device__ __forceinline__ static void sumv(float (&v)[K], Red&, int, int) {
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
     }
     static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
     static constexpr int kRefFloats = (int)(sizeof(WaveScratch64) / sizeof(float));
-    __device__ static void load(Shared& T, const Model& M) { load_tables64(T, M); }
+    __device__ __forceinline__ static void load(Shared& T, const Model& M) { load_tables64(T, M); }
     template <int R>
-    __device__ static void eval(const Model& M, const Shared& T, float* scr, int lane, int, float x, float y,
+    __device__ __forceinline__ static void eval(const Model& M, const Shared& T, float* scr, int lane, int, float x, float y,
                                 float z, float zoil, float (&v)[RoiLayout<R>::RPL]) {
         eval_E64<R>(M, T, *reinterpret_cast<WaveScratch64*>(scr), lane, T.h3[lane], x, y, z, zoil, v);
     }
     template <int R>
-    __device__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int, float x, float y,
+    __device__ __forceinline__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int, float x, float y,
                                    float z, float zoil, float (&vB)[RoiLayout<R>::RPL],
                                    float (&vX)[RoiLayout<R>::RPL], float (&vY)[RoiLayout<R>::RPL],
                                    float (&vZ)[RoiLayout<R>::RPL]) {
@@ -244,15 +244,15 @@ struct QuadWaves {
     template <int R>
     static constexpr int RPL = (R + 3) / 4;
     template <int R>
-    __device__ static bool valid(int lane, int wr, int t) { return lane < R && wr * RPL<R> + t < R; }
+    __device__ __forceinline__ static bool valid(int lane, int wr, int t) { return lane < R && wr * RPL<R> + t < R; }
     template <int R>
-    __device__ static int pix(int lane, int wr, int t) { return (wr * RPL<R> + t) * R + lane; }
+    __device__ __forceinline__ static int pix(int lane, int wr, int t) { return (wr * RPL<R> + t) * R + lane; }
     struct Red {
         double d[4][24];
         float f[4][4];
     };
     template <int K>
-    __device__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
+    __device__ __forceinline__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
         static_assert(K <= 24, "reduction width");
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -265,7 +265,7 @@ struct QuadWaves {
         __syncthreads();
     }
     template <int K>
-    __device__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
+    __device__ __forceinline__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
         static_assert(K <= 4, "reduction width");
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -277,12 +277,12 @@ struct QuadWaves {
         for (int k = 0; k < K; k++) v[k] = ((r.f[0][k] + r.f[1][k]) + r.f[2][k]) + r.f[3][k];
         __syncthreads();
     }
-    __device__ static double sum(double v, Red& r, int lane, int wr) {
+    __device__ __forceinline__ static double sum(double v, Red& r, int lane, int wr) {
         double a[1] = {v};
         sumv<1>(a, r, lane, wr);
         return a[0];
     }
-    __device__ static float sum(float v, Red& r, int lane, int wr) {
+    __device__ __forceinline__ static float sum(float v, Red& r, int lane, int wr) {
         float a[1] = {v};
         sumv<1>(a, r, lane, wr);
         return a[0];
@@ -294,15 +294,15 @@ struct Synth128 : QuadWaves {
     using Shared = NoShared;
     static constexpr int kLmFloats = 2 * (int)(sizeof(Buf128) / sizeof(float));
     static constexpr int kRefFloats = kLmFloats;
-    __device__ static void load(Shared&, const Model&) {}
+    __device__ __forceinline__ static void load(Shared&, const Model&) {}
     template <int R>
-    __device__ static void eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
+    __device__ __forceinline__ static void eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
                                 float z, float zoil, float (&v)[Roi128<R>::RQ]) {
         float d0[Roi128<R>::RQ], d1[Roi128<R>::RQ], d2[Roi128<R>::RQ];
         eval128<R, false>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, v, d0, d1, d2);
     }
     template <int R>
-    __device__ static void lm_eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
+    __device__ __forceinline__ static void lm_eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x, float y,
                                    float z, float zoil, float (&vB)[Roi128<R>::RQ], float (&vX)[Roi128<R>::RQ],
                                    float (&vY)[Roi128<R>::RQ], float (&vZ)[Roi128<R>::RQ]) {
         eval128<R, true>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
@@ -315,15 +315,15 @@ struct Synth64q : QuadWaves {
     using Shared = BlockTables64;
     static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
     static constexpr int kRefFloats = (int)(sizeof(WaveScratch64) / sizeof(float));
-    __device__ static void load(Shared& T, const Model& M) { load_tables64(T, M); }
+    __device__ __forceinline__ static void load(Shared& T, const Model& M) { load_tables64(T, M); }
     template <int R>
-    __device__ static void eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
+    __device__ __forceinline__ static void eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
                                 float z, float zoil, float (&v)[Roi64q<R>::RQ]) {
         float d0[Roi64q<R>::RQ], d1[Roi64q<R>::RQ], d2[Roi64q<R>::RQ];
         eval64q<R, false>(M, T, scr, lane, wr, x, y, z, zoil, v, d0, d1, d2);
     }
     template <int R>
-    __device__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
+    __device__ __forceinline__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
                                    float z, float zoil, float (&vB)[Roi64q<R>::RQ], float (&vX)[Roi64q<R>::RQ],
                                    float (&vY)[Roi64q<R>::RQ], float (&vZ)[Roi64q<R>::RQ]) {
         eval64q<R, true>(M, T, scr, lane, wr, x, y, z, zoil, vB, vX, vY, vZ);

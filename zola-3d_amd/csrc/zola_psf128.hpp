@@ -166,11 +166,12 @@ __device__ __forceinline__ void row_blur_full(const float* Ib, const float* Izb,
 // NO = 4: (h1, Rb0), (hd, Rb0), (h1, Rb1), (h1, Rb2).
 template <int R, int I0, int NI, int NO>
 __device__ __forceinline__ void column_blur_full(const float* Rb, int j, const float (&h1)[2], const float (&hd)[2],
-                                                 float (&out)[NO][NI]) {
+                                                 float (&o0)[NI], float (&o1)[NI], float (&o2)[NI], float (&o3)[NI]) {
 #pragma unroll
-    for (int o = 0; o < NO; o++)
-#pragma unroll
-        for (int ii = 0; ii < NI; ii++) out[o][ii] = 0.f;
+    for (int ii = 0; ii < NI; ii++) {
+        o0[ii] = 0.f;
+        if constexpr (NO == 4) o1[ii] = o2[ii] = o3[ii] = 0.f;
+    }
     static_for<0, 16>([&](auto pcc) {
         constexpr int p0 = decltype(pcc)::value * 8;
         float r0[8], r1[NO == 4 ? 8 : 1], r2[NO == 4 ? 8 : 1];
@@ -193,11 +194,11 @@ __device__ __forceinline__ void column_blur_full(const float* Rb, int j, const f
                 constexpr int t = decltype(tc)::value;
                 constexpr int ii = d + R / 2 + p0 + t - I0;
                 if constexpr (ii >= 0 && ii < NI) {
-                    out[0][ii] += c1 * r0[t];
+                    o0[ii] += c1 * r0[t];
                     if constexpr (NO == 4) {
-                        out[1][ii] += cd * r0[t];
-                        out[2][ii] += c1 * r1[t];
-                        out[3][ii] += c1 * r2[t];
+                        o1[ii] += cd * r0[t];
+                        o2[ii] += c1 * r1[t];
+                        o3[ii] += c1 * r2[t];
                     }
                 }
             });
@@ -299,21 +300,11 @@ __device__ __forceinline__ void eval128(const Model128& M, Buf128 (&S)[2], int l
     // ---- column blur: lane j, ROI rows [wr*RQ, wr*RQ + RQ)
     const int j = lane < R ? lane : 0;
     constexpr int NO = TANG ? 4 : 1;
-    float o[NO][RQ];
     switch (wr) {
-        case 0: column_blur_full<R, 0 * RQ, RQ, NO>(Rb, j, h1, hd, o); break;
-        case 1: column_blur_full<R, 1 * RQ, RQ, NO>(Rb, j, h1, hd, o); break;
-        case 2: column_blur_full<R, 2 * RQ, RQ, NO>(Rb, j, h1, hd, o); break;
-        default: column_blur_full<R, 3 * RQ, RQ, NO>(Rb, j, h1, hd, o); break;
-    }
-#pragma unroll
-    for (int t = 0; t < RQ; t++) {
-        vB[t] = o[0][t];
-        if constexpr (TANG) {
-            vX[t] = o[1][t];
-            vY[t] = o[2][t];
-            vZ[t] = o[3][t];
-        }
+        case 0: column_blur_full<R, 0 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
+        case 1: column_blur_full<R, 1 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
+        case 2: column_blur_full<R, 2 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
+        default: column_blur_full<R, 3 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
     }
     __syncthreads();  // the buffers are reused by the next evaluation
 }
