@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-frames", action="store_true")
     p.add_argument("--no-formats", action="store_true")
+    p.add_argument("--no-beads", action="store_true")
     p.add_argument("--n128-batch", type=int, default=65536)
     p.add_argument("--mixed-total", type=int, default=200000)
     p.add_argument("--frames", type=int, default=2000)
@@ -390,6 +391,53 @@ def formats_bench(Z, n=1_000_000):
             "calib_write_us": round(t_w * 1e6, 2), "calib_read_us": round(t_r * 1e6, 2)}
 
 
+def beads_bench(Z, stream, cpu=True):
+    """Calibration bead preparation (InitBackgroundAndPhotonNumber): a raw
+    101-slice 512x512 stack (20 nm steps), 12 beads of 3e4 photons on a
+    background of 30, Poisson noise, 32 px patches; host arrays in and out."""
+    S, W, H, P, nb, zstep = 101, 512, 512, 32, 12, 0.02
+    rng = np.random.default_rng(0x20A1D3 + 5)
+    lam = np.full((S, W, H), 30.0)
+    beads = rng.uniform(40, 470, (nb, 2))
+    u = np.arange(-16, 17)
+    for z in range(S):
+        s = 1.2 + 0.02 * abs(z - S // 2)
+        for bx, by in beads:
+            ix, iy = int(bx) + u, int(by) + u
+            lam[z][np.ix_(ix, iy)] += 3e4 / (2 * np.pi * s * s) * np.exp(
+                -((ix[:, None] - bx) ** 2 + (iy[None, :] - by) ** 2) / (2 * s * s))
+    adu = (rng.poisson(lam) + 100).astype(np.float32)
+    del lam
+    xp, yp = beads[:, 0] + 0.7, beads[:, 1] - 0.6
+    ctx = Z.Context(Z.optics(), R, Z.astigmatic_coeffs(28))
+    ctx.set_stream(stream.cuda_stream)
+    ctx.bead_prepare(adu, xp, yp, P, zstep, offset=100.0)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        res = ctx.bead_prepare(adu, xp, yp, P, zstep, offset=100.0)
+        ts.append(time.perf_counter() - t0)
+    dt = float(np.median(ts))
+    ctx.close()
+    out = {"workload": f"{S} slices x {W}x{H} px raw stack, {nb} beads, {P} px patches, z step {zstep} um "
+                       "(background fit window +-12 slices)",
+           "call_ms": round(dt * 1e3, 3), "stacks_per_s": round(1.0 / dt, 3),
+           "voxels_per_s": round(S * W * H / dt, 1), "centre_slice": res["diag"]["miniposition"],
+           "background_pixels": res["diag"]["ndata"],
+           "note": "host arrays in and out: includes the 106 MB host-to-device copy of the stack"}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        img = O.bead_convert(adu, 1.0, 1.0, 100.0)
+        t0 = time.perf_counter()
+        O.bead_prepare(img, xp, yp, P, zstep)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / tc, 4), "unit": "stacks/s", "cores": 1, "kind": "port",
+                               "sample": f"the same stack through the FP64 C restatement on one core, "
+                                         f"{tc:.2f} s (conversion excluded)"}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -544,6 +592,8 @@ def main():
         out["frames_n128"] = frames_bench(torch, Z, stream, args.frames)
     if not args.no_formats:
         out["formats"] = formats_bench(Z)
+    if not args.no_beads:
+        out["bead_prep"] = beads_bench(Z, stream, cpu=not args.no_cpu)
 
     if args.copy_probe_mb:
         n = args.copy_probe_mb * (1 << 20) // 4

@@ -322,6 +322,50 @@ int  zola_localize_frames_scmos(zola_detector* det, const zola_fit_config* fit,
                                 const float* offset, const float* gain, const float* variance, int cap,
                                 double* locs, int* n_locs, int* per_frame, int* rounds);
 
+/* ---------------------------------------------------------------------------
+ * Calibration bead preparation, replacing InitBackgroundAndPhotonNumber
+ * (process/InitBackgroundAndPhotonNumber.java: constructors 50-97,
+ * initBackgroundAndPhotonNumberMany 99-468, refinePoints 563-618,
+ * computeBackground 622-904, with utils/PolynomialFit.java at order 2 and
+ * utils/GaussianElimination.java), which ZOLA.java:2671-2675 builds from the
+ * bead z-stack before the phase retrieval.  Host arrays; the stack is
+ * processed in FP64 on the context's device and stream.
+ *
+ * adu: S x W x H raw stack [z][i][ii] (i the ImageJ x column, ii fastest),
+ * S*W*H < 2^31.  xp, yp: nb seeds (float pixels), refined in place.  P: patch
+ * side (sizePatchPhaseRet).  Outputs: patches nb x S x P x P (converted stack
+ * + var/gain^2 for sCMOS inside the frame, B outside), B nb (mean of the
+ * polynomial background over the bead square), A nb x S (photon sum of each
+ * slice minus B, averaged over +-max((int)(0.5/zstep)/2, 1) slices),
+ * background W x H (the fitted polynomial; may be NULL), diag (may be NULL).
+ * Returns ZOLA_EINVAL, with the message of zola_last_error, when the
+ * background normal equations are singular (the reference fails there too). */
+typedef struct zola_bead_diag {
+    double mean0, std0;   /* mean and std of the whole stack                   */
+    double mean, std;     /* mean over v <= mean0 + 2 std0; std over v <= mean0 + 3 std0 */
+    double coef[6];       /* background a0 + a1 x + a2 x^2 + a3 y + a4 xy + a5 y^2, x = i */
+    int size_filt;        /* slice filter half-width                           */
+    int miniposition;     /* centre slice of the background window             */
+    int ndata;            /* pixels in the background fit                      */
+} zola_bead_diag;
+
+/* CCD/EMCCD constructor (50-77): photons = (adu*adu_factor/gain -
+ * offset*adu_factor/gain) + 1, clamped at 0. */
+int  zola_bead_prepare(zola_ctx* ctx, int S, int W, int H, const float* adu, double adu_factor,
+                       double gain, double offset, double zstep, int P, int nb, float* xp, float* yp,
+                       double* patches, double* B, double* A, double* background,
+                       zola_bead_diag* diag);
+
+/* sCMOS constructor (79-97): photons = (adu - offset)/gain, clamped at 0;
+ * offset, gain, variance W x H maps (utils/SCMOScamera.java:90-96, var/gain^2
+ * formed in FP64).  scmos_patches nb x P x P (may be NULL) as the reference
+ * leaves them: var/gain^2 for the last bead only, zero for the others (the
+ * array is re-allocated inside its bead loop, 433-447). */
+int  zola_bead_prepare_scmos(zola_ctx* ctx, int S, int W, int H, const float* adu, const float* offset,
+                             const float* gain, const float* variance, double zstep, int P, int nb,
+                             float* xp, float* yp, double* patches, double* B, double* A,
+                             double* scmos_patches, double* background, zola_bead_diag* diag);
+
 #ifdef __cplusplus
 }
 #endif

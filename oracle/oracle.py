@@ -89,6 +89,12 @@ def lib():
         L.zo_search_psf_center.restype = C.c_double
         L.zo_search_psf_center.argtypes = [vp, C.c_double, C.c_double]
         L.zo_calib_nll.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.zo_bead_convert.argtypes = [C.c_long, _dp, C.c_double, C.c_double, C.c_double]
+        L.zo_bead_convert_scmos.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.zo_bead_prepare.restype = C.c_int
+        L.zo_bead_prepare.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_void_p, C.c_double, C.c_int,
+                                      C.c_int, _fp, _fp, _dp, _dp, _dp, C.c_void_p, _dp, _dp, _dp, _ip]
         _lib = L
     return _lib
 
@@ -312,3 +318,43 @@ def calib_nll(model, stack, dz, xs, ys, zs, Aslice, bg, zwat=0.0):
     S = stack.shape[0]
     return lib().zo_calib_nll(model._h, S, _d(stack), _d(dz), _d(xs), _d(ys), _d(zs),
                               _d(Aslice), _d(bg), zwat)
+
+
+def bead_convert(adu, adu_factor, gain, offset):
+    """CCD/EMCCD conversion of a raw bead stack (float ADU) to photons + 1, clamped at 0."""
+    img = np.ascontiguousarray(adu, dtype=np.float64).copy()
+    lib().zo_bead_convert(img.size, img, adu_factor, gain, offset)
+    return img
+
+
+def bead_convert_scmos(adu, offset, gain):
+    S, W, H = adu.shape
+    img = np.ascontiguousarray(adu, dtype=np.float64).copy()
+    lib().zo_bead_convert_scmos(S, W, H, img, _d(np.asarray(offset, np.float32).astype(np.float64)),
+                                _d(np.asarray(gain, np.float32).astype(np.float64)))
+    return img
+
+
+def bead_prepare(image, xp, yp, P, zstep, vargain=None):
+    """Bead preparation on a converted stack image (S x W x H FP64) with float
+    seeds xp, yp (refined copies are returned)."""
+    S, W, H = image.shape
+    nb = len(xp)
+    xp = np.array(xp, dtype=np.float32)
+    yp = np.array(yp, dtype=np.float32)
+    patches = np.zeros((nb, S, P, P))
+    B = np.zeros(max(nb, 1))
+    A = np.zeros((max(nb, 1), S))
+    sp = np.zeros((max(nb, 1), P, P)) if vargain is not None else None
+    poly = np.zeros((W, H))
+    coef = np.zeros(6)
+    stats = np.zeros(4)
+    info = np.zeros(3, np.int32)
+    vg = None if vargain is None else _d(vargain)
+    rc = lib().zo_bead_prepare(S, W, H, _d(image), None if vg is None else vg.ctypes.data, zstep, P, nb, xp, yp,
+                               patches if nb else np.zeros(1), B, A, None if sp is None else sp.ctypes.data,
+                               poly, coef, stats, info)
+    return dict(rc=rc, x=xp, y=yp, patches=patches, B=B[:nb], A=A[:nb], scmos=None if sp is None else sp[:nb],
+                poly=poly, coef=coef, stats=stats, size_filt=int(info[0]), miniposition=int(info[1]),
+                ndata=int(info[2]))
+

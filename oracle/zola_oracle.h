@@ -22,6 +22,9 @@
  *   zo_frame_localize    process/cpu/LocalizationPipeline.java:721-1073,1099-1250
  *   zo_search_psf_center process/cpu/SearchPSFcenter.java:37-351
  *   zo_calib_nll         process/cpu/ZernikePhaseRetrieval.java:131-140,1729-1736,1842-1879
+ *   zo_bead_*            process/InitBackgroundAndPhotonNumber.java:50-468,563-904,
+ *                        utils/PolynomialFit.java, utils/GaussianElimination.java
+ *                        (zola_beads.c)
  * Functions with no reference counterpart (analytic tangents, the adjoint,
  * the FP64 Levenberg-Marquardt optimum) are derived from the same model and
  * are pinned by central finite differences of zo_model_psf.
@@ -168,6 +171,21 @@ double zo_search_psf_center(zo_model* m, double zfocus, double axialRange);
 double zo_calib_nll(zo_model* m, int S, const double* stack /* S*R*R */, const double* dz,
                     const double* xs, const double* ys, const double* zs,
                     const double* Aslice, const double* bg /* R*R */, double zwat);
+
+/* Calibration bead preparation (zola_beads.c).  image S x W x H [z][i][ii].
+ * zo_bead_convert: (v*adu/gain - offset*adu/gain) + 1, clamped at 0 (CCD/EMCCD
+ * constructor); zo_bead_convert_scmos: (v - offset)/gain, clamped at 0.
+ * zo_bead_prepare refines the seeds xp/yp (nb, in place), fits the order-2
+ * polynomial background and writes patches nb x S x P x P (+ vargain where
+ * given, B outside the frame), B nb, A nb x S, scmos_patches nb x P x P (may
+ * be NULL), poly W x H, coef 6, stats {mean0, std0, mean, std}, info
+ * {sizeFilt, miniposition, ndata}.  Returns 0, -1 (singular normal
+ * equations) or -2 (bad sizes). */
+void zo_bead_convert(long n, double* image, double adu, double gain, double offset);
+void zo_bead_convert_scmos(int S, int W, int H, double* image, const double* offset, const double* gain);
+int  zo_bead_prepare(int S, int W, int H, const double* image, const double* vargain, double zstep, int P,
+                     int nb, float* xp, float* yp, double* patches, double* B, double* A,
+                     double* scmos_patches, double* poly, double* coef, double* stats, int* info);
 
 #ifdef __cplusplus
 }

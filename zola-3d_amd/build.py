@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libzola_hip.so")
 SOURCES = (["zola_dispatch.hip", "zola_host.cpp", "zola_calib.cpp", "zola_detect.hip", "zola_frames.cpp",
-            "zola_center.cpp", "zola_formats.cpp"] +
+            "zola_center.cpp", "zola_formats.cpp", "zola_beads.hip"] +
            sorted(os.path.basename(p) for p in glob.glob(os.path.join(CSRC, "zola_inst_*.hip"))))
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature",
          "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
@@ -21,7 +21,8 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-tar
 
 def _deps_mtime(src):
     """Newest mtime of the source and of every header it may include."""
-    hdrs = glob.glob(os.path.join(CSRC, "*.h*")) + glob.glob(os.path.join(HERE, "..", "include", "*.h"))
+    hdrs = (glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(CSRC, "*.hpp")) +
+            glob.glob(os.path.join(HERE, "..", "include", "*.h")))
     return max(os.path.getmtime(p) for p in [os.path.join(CSRC, src)] + hdrs)
 
 

@@ -33,7 +33,7 @@ SYMBOLS = [
     "zola_search_psf_center", "zola_set_fit_layout", "zola_autotune_fit",
     "zola_java_double_string",
     "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
-    "zola_formats_last_error",
+    "zola_formats_last_error", "zola_bead_prepare", "zola_bead_prepare_scmos",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -54,6 +54,11 @@ class FitConfig(C.Structure):
     _fields_ = [("mode", C.c_int), ("max_iter", C.c_int), ("zmin", C.c_double),
                 ("zmax", C.c_double), ("zfocus", C.c_double), ("photon_threshold", C.c_double),
                 ("tol", C.c_double)]
+
+
+class BeadDiag(C.Structure):
+    _fields_ = [("mean0", C.c_double), ("std0", C.c_double), ("mean", C.c_double), ("std", C.c_double),
+                ("coef", C.c_double * 6), ("size_filt", C.c_int), ("miniposition", C.c_int), ("ndata", C.c_int)]
 
 
 class AcceptConfig(C.Structure):
@@ -122,6 +127,13 @@ def lib():
                                                C.c_int, vp, C.c_int, vp, vp, vp, vp]
             L.zola_localize_frames_scmos.argtypes = [vp, C.POINTER(FitConfig), C.POINTER(AcceptConfig),
                                                      C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]
+        if hasattr(L, "zola_bead_prepare"):
+            L.zola_bead_prepare.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp,
+                                            C.POINTER(BeadDiag)]
+            L.zola_bead_prepare_scmos.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_double,
+                                                  C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                                  C.POINTER(BeadDiag)]
         L.zola_set_fit_layout.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.zola_autotune_fit.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), vp]
         L.zola_java_double_string.argtypes = [C.c_double, C.c_char_p, C.c_int]
@@ -320,6 +332,50 @@ class Context:
         self._check(lib().zola_search_psf_center(self._h, zfocus, axial_range, C.byref(out)),
                     "search_psf_center")
         return out.value
+
+    # -- calibration bead preparation ---------------------------------------
+    def _bead_out(self, S, W, H, P, xp, yp):
+        nb = len(xp)
+        return dict(x=np.array(xp, dtype=np.float32), y=np.array(yp, dtype=np.float32),
+                    patches=np.zeros((nb, S, P, P)), B=np.zeros(nb), A=np.zeros((nb, S)),
+                    background=np.zeros((W, H)))
+
+    @staticmethod
+    def _bead_diag(d):
+        return dict(mean0=d.mean0, std0=d.std0, mean=d.mean, std=d.std, coef=np.array(d.coef[:]),
+                    size_filt=d.size_filt, miniposition=d.miniposition, ndata=d.ndata)
+
+    def bead_prepare(self, adu, xp, yp, P, zstep, adu_factor=1.0, gain=1.0, offset=0.0):
+        """InitBackgroundAndPhotonNumber (CCD/EMCCD form).  adu (S, W, H) float32
+        raw stack [z][i][ii]; xp, yp float seeds.  Returns refined seeds,
+        patches (nb, S, P, P), B (nb), A (nb, S), background (W, H) and diag."""
+        a = np.ascontiguousarray(adu, dtype=np.float32)
+        S, W, H = a.shape
+        o = self._bead_out(S, W, H, P, xp, yp)
+        d = BeadDiag()
+        self._check(lib().zola_bead_prepare(self._h, S, W, H, _ptr(a), adu_factor, gain, offset, zstep, P,
+                                            len(o["x"]), _ptr(o["x"]), _ptr(o["y"]), _ptr(o["patches"]),
+                                            _ptr(o["B"]), _ptr(o["A"]), _ptr(o["background"]), C.byref(d)),
+                    "bead_prepare")
+        o["diag"] = self._bead_diag(d)
+        return o
+
+    def bead_prepare_scmos(self, adu, offset, gain, variance, xp, yp, P, zstep):
+        """InitBackgroundAndPhotonNumber (sCMOS form) with float camera maps (W, H);
+        adds scmos (nb, P, P) as the reference leaves it (last bead only)."""
+        a = np.ascontiguousarray(adu, dtype=np.float32)
+        S, W, H = a.shape
+        maps = [np.ascontiguousarray(m, dtype=np.float32) for m in (offset, gain, variance)]
+        o = self._bead_out(S, W, H, P, xp, yp)
+        o["scmos"] = np.zeros((len(o["x"]), P, P))
+        d = BeadDiag()
+        self._check(lib().zola_bead_prepare_scmos(self._h, S, W, H, _ptr(a), _ptr(maps[0]), _ptr(maps[1]),
+                                                  _ptr(maps[2]), zstep, P, len(o["x"]), _ptr(o["x"]), _ptr(o["y"]),
+                                                  _ptr(o["patches"]), _ptr(o["B"]), _ptr(o["A"]),
+                                                  _ptr(o["scmos"]), _ptr(o["background"]), C.byref(d)),
+                    "bead_prepare_scmos")
+        o["diag"] = self._bead_diag(d)
+        return o
 
     def fit_zoil_dev(self, cfg, pixels, init, zoil, theta, nll, crlb, iters, status):
         """Device form with a per-ROI focus term zoil (B floats)."""
