@@ -194,7 +194,10 @@ int  zola_fit_mixed_dev(zola_ctx* ctx, const zola_fit_config* cfg, int G, const 
  * ROI: z from the best zero-shift normalised correlation against nZ templates
  * PSF(0,0,z_r) (z_r = zmin_t + r*zstep_t), then getAandB with that template
  * (process/cpu/LocalizationPipeline.java:1119-1162, threshold 0.1).  Writes
- * init B x 5 doubles [A, B, 0, 0, z_r]. */
+ * init B x 5 doubles [A, B, 0, 0, z_r].  The template stack is built on the
+ * first call and kept while the model (coefficients, sigma) and (nZ, zmin_t,
+ * zstep_t, zfocus) stay the same; a call that rebuilds it waits for the
+ * stream, a call that reuses it only enqueues the kernel. */
 int  zola_fit_init_dev(zola_ctx* ctx, int B, const float* pixels, int nZ, double zmin_t,
                        double zstep_t, double zfocus, double* init);
 

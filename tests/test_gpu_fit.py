@@ -45,6 +45,39 @@ def test_fit_init_matches_restatement(setup, torch_cuda):
     np.testing.assert_allclose(got[:, 1], init[:, 1], rtol=1e-3)
 
 
+def test_fit_init_keeps_templates_until_the_model_changes(setup, torch_cuda):
+    """The template stack is reused while model and z grid are unchanged: a
+    repeated call is bit-identical, and after new coefficients or another grid
+    the result equals a fresh context's."""
+    torch = torch_cuda
+    m, ctx, rois, truth, init = setup
+    print("case: init template cache", flush=True)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    px = torch.from_numpy(rois).cuda()
+
+    def run(c, nz, z0, dz):
+        out = torch.zeros(B, 5, dtype=torch.float64, device="cuda")
+        c.fit_init_dev(px, nz, z0, dz, 0.0, out)
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    first = run(ctx, NZ, ZMIN_T, ZSTEP_T)
+    np.testing.assert_array_equal(run(ctx, NZ, ZMIN_T, ZSTEP_T), first)
+    a0 = Z.astigmatic_coeffs(28)
+    a1 = a0.copy()
+    a1[4] += 0.3
+    fresh = Z.Context(Z.optics(), R, a1)
+    fresh.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        ctx.set_coefficients(a1)
+        np.testing.assert_array_equal(run(ctx, NZ, ZMIN_T, ZSTEP_T), run(fresh, NZ, ZMIN_T, ZSTEP_T))
+        np.testing.assert_array_equal(run(ctx, 41, -1.0, 0.05), run(fresh, 41, -1.0, 0.05))
+    finally:
+        ctx.set_coefficients(a0)
+        fresh.close()
+    np.testing.assert_array_equal(run(ctx, NZ, ZMIN_T, ZSTEP_T), first)
+
+
 def test_lm_fit_matches_fp64_optimum(setup):
     """GPU LM (FP32 synthesis, tol 0.1 nm) against the FP64 optimum (tol 1e-10 um)."""
     m, ctx, rois, truth, init = setup

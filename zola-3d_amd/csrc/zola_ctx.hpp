@@ -30,8 +30,13 @@ struct zola_ctx {
     // grow-only staging for the host-pointer entry points
     void* d_stage = nullptr;
     size_t stage_bytes = 0;
-    float* d_tmpl = nullptr;      // template stack for zola_fit_init_dev
+    float* d_tmpl = nullptr;      // template stack for zola_fit_init_dev, then its per-template sums
     size_t tmpl_bytes = 0;
+    // the template stack is kept while the model and its z grid are unchanged
+    unsigned model_ver = 0;       // bumped whenever the pupil tables or blur taps change
+    unsigned tmpl_ver = ~0u;
+    int tmpl_nZ = 0;
+    double tmpl_key[3] = {};      // zmin_t, zstep_t, zfocus
     double* d_Zw = nullptr;       // Zernike basis on the 32x32 window, J x 1024 (FP64)
     float* d_qbar = nullptr;      // per-PSF dL/dphase for the adjoint, grow-only
     size_t qbar_bytes = 0;

@@ -20,8 +20,10 @@ hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params
 template <int R>
 hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st);
 template <int R>
-hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
-                         double zstep_t, double* init, hipStream_t st);
+hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, const double* tstat,
+                         double zmin_t, double zstep_t, double* init, hipStream_t st);
+template <int R>
+hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t st);
 
 // da_j += sum_b sum_p Z_j(p) qbar[b][p] (FP64), one block per coefficient.
 __global__ __launch_bounds__(256) void k_project_zernike(int B, int J, const double* __restrict__ Zw,
@@ -84,8 +86,15 @@ hipError_t zola_launch_fit64(const Model64& M, int R, int mode, int quad, const 
 }
 
 hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,
-                                  double zmin_t, double zstep_t, double* init, hipStream_t st) {
-#define CALL(RR) launch_init_r<RR>(B, pixels, nZ, tmpl, zmin_t, zstep_t, init, st)
+                                  const double* tstat, double zmin_t, double zstep_t, double* init,
+                                  hipStream_t st) {
+#define CALL(RR) launch_init_r<RR>(B, pixels, nZ, tmpl, tstat, zmin_t, zstep_t, init, st)
+    ZOLA_R_DISPATCH(R, CALL)
+#undef CALL
+}
+
+hipError_t zola_launch_template_stats64(int R, int nZ, const float* tmpl, double* tstat, hipStream_t st) {
+#define CALL(RR) launch_tstats_r<RR>(nZ, tmpl, tstat, st)
     ZOLA_R_DISPATCH(R, CALL)
 #undef CALL
 }

@@ -85,6 +85,7 @@ static int compute_h1(zola_ctx* c) {
         }
         hd[d] = acc / s2 / c->o.xystep;
     }
+    c->model_ver++;
     if (s2 == zola::kN) {
         for (int d = 0; d < s2; d++) {
             c->model.h1[d] = (float)h1[d];
@@ -109,6 +110,7 @@ static int upload_taps128(zola_ctx* c) {
 }
 
 static int upload_tables(zola_ctx* c) {
+    c->model_ver++;
     if (c->N == zola::kN2) {
         std::vector<float4> tab(zola::kW2 * zola::kW2, make_float4(0.f, 0.f, 0.f, 0.f));
         for (int p = 0; p < c->D; p++)
@@ -760,29 +762,46 @@ int zola_fit_init_dev(zola_ctx* c, int B, const float* pixels, int nZ, double zm
     if (B == 0) return ZOLA_OK;
     if (!pixels || !init) return set_err(c, ZOLA_EINVAL, "null buffer");
     const size_t R2 = (size_t)c->R * c->R;
-    const size_t need = (size_t)nZ * (R2 + 4) * sizeof(float);
-    if (need > c->tmpl_bytes) {
-        if (c->d_tmpl) HIP_TRY(c, hipFree(c->d_tmpl));
-        c->d_tmpl = nullptr;
-        c->tmpl_bytes = 0;
-        HIP_TRY(c, hipMalloc((void**)&c->d_tmpl, need));
-        c->tmpl_bytes = need;
+    // [nZ R^2 template floats][nZ x 4 parameter floats][pad to 16 B][nZ x 2 doubles of sums]
+    const size_t stat_off = ((size_t)nZ * (R2 + 4) * sizeof(float) + 15) & ~(size_t)15;
+    const size_t need = stat_off + (size_t)nZ * 2 * sizeof(double);
+    const bool cached = c->d_tmpl && c->tmpl_ver == c->model_ver && c->tmpl_nZ == nZ &&
+                        c->tmpl_key[0] == zmin_t && c->tmpl_key[1] == zstep_t && c->tmpl_key[2] == zfocus;
+    double* tstat = nullptr;
+    if (!cached) {
+        c->tmpl_ver = ~0u;
+        if (need > c->tmpl_bytes) {
+            if (c->d_tmpl) HIP_TRY(c, hipFree(c->d_tmpl));
+            c->d_tmpl = nullptr;
+            c->tmpl_bytes = 0;
+            HIP_TRY(c, hipMalloc((void**)&c->d_tmpl, need));
+            c->tmpl_bytes = need;
+        }
+        tstat = reinterpret_cast<double*>(reinterpret_cast<char*>(c->d_tmpl) + stat_off);
+        // templates PSF(0, 0, z_r) with z_r = zmin_t + r * zstep_t (Predetection.java:118-125,190)
+        std::vector<float> par((size_t)nZ * 4, 0.f);
+        for (int r = 0; r < nZ; r++) {
+            par[(size_t)r * 4 + 2] = (float)(zmin_t + r * zstep_t);
+            par[(size_t)r * 4 + 3] = (float)zfocus;
+        }
+        float* dpar = c->d_tmpl + (size_t)nZ * R2;
+        HIP_TRY(c, hipMemcpyAsync(dpar, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice,
+                                  c->stream));
+        int rc = zola_psf_forward_dev(c, nZ, dpar, c->d_tmpl);
+        if (rc) return rc;
+        HIP_TRY(c, zola_launch_template_stats64(c->R, nZ, c->d_tmpl, tstat, c->stream));
+        // the parameter upload must not be overwritten before the kernels read it
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->tmpl_ver = c->model_ver;
+        c->tmpl_nZ = nZ;
+        c->tmpl_key[0] = zmin_t;
+        c->tmpl_key[1] = zstep_t;
+        c->tmpl_key[2] = zfocus;
+    } else {
+        tstat = reinterpret_cast<double*>(reinterpret_cast<char*>(c->d_tmpl) + stat_off);
     }
-    // templates PSF(0, 0, z_r) with z_r = zmin_t + r * zstep_t (Predetection.java:118-125,190)
-    std::vector<float> par((size_t)nZ * 4, 0.f);
-    for (int r = 0; r < nZ; r++) {
-        par[(size_t)r * 4 + 2] = (float)(zmin_t + r * zstep_t);
-        par[(size_t)r * 4 + 3] = (float)zfocus;
-    }
-    float* dpar = c->d_tmpl + (size_t)nZ * R2;
-    HIP_TRY(c, hipMemcpyAsync(dpar, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice,
-                              c->stream));
-    int rc = zola_psf_forward_dev(c, nZ, dpar, c->d_tmpl);
-    if (rc) return rc;
-    HIP_TRY(c, zola_launch_fit_init64(c->R, B, pixels, nZ, c->d_tmpl, zmin_t, zstep_t, init,
+    HIP_TRY(c, zola_launch_fit_init64(c->R, B, pixels, nZ, c->d_tmpl, tstat, zmin_t, zstep_t, init,
                                       c->stream));
-    // the parameter upload must not be overwritten before the kernels read it
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return ZOLA_OK;
 }
 

@@ -18,8 +18,12 @@ hipError_t zola_launch_psf_forward64(const zola::Model64& M, int R, int B, const
 hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, int quad, const zola::FitArgs& A,
                              hipStream_t st);
 
+// tstat: per-template {sum t, sum t^2 - (sum t)^2 / R^2} from zola_launch_template_stats64
 hipError_t zola_launch_fit_init64(int R, int B, const float* pixels, int nZ, const float* tmpl,
-                                  double zmin_t, double zstep_t, double* init, hipStream_t st);
+                                  const double* tstat, double zmin_t, double zstep_t, double* init,
+                                  hipStream_t st);
+
+hipError_t zola_launch_template_stats64(int R, int nZ, const float* tmpl, double* tstat, hipStream_t st);
 
 hipError_t zola_launch_adjoint64(const zola::Model64& M, int R, const zola::AdjArgs& A,
                                  hipStream_t st);

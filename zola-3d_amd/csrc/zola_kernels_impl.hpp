@@ -75,9 +75,37 @@ namespace zola {
 // against the template stack (z refined by the reference's parabola,
 // process/cpu/Predetection.java:529-531,589-593), then getAandB with that
 // template (process/cpu/LocalizationPipeline.java:1119-1162).
+// Per-template sums of the correlation (sum t, and sum t^2 - (sum t)^2 / R^2),
+// computed once per template stack with the lane layout and reduction order of
+// k_fit_init64.  One wave per template.
+template <int R, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_template_stats64(int nZ, const float* __restrict__ tmpl,
+                                                               double* __restrict__ tstat) {
+    constexpr int RPL = RoiLayout<R>::RPL;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * WPB + w;
+    if (r >= nZ) return;
+    const int j = lane % R, g = lane / R;
+    double st = 0, stt = 0;
+#pragma unroll
+    for (int t = 0; t < RPL; t++) {
+        if (!Px<R>::valid(lane, t)) continue;
+        const double tv = tmpl[(size_t)r * R * R + (g * RPL + t) * R + j];
+        st += tv;
+        stt += tv * tv;
+    }
+    st = wave_sum(st);
+    stt = wave_sum(stt);
+    if (lane == 0) {
+        tstat[2 * r] = st;
+        tstat[2 * r + 1] = stt - st * st / ((double)R * R);
+    }
+}
+
 template <int R, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __restrict__ pixels,
                                                          int nZ, const float* __restrict__ tmpl,
+                                                         const double* __restrict__ tstat,
                                                          double zmin_t, double zstep_t,
                                                          double* __restrict__ init) {
     constexpr int RPL = RoiLayout<R>::RPL;
@@ -98,19 +126,15 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
     const double P = (double)R * R;
     const double vn = snn - sn * sn / P;
     auto corr = [&](int r) -> double {
-        double st = 0, stt = 0, stn = 0;
+        double stn = 0;
 #pragma unroll
         for (int t = 0; t < RPL; t++) {
             if (!Px<R>::valid(lane, t)) continue;
             const double tv = tmpl[(size_t)r * R * R + (g * RPL + t) * R + j];
-            st += tv;
-            stt += tv * tv;
             stn += tv * n[t];
         }
-        st = wave_sum(st);
-        stt = wave_sum(stt);
         stn = wave_sum(stn);
-        const double vt = stt - st * st / P;
+        const double st = tstat[2 * r], vt = tstat[2 * r + 1];
         return (stn - st * sn / P) / sqrt(vt * vn);
     };
     int best = 0;
@@ -254,11 +278,18 @@ hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStr
 }
 
 template <int R>
-hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, double zmin_t,
-                                double zstep_t, double* init, hipStream_t st) {
+hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t st) {
+    constexpr int W = 4;
+    hipLaunchKernelGGL((k_template_stats64<R, W>), dim3((nZ + W - 1) / W), dim3(64 * W), 0, st, nZ, tmpl, tstat);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, const double* tstat,
+                         double zmin_t, double zstep_t, double* init, hipStream_t st) {
     constexpr int W = 4;
     dim3 grid((B + W - 1) / W), block(64 * W);
-    hipLaunchKernelGGL((k_fit_init64<R, W>), grid, block, 0, st, B, pixels, nZ, tmpl, zmin_t,
+    hipLaunchKernelGGL((k_fit_init64<R, W>), grid, block, 0, st, B, pixels, nZ, tmpl, tstat, zmin_t,
                        zstep_t, init);
     return hipGetLastError();
 }
