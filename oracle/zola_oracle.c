@@ -973,6 +973,29 @@ static void lm_secant_update(double* S, const double* F, const double* s, const 
         }
 }
 
+/* Trial point of an LM step d from th, clamped as every trial is: x, y by at
+ * most one pixel dx, z by at most 0.2 um, A >= 1, B >= 0.01, zmin <= z <= zmax. */
+static void lm_clamp_trial(const double* th, const double* d, double dx, double zmin, double zmax,
+                           double* tn) {
+    for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
+    for (int p = 2; p < 4; p++) {
+        if (d[p] > dx) tn[p] = th[p] + dx;
+        if (d[p] < -dx) tn[p] = th[p] - dx;
+    }
+    if (d[4] > 0.2) tn[4] = th[4] + 0.2;
+    if (d[4] < -0.2) tn[4] = th[4] - 0.2;
+    tn[0] = fmax(tn[0], 1.0);
+    tn[1] = fmax(tn[1], 0.01);
+    tn[4] = fmin(fmax(tn[4], zmin), zmax);
+}
+
+/* The trial tc from th is below tol in x, y, z and below 1% of the statistical
+ * precision 1/sqrt(F_ii) in A and B (F row-major 5 x 5). */
+static int lm_small_step(const double* th, const double* tc, const double* F, double tol) {
+    return fmax(fabs(tc[2] - th[2]), fmax(fabs(tc[3] - th[3]), fabs(tc[4] - th[4]))) < tol &&
+           fabs(tc[0] - th[0]) * sqrt(F[0]) < 0.01 && fabs(tc[1] - th[1]) * sqrt(F[6]) < 0.01;
+}
+
 int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, double zmin,
               double zmax, int maxit, double tol, double* out_nll, int* out_iter,
               double* out_crlb) {
@@ -1000,14 +1023,37 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
             for (int p = 0; p < 25; p++) A[p] = H[p] + S[p];
             for (int p = 0; p < 5; p++) { A[p * 5 + p] += lambda * H[p * 5 + p]; d[p] = -g[p]; }
             if (chol_solve(5, A, d) != 0) { lambda *= nu; nu *= 2; continue; }
-            if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < tol &&
-                fabs(d[0]) * sqrt(H[0]) < 0.01 && fabs(d[1]) * sqrt(H[6]) < 0.01) {
-                /* the step from the point just accepted is below tol in x, y, z
-                 * and below 1% of the statistical precision in A and B: converged
-                 * there, without evaluating the step */
-                conv = 1;
-                accepted = 1;
-                break;
+            if (tries == 0) {
+                /* converged at the point just accepted when the step from it,
+                 * almost undamped (lambda at most 1e-4, raised tenfold while the
+                 * system is not positive definite) and clamped as trials are, is
+                 * below tol in x, y, z and below 1% of the statistical precision
+                 * in A and B; a damped step understates the distance along weakly
+                 * determined directions.  The almost undamped step is solved only
+                 * when the damped one already passes (it is the longer one). */
+                double tc[5];
+                lm_clamp_trial(th, d, dx, zmin, zmax, tc);
+                int small = lm_small_step(th, tc, H, tol);
+                if (small && lambda > 1e-4) {
+                    double dc[5], lc = 1e-4;
+                    int have = 0;
+                    for (int k = 0; k < 8 && !have && lc < lambda; k++) {
+                        double Ac[25];
+                        for (int p = 0; p < 25; p++) Ac[p] = H[p] + S[p];
+                        for (int p = 0; p < 5; p++) { Ac[p * 5 + p] += lc * H[p * 5 + p]; dc[p] = -g[p]; }
+                        if (chol_solve(5, Ac, dc) == 0) have = 1;
+                        else lc *= 10.0;
+                    }
+                    if (have) {
+                        lm_clamp_trial(th, dc, dx, zmin, zmax, tc);
+                        small = lm_small_step(th, tc, H, tol);
+                    }
+                }
+                if (small) {
+                    conv = 1;
+                    accepted = 1;
+                    break;
+                }
             }
             double pred = 0.0;
             for (int p = 0; p < 5; p++) {
@@ -1015,16 +1061,7 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
                 for (int q = 0; q < 5; q++) hd += (H[p * 5 + q] + S[p * 5 + q]) * d[q];
                 pred -= g[p] * d[p] + 0.5 * d[p] * hd;
             }
-            for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
-            for (int p = 2; p < 4; p++) {
-                if (d[p] > dx) tn[p] = th[p] + dx;
-                if (d[p] < -dx) tn[p] = th[p] - dx;
-            }
-            if (d[4] > 0.2) tn[4] = th[4] + 0.2;
-            if (d[4] < -0.2) tn[4] = th[4] - 0.2;
-            tn[0] = fmax(tn[0], 1.0);
-            tn[1] = fmax(tn[1], 0.01);
-            tn[4] = fmin(fmax(tn[4], zmin), zmax);
+            lm_clamp_trial(th, d, dx, zmin, zmax, tn);
             double nn = lm_nll(m, roi, tn, zfocus, psf);
             if (nn <= nll) {
                 double step = fmax(fabs(tn[2] - th[2]), fmax(fabs(tn[3] - th[3]), fabs(tn[4] - th[4])));
@@ -1037,7 +1074,9 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
                 memcpy(gold, g, sizeof(gold));
                 memcpy(th, tn, sizeof(th));
                 accepted = 1;
-                if (step < tol) conv = 1;
+                /* at a bound the clamped steps shrink to zero while the unclamped
+                 * step does not: a step below tol ends the fit there */
+                if (step < tol && (th[4] <= zmin || th[4] >= zmax || th[0] <= 1.0 || th[1] <= 0.01)) conv = 1;
                 nll = lm_eval(m, roi, th, zfocus, psf, dpsf, g, H);
                 lm_secant_update(S, H, sv, g, gold);
             } else if (pred <= lm_noise_floor * th[0]) {

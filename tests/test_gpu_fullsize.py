@@ -1,0 +1,232 @@
+"""Parity at the benchmark's full sizes, through properties that do not depend
+on the batch size (the FP64 oracle cannot fit a whole batch in test time):
+
+* isolated 21x21 ROIs, 64x64 pupil, 65,536 per launch (the headline workload):
+  - noise-free ROIs (n = mu exactly) return the parameters they were made from,
+    for every ROI of the batch (0.5 nm x/y, 1 nm z);
+  - each ROI's result does not depend on the batch around it: a 1,024-ROI
+    subset fitted alone is bit-identical to the same ROIs inside the full batch;
+  - a random sample of the batch matches the oracle's FP64 optimum (0.5 nm x/y,
+    1 nm z, 1e-5 relative NLL);
+  - on Poisson data the errors divided by the returned CRLB, over the emitters
+    of 1,000 photons or more, have a robust (MAD) width within 5% of 1; for
+    |z| < 0.6 um their mean is within 0.05 of 0 and their standard deviation
+    within 5% of 1 (CRLB and fit checked together; the MLE is near-efficient
+    there); fewer than 3% of the fits, mostly at |z| > 0.6 um, end in the
+    other lobe of the astigmatic z response, where their template start lay;
+* 41x41 ROIs on the 128x128 pupil with a per-ROI focus term, 65,536 per launch:
+  noise-free recovery for every ROI and a sample against the oracle;
+* 200,000 ROIs of mixed sides in one graph launch: every group equals its own
+  launch, bit for bit, and noise-free ROIs return their parameters.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import zola_hip as Z
+
+pytestmark = pytest.mark.gpu
+DX = 0.11
+ZMIN_T, ZSTEP_T, NZ = -1.0, 0.1, 21
+
+
+def _outs(torch, B):
+    f64 = lambda *s: torch.zeros(*s, dtype=torch.float64, device="cuda")
+    return dict(theta=f64(B, 5), nll=f64(B), crlb=f64(B, 3),
+                iters=torch.zeros(B, dtype=torch.int32, device="cuda"),
+                status=torch.zeros(B, dtype=torch.int32, device="cuda"))
+
+
+def _synth(torch, ctx, R, B, seed, noise, alo=500.0, ahi=5000.0):
+    """Isolated ROIs as the benchmark draws them: x, y U(+-1 px), z U[-1, 1] um,
+    photons log-uniform, background U[5, 50]; mu exactly when noise is False."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    u = lambda: torch.rand(B, device="cuda", generator=g, dtype=torch.float64)
+    x, y, z = (2 * u() - 1) * DX, (2 * u() - 1) * DX, 2 * u() - 1
+    A = torch.exp(np.log(alo) + (np.log(ahi) - np.log(alo)) * u())
+    bg = 5 + 45 * u()
+    par = torch.stack([x, y, z, torch.zeros_like(x)], 1).float().contiguous()
+    psf = torch.empty(B, R, R, device="cuda")
+    ctx.psf_forward_dev(par, psf)
+    mu = A.float()[:, None, None] * psf + bg.float()[:, None, None]
+    px = (torch.poisson(mu, generator=g) if noise else mu).float().contiguous()
+    # the parameters the pixels were made from, at the precision they entered
+    truth = torch.stack([A.float().double(), bg.float().double(), par[:, 0].double(), par[:, 1].double(),
+                         par[:, 2].double()], 1)
+    return px, truth
+
+
+def _fit(torch, ctx, cfg, px, init):
+    o = _outs(torch, px.shape[0])
+    ctx.fit_dev(cfg, px, init, o["theta"], o["nll"], o["crlb"], o["iters"], o["status"])
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in o.items()}
+
+
+@pytest.fixture(scope="module")
+def ctx21(torch_cuda):
+    c = Z.Context(Z.optics(), 21, Z.astigmatic_coeffs(28))
+    c.set_stream(torch_cuda.cuda.current_stream().cuda_stream)
+    yield c
+    c.close()
+
+
+def _init(torch, ctx, px):
+    init = torch.empty(px.shape[0], 5, dtype=torch.float64, device="cuda")
+    ctx.fit_init_dev(px, NZ, ZMIN_T, ZSTEP_T, 0.0, init)
+    return init
+
+
+def test_headline_batch_noise_free_recovers_every_roi(torch_cuda, ctx21):
+    torch = torch_cuda
+    B = 65536
+    print(f"case: {B} noise-free 21x21 ROIs", flush=True)
+    px, truth = _synth(torch, ctx21, 21, B, 0x51, noise=False)
+    cfg = ctx21.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    res = _fit(torch, ctx21, cfg, px, _init(torch, ctx21, px))
+    t = truth.cpu().numpy()
+    d = np.abs(res["theta"][:, 2:] - t[:, 2:]) * 1000
+    print("max |d| nm", d.max(axis=0), "converged", np.mean((res["status"] & Z.ST_CONVERGED) != 0))
+    assert np.all((res["status"] & Z.ST_CONVERGED) != 0)
+    assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
+    # A stops within 1% of its statistical precision (at least sqrt(A)) per step
+    assert np.all(np.abs(res["theta"][:, 0] - t[:, 0]) <= 0.05 * np.sqrt(t[:, 0]))
+
+
+def test_headline_batch_results_do_not_depend_on_the_batch(torch_cuda, ctx21):
+    torch = torch_cuda
+    B = 65536
+    print(f"case: subset of {B} fitted alone", flush=True)
+    px, _ = _synth(torch, ctx21, 21, B, 0x52, noise=True)
+    init = _init(torch, ctx21, px)
+    cfg = ctx21.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    full = _fit(torch, ctx21, cfg, px, init)
+    idx = torch.from_numpy(np.sort(np.random.default_rng(3).choice(B, 1024, replace=False))).cuda()
+    sub = _fit(torch, ctx21, cfg, px[idx].contiguous(), init[idx].contiguous())
+    ii = idx.cpu().numpy()
+    for k in ("theta", "nll", "crlb", "iters", "status"):
+        np.testing.assert_array_equal(sub[k], full[k][ii])
+
+
+def test_headline_batch_sample_matches_oracle_and_pulls_match_crlb(torch_cuda, ctx21):
+    torch = torch_cuda
+    B = 65536
+    print(f"case: {B} Poisson 21x21 ROIs, oracle sample and CRLB pulls", flush=True)
+    px, truth = _synth(torch, ctx21, 21, B, 0x53, noise=True)
+    init = _init(torch, ctx21, px)
+    cfg = ctx21.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    res = _fit(torch, ctx21, cfg, px, init)
+    t = truth.cpu().numpy()
+    ok = (res["status"] & Z.ST_CONVERGED) != 0
+    assert ok.mean() > 0.999
+    ok &= t[:, 0] >= 1000.0  # efficiency is asymptotic: keep emitters of 1,000 photons or more
+    pull = (res["theta"][ok, 2:] - t[ok, 2:]) / res["crlb"][ok]
+    zt = t[ok, 4]
+    # fits whose template start lay in the other lobe of the astigmatic z
+    # response converge to that local optimum (as the FP64 fit from the same
+    # start does): about 1% of the batch, mostly at |z| > 0.6 um
+    core = np.all(np.abs(pull) < 5, axis=1)
+    width = 1.4826 * np.median(np.abs(pull - np.median(pull, axis=0)), axis=0)
+    mid = core & (np.abs(zt) < 0.6)
+    print("pull robust width", width, "outside 5 sigma", 1 - core.mean(), "|z|<0.6 um mean", pull[mid].mean(axis=0),
+          "std", pull[mid].std(axis=0))
+    assert core.mean() > 0.97
+    assert np.all(np.abs(width - 1.0) < 0.05)
+    assert np.all(np.abs(pull[mid].mean(axis=0)) < 0.05)
+    assert np.all(np.abs(pull[mid].std(axis=0) - 1.0) < 0.05)
+    m = O.Model(O.optics(), 21, Z.astigmatic_coeffs(28))
+    sample = np.random.default_rng(4).choice(B, 16, replace=False)
+    hp, hi = px[sample].cpu().numpy(), init[sample].cpu().numpy()
+    for k, b in enumerate(sample):
+        r = m.lm_fit(hp[k].astype(np.float64), hi[k], zmin=-1.3, zmax=1.3, maxit=500, tol=1e-10)
+        d = (res["theta"][b, 2:] - r["theta"][2:]) * 1000
+        assert np.abs(d[:2]).max() <= 0.5 and abs(d[2]) <= 1.0, (b, d)
+        assert abs(res["nll"][b] - r["nll"]) <= 1e-5 * abs(r["nll"])
+
+
+def test_depth128_batch_noise_free_and_oracle_sample(torch_cuda):
+    torch = torch_cuda
+    B, R = 65536, 41
+    print(f"case: {B} 41x41 ROIs on the 128 pupil with per-ROI focus", flush=True)
+    opt = dict(size_fft=128, nwat=1.33)
+    coef = Z.tetrapod_coeffs(45)
+    ctx = Z.Context(Z.optics(**opt), R, coef)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(0x128)
+        u = lambda: torch.rand(B, device="cuda", generator=g, dtype=torch.float64)
+        x, y = (2 * u() - 1) * DX, (2 * u() - 1) * DX
+        z, zf = 6 * u() - 3, 4 * u()
+        A = torch.exp(np.log(1e3) + (np.log(5e4) - np.log(1e3)) * u())
+        bg = 10 + 90 * u()
+        par = torch.stack([x, y, z, zf], 1).float().contiguous()
+        psf = torch.empty(B, R, R, device="cuda")
+        ctx.psf_forward_dev(par, psf)
+        mu = (A.float()[:, None, None] * psf + bg.float()[:, None, None]).contiguous()
+        noisy = torch.poisson(mu, generator=g).contiguous()
+        del psf
+        init = torch.stack([A * 1.1, bg * 0.9, x + 0.03, y - 0.03, z + 0.1], 1).contiguous()
+        zoil = zf.float().contiguous()
+        cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-3.3, zmax=3.3, tol=1e-4)
+        o = _outs(torch, B)
+        ctx.fit_zoil_dev(cfg, mu, init, zoil, o["theta"], o["nll"], o["crlb"], o["iters"], o["status"])
+        torch.cuda.synchronize()
+        th, st = o["theta"].cpu().numpy(), o["status"].cpu().numpy()
+        t = torch.stack([x, y, z], 1).float().double().cpu().numpy()
+        d = np.abs(th[:, 2:] - t) * 1000
+        print("noise-free max |d| nm", d.max(axis=0))
+        assert np.all((st & Z.ST_CONVERGED) != 0)
+        assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
+        ctx.fit_zoil_dev(cfg, noisy, init, zoil, o["theta"], o["nll"], o["crlb"], o["iters"], o["status"])
+        torch.cuda.synchronize()
+        th, nl = o["theta"].cpu().numpy(), o["nll"].cpu().numpy()
+        m = O.Model(O.optics(**opt), R, coef)
+        sample = np.random.default_rng(5).choice(B, 6, replace=False)
+        hp, hi, hz = noisy[sample].cpu().numpy(), init[sample].cpu().numpy(), zoil[sample].cpu().numpy()
+        for k, b in enumerate(sample):
+            r = m.lm_fit(hp[k].astype(np.float64), hi[k], zfocus=float(hz[k]), zmin=-3.3, zmax=3.3, maxit=500,
+                         tol=1e-10)
+            dd = (th[b, 2:] - r["theta"][2:]) * 1000
+            assert np.abs(dd[:2]).max() <= 0.5 and abs(dd[2]) <= 1.0, (b, dd)
+            assert abs(nl[b] - r["nll"]) <= 1e-5 * abs(r["nll"])
+    finally:
+        ctx.close()
+
+
+def test_mixed_200k_groups_equal_their_own_launch_and_recover_noise_free(torch_cuda):
+    torch = torch_cuda
+    sizes, frac, total = (17, 21, 31, 41), (0.4, 0.3, 0.2, 0.1), 200000
+    print(f"case: {total} mixed ROIs in one graph launch", flush=True)
+    coef = Z.astigmatic_coeffs(28)
+    ctxs, groups, truths = [], [], []
+    try:
+        for k, (Rg, fr) in enumerate(zip(sizes, frac)):
+            Bg = int(round(total * fr))
+            c = Z.Context(Z.optics(), Rg, coef)
+            c.set_stream(torch.cuda.current_stream().cuda_stream)
+            ctxs.append(c)
+            # half of each group noise-free, half Poisson
+            pn, tn = _synth(torch, c, Rg, Bg // 2, 0x3000 + k, noise=False, alo=200.0, ahi=2e4)
+            pp, tp = _synth(torch, c, Rg, Bg - Bg // 2, 0x3100 + k, noise=True, alo=200.0, ahi=2e4)
+            px = torch.cat([pn, pp]).contiguous()
+            gr = dict(R=Rg, pixels=px, init=_init(torch, c, px), **_outs(torch, Bg))
+            groups.append(gr)
+            truths.append(tn.cpu().numpy())
+        ctx = ctxs[1]
+        cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+        ctx.fit_mixed_dev(cfg, groups)
+        torch.cuda.synchronize()
+        for gr, c, tn in zip(groups, ctxs, truths):
+            alone = _fit(torch, c, cfg, gr["pixels"], gr["init"])
+            for k in ("theta", "nll", "crlb", "iters", "status"):
+                np.testing.assert_array_equal(gr[k].cpu().numpy(), alone[k])
+            nf = tn.shape[0]
+            d = np.abs(alone["theta"][:nf, 2:] - tn[:, 2:]) * 1000
+            print("R", gr["R"], "noise-free max |d| nm", d.max(axis=0))
+            assert np.all((alone["status"][:nf] & Z.ST_CONVERGED) != 0)
+            assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
+    finally:
+        for c in ctxs:
+            c.close()

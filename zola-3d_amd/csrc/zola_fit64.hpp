@@ -330,6 +330,25 @@ struct Synth64q : QuadWaves {
     }
 };
 
+// Trial point of an LM step d from th, clamped as the fit clamps every trial:
+// x, y by at most one pixel, z by at most 0.2 um, A >= 1, B >= 0.01,
+// zmin <= z <= zmax.
+__device__ __forceinline__ void lm_clamp_trial(const double* th, const double (&d)[5], double dxmax, double zmin,
+                                               double zmax, double (&tn)[5]) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) tn[p] = th[p] + d[p];
+#pragma unroll
+    for (int p = 2; p < 4; p++) {
+        if (d[p] > dxmax) tn[p] = th[p] + dxmax;
+        if (d[p] < -dxmax) tn[p] = th[p] - dxmax;
+    }
+    if (d[4] > 0.2) tn[4] = th[4] + 0.2;
+    if (d[4] < -0.2) tn[4] = th[4] - 0.2;
+    tn[0] = fmax(tn[0], 1.0);
+    tn[1] = fmax(tn[1], 0.01);
+    tn[4] = fmin(fmax(tn[4], zmin), zmax);
+}
+
 struct LmState {
     double H[15];   // Fisher matrix, packed upper triangle
     double S[15];   // secant estimate of the rest of the Hessian, packed upper
@@ -339,6 +358,13 @@ struct LmState {
     double tn[5];   // trial point
     double nll, lambda, nu, pred;
 };
+
+// The trial tc from L.th is below tol in x, y, z and below 1% of the
+// statistical precision (1/sqrt(F_ii)) in A and B.
+__device__ __forceinline__ bool lm_small_step(const LmState& L, const double (&tc)[5], double tol) {
+    return fmax(fabs(tc[2] - L.th[2]), fmax(fabs(tc[3] - L.th[3]), fabs(tc[4] - L.th[4]))) < tol &&
+           fabs(tc[0] - L.th[0]) * sqrt(L.H[0]) < 0.01 && fabs(tc[1] - L.th[1]) * sqrt(L.H[5]) < 0.01;
+}
 
 __device__ __forceinline__ constexpr int hidx(int p, int q) {
     // packed upper triangle, row-major
@@ -505,13 +531,46 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     eval_now = false;
                 } else {
                     chol5_solve(Lc, d);
-                    if (tries == 0 && fmax(fabs(d[2]), fmax(fabs(d[3]), fabs(d[4]))) < A.tol &&
-                        fabs(d[0]) * sqrt(hget(L.H, 0, 0)) < 0.01 && fabs(d[1]) * sqrt(hget(L.H, 1, 1)) < 0.01) {
-                        // the step from the point just accepted is below tol in x,
-                        // y, z and below 1% of the statistical precision in A and
-                        // B: converged there, without evaluating the step
-                        pre_conv = true;
-                        eval_now = false;
+                    if (tries == 0) {
+                        // Converged at the point just accepted when the step the fit
+                        // would take from it, almost undamped (lambda at most 1e-4,
+                        // raised tenfold while the system is not positive definite)
+                        // and clamped as trials are, is below tol in x, y, z and below
+                        // 1% of the statistical precision in A and B.  A damped step
+                        // would understate the distance along weakly determined
+                        // directions (lambda diag(F) dominates their small eigenvalue);
+                        // the almost undamped step is solved only when the damped one
+                        // already passes, since it is the longer of the two.
+                        double tc[5];
+                        lm_clamp_trial(L.th, d, dxmax, A.zmin, A.zmax, tc);
+                        bool small = lm_small_step(L, tc, A.tol);
+                        if (small && L.lambda > 1e-4) {
+                            double dc[5], lc = 1e-4;
+                            bool have = false;
+                            for (int k = 0; k < 8 && !have && lc < L.lambda; k++) {
+    #pragma unroll
+                                for (int p = 0; p < 5; p++) {
+    #pragma unroll
+                                    for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q) + hget(L.S, p, q);
+                                    Am[p * 5 + p] += lc * hget(L.H, p, p);
+                                    dc[p] = -L.gr[p];
+                                }
+                                if (chol5(Am, Lc)) {
+                                    chol5_solve(Lc, dc);
+                                    have = true;
+                                } else {
+                                    lc *= 10.0;
+                                }
+                            }
+                            if (have) {
+                                lm_clamp_trial(L.th, dc, dxmax, A.zmin, A.zmax, tc);
+                                small = lm_small_step(L, tc, A.tol);
+                            }
+                        }
+                        if (small) {
+                            pre_conv = true;
+                            eval_now = false;
+                        }
                     }
                 }
                 if (eval_now) {
@@ -525,18 +584,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     }
                     L.pred = pred;
                     double tn[5];
-    #pragma unroll
-                    for (int p = 0; p < 5; p++) tn[p] = L.th[p] + d[p];
-    #pragma unroll
-                    for (int p = 2; p < 4; p++) {
-                        if (d[p] > dxmax) tn[p] = L.th[p] + dxmax;
-                        if (d[p] < -dxmax) tn[p] = L.th[p] - dxmax;
-                    }
-                    if (d[4] > 0.2) tn[4] = L.th[4] + 0.2;
-                    if (d[4] < -0.2) tn[4] = L.th[4] - 0.2;
-                    tn[0] = fmax(tn[0], 1.0);
-                    tn[1] = fmax(tn[1], 0.01);
-                    tn[4] = fmin(fmax(tn[4], A.zmin), A.zmax);
+                    lm_clamp_trial(L.th, d, dxmax, A.zmin, A.zmax, tn);
     #pragma unroll
                     for (int p = 0; p < 5; p++) L.tn[p] = tn[p];
                 }
@@ -575,7 +623,11 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                         const double t3 = 2.0 * rho - 1.0;
                         L.lambda *= fmax(1.0 / 3.0, 1.0 - t3 * t3 * t3);
                         L.nu = 2.0;
-                        if (step < A.tol) conv = 1;
+                        // at a bound the clamped steps shrink to zero while the
+                        // unclamped step does not: a step below tol ends the fit there
+                        const bool at_bound = L.tn[4] <= A.zmin || L.tn[4] >= A.zmax || L.tn[0] <= 1.0 ||
+                                              L.tn[1] <= 0.01;
+                        if (step < A.tol && at_bound) conv = 1;
                     }
                     L.nll = nn;
                     // d psf = (sgn dv - psf * sum(sgn dv)) / S for the three tangents
