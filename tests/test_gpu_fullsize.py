@@ -17,7 +17,10 @@ on the batch size (the FP64 oracle cannot fit a whole batch in test time):
 * 41x41 ROIs on the 128x128 pupil with a per-ROI focus term, 65,536 per launch:
   noise-free recovery for every ROI and a sample against the oracle;
 * 200,000 ROIs of mixed sides in one graph launch: every group equals its own
-  launch, bit for bit, and noise-free ROIs return their parameters.
+  launch, bit for bit, and noise-free ROIs return their parameters;
+* 2,000 frames of 256x256 through detection and the acceptance walk: a few
+  frames processed alone give the same candidate lists and records, bit for bit,
+  as inside the whole run.
 """
 import numpy as np
 import pytest
@@ -230,3 +233,76 @@ def test_mixed_200k_groups_equal_their_own_launch_and_recover_noise_free(torch_c
     finally:
         for c in ctxs:
             c.close()
+
+
+def _frames_2000(torch, ctx, F, W=256, H=256):
+    """The frame workload of the benchmark: emitters Poisson at 0.5 per um^2 per
+    frame, tetrapod PSF on the 128 pupil, z U[-3, 3] um, photons log-uniform
+    1e3..2e4, background U[20, 100] per frame, Poisson noise."""
+    Rn, h = 41, 20
+    rng = np.random.default_rng(0xF1)
+    counts = rng.poisson(0.5 * (W * DX) * (H * DX), F)
+    n = int(counts.sum())
+    fidx = np.repeat(np.arange(F), counts)
+    pi, pj = rng.uniform(0, W, n), rng.uniform(0, H, n)
+    ci, cj = np.rint(pi).astype(np.int64), np.rint(pj).astype(np.int64)
+    z = rng.uniform(-3, 3, n)
+    A = np.exp(rng.uniform(np.log(1e3), np.log(2e4), n))
+    par = np.stack([(ci - pi) * DX, (cj - pj) * DX, z, np.zeros(n)], 1).astype(np.float32)
+    Wp, Hp = W + 2 * Rn, H + 2 * Rn
+    frames = torch.zeros(F * Wp * Hp, device="cuda")
+    a = torch.arange(Rn, device="cuda")
+    off = (a[:, None] * Hp + a[None, :]).reshape(-1)
+    base_all = fidx * Wp * Hp + (ci - h + Rn) * Hp + (cj - h + Rn)
+    chunk = 65536
+    psf = torch.empty(chunk, Rn, Rn, device="cuda")
+    for e0 in range(0, n, chunk):
+        e1 = min(n, e0 + chunk)
+        ctx.psf_forward_dev(torch.from_numpy(par[e0:e1]).cuda(), psf[:e1 - e0])
+        base = torch.from_numpy(base_all[e0:e1]).cuda()
+        frames.index_add_(0, (base[:, None] + off[None, :]).reshape(-1),
+                          (torch.from_numpy(A[e0:e1]).float().cuda()[:, None, None] * psf[:e1 - e0]).reshape(-1))
+    frames = frames.view(F, Wp, Hp)[:, Rn:Rn + W, Rn:Rn + H]
+    bg = torch.from_numpy(rng.uniform(20, 100, F)).float().cuda()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0xF1)
+    return torch.poisson(frames + bg[:, None, None], generator=g).cpu().numpy().astype(np.float32)
+
+
+def test_frames_2000_records_do_not_depend_on_the_batch(torch_cuda):
+    """2,000 frames of 256x256 (the benchmark's frame workload) through detection
+    and the acceptance walk: the candidate lists and the records of a few frames
+    processed alone equal, bit for bit, those frames' lists and records inside
+    the 2,000-frame run; the records obey the acceptance rules."""
+    torch = torch_cuda
+    F = 2000
+    print(f"case: {F} frames 256x256, detection and LM walk", flush=True)
+    ctx = Z.Context(Z.optics(size_fft=128), 41, Z.tetrapod_coeffs(45))
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    det = None
+    try:
+        host = _frames_2000(torch, ctx, F)
+        det = Z.Detector(ctx, 256, 256, -3.0, 3.05, 0.1, 0.1, 0.0)
+        acc = det.accept_config(photon_threshold=500.0, min_z=-3.0, max_z=3.0)
+        fit = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-3.3, zmax=3.3, tol=1e-4, photon_threshold=0.0)
+        lists, cnt = det.detect(host, cap=8192)
+        locs, per, rounds = det.localize(host, fit, acc)
+        print("records", len(locs), "per frame", per.mean(), "rounds", rounds, flush=True)
+        assert per.sum() == len(locs) and np.all(np.diff(locs[:, 0]) >= 0)
+        assert np.all(locs[:, 4] > 500.0) and np.all(np.abs(locs[:, 3]) < 3000.0)
+        assert np.all((locs[:, 1] > 0) & (locs[:, 1] < 256 * DX * 1000) & (locs[:, 2] > 0) &
+                      (locs[:, 2] < 256 * DX * 1000))
+        for a, b in ((0, 8), (996, 1004), (1992, 2000)):
+            sub_lists, sub_cnt = det.detect(host[a:b], cap=8192)
+            np.testing.assert_array_equal(sub_cnt, cnt[a:b])
+            for f in range(b - a):
+                np.testing.assert_array_equal(sub_lists[f], lists[a + f])
+            sub, sub_per, _ = det.localize(host[a:b], fit, acc)
+            ref = locs[(locs[:, 0] >= a) & (locs[:, 0] < b)].copy()
+            ref[:, 0] -= a
+            np.testing.assert_array_equal(sub_per, per[a:b])
+            np.testing.assert_array_equal(sub, ref)
+    finally:
+        if det is not None:
+            det.close()
+        ctx.close()
