@@ -69,7 +69,8 @@ int  zola_ctx_info(const zola_ctx* ctx, int* info);
 int  zola_set_coefficients(zola_ctx* ctx, const double* coeffs, int J);
 /* Replaces PSFmany_float_.updateSigmaGaussianKernel. */
 int  zola_set_sigma(zola_ctx* ctx, double sigma);
-/* Runs subsequent work on this HIP stream (hipStream_t passed as void*). */
+/* Runs subsequent work on this HIP stream (hipStream_t passed as void*).  Waits
+ * for the context's work on the previous stream first. */
 int  zola_set_stream(zola_ctx* ctx, void* hip_stream);
 
 /* Batched PSF synthesis.  params: B x 4 floats [x, y, z, zoil] (um; z is the

@@ -331,6 +331,9 @@ int zola_set_sigma(zola_ctx* c, double sigma) {
 
 int zola_set_stream(zola_ctx* c, void* s) {
     if (!c) return ZOLA_EINVAL;
+    // the context's buffers (templates, staging) may still be in use by work on
+    // the previous stream: finish it before work on the new stream can reuse them
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (c->own_stream && c->stream) HIP_TRY(c, hipStreamDestroy(c->stream));
     c->own_stream = false;
     c->stream = (hipStream_t)s;
