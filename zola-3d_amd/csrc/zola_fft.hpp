@@ -150,6 +150,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// wave_sum of K values at once, butterfly step by butterfly step: every lane
+// performs exactly the additions of K separate wave_sum calls (same results),
+// but the K cross-lane exchanges of a step are in flight together instead of
+// each waiting out the exchange latency alone.
+template <int K, class T>
+__device__ __forceinline__ void wave_sum_v(T (&v)[K]) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        T t[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) t[k] = __shfl_xor(v[k], o, 64);
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] += t[k];
+    }
+}
+
 // Orders LDS traffic of one wave around a cross-lane exchange: all lanes'
 // LDS writes before the fence are complete before any read after it.  Only
 // the issuing wave is involved (waves of a block run independent ROIs).

@@ -208,13 +208,11 @@ struct Synth64 {
     __device__ __forceinline__ static double sum(double v, Red&, int, int) { return wave_sum(v); }
     template <int K>
     __device__ __forceinline__ static void sumv(double (&v)[K], Red&, int, int) {
-#pragma unroll
-        for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+        wave_sum_v<K>(v);
     }
     template <int K>
     __device__ __forceinline__ static void sumv(float (&v)[K], Red&, int, int) {
-#pragma unroll
-        for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+        wave_sum_v<K>(v);
     }
     static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
     static constexpr int kRefFloats = (int)(sizeof(WaveScratch64) / sizeof(float));
@@ -254,11 +252,10 @@ struct QuadWaves {
     template <int K>
     __device__ __forceinline__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
         static_assert(K <= 24, "reduction width");
+        wave_sum_v<K>(v);
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            v[k] = wave_sum(v[k]);
+        for (int k = 0; k < K; k++)
             if (lane == 0) r.d[wr][k] = v[k];
-        }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = ((r.d[0][k] + r.d[1][k]) + r.d[2][k]) + r.d[3][k];
@@ -267,11 +264,10 @@ struct QuadWaves {
     template <int K>
     __device__ __forceinline__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
         static_assert(K <= 4, "reduction width");
+        wave_sum_v<K>(v);
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            v[k] = wave_sum(v[k]);
+        for (int k = 0; k < K; k++)
             if (lane == 0) r.f[wr][k] = v[k];
-        }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] = ((r.f[0][k] + r.f[1][k]) + r.f[2][k]) + r.f[3][k];
