@@ -1055,12 +1055,14 @@ int zo_lm_fit(zo_model* m, const double* roi, double* theta, double zfocus, doub
                     break;
                 }
             }
-            double pred = 0.0;
+            /* predicted decrease -(g'd + d'(F + S)d / 2); with
+             * (F + S + lambda diag F) d = -g this is -g'd / 2 + lambda sum F_pp d_p^2 / 2 */
+            double gd = 0.0, dd = 0.0;
             for (int p = 0; p < 5; p++) {
-                double hd = 0.0;
-                for (int q = 0; q < 5; q++) hd += (H[p * 5 + q] + S[p * 5 + q]) * d[q];
-                pred -= g[p] * d[p] + 0.5 * d[p] * hd;
+                gd += g[p] * d[p];
+                dd += H[p * 5 + p] * d[p] * d[p];
             }
+            double pred = -0.5 * gd + 0.5 * lambda * dd;
             lm_clamp_trial(th, d, dx, zmin, zmax, tn);
             double nn = lm_nll(m, roi, tn, zfocus, psf);
             if (nn <= nll) {

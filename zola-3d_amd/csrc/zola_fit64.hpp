@@ -570,14 +570,16 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     }
                 }
                 if (eval_now) {
-                    double pred = 0.0;
+                    // predicted decrease -(g'd + d'(F + S)d / 2); since
+                    // (F + S + lambda diag F) d = -g this is
+                    // -g'd / 2 + lambda sum F_pp d_p^2 / 2 (no second pass over F, S)
+                    double gd = 0.0, dd = 0.0;
     #pragma unroll
                     for (int p = 0; p < 5; p++) {
-                        double hd = 0.0;
-    #pragma unroll
-                        for (int q = 0; q < 5; q++) hd += (hget(L.H, p, q) + hget(L.S, p, q)) * d[q];
-                        pred -= L.gr[p] * d[p] + 0.5 * d[p] * hd;
+                        gd += L.gr[p] * d[p];
+                        dd += hget(L.H, p, p) * d[p] * d[p];
                     }
+                    const double pred = -0.5 * gd + 0.5 * L.lambda * dd;
                     L.pred = pred;
                     double tn[5];
                     lm_clamp_trial(L.th, d, dxmax, A.zmin, A.zmax, tn);
