@@ -378,11 +378,23 @@ __device__ __forceinline__ double hget(const double (&H)[15], int p, int q) { re
 // L.gp, the step from L.tn - L.th (all in LDS: no register array is indexed
 // at run time).
 __device__ __forceinline__ void lm_secant_update(LmState& L) {
+    // register copies of the LDS state: its loads issue together
+    double H[15], S[15], dth[5], dg[5];
+#pragma unroll
+    for (int k = 0; k < 15; k++) {
+        H[k] = L.H[k];
+        S[k] = L.S[k];
+    }
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+        dth[p] = L.tn[p] - L.th[p];
+        dg[p] = L.gr[p] - L.gp[p];
+    }
     double D[5], iD[5], sh[5], yh[5];
     bool ok = true;
 #pragma unroll
     for (int p = 0; p < 5; p++) {
-        const double f = L.H[hidx(p, p)];
+        const double f = H[hidx(p, p)];
         ok = ok && (f > 0.0);
         D[p] = sqrt(fmax(f, 1e-300));
         iD[p] = 1.0 / D[p];
@@ -390,19 +402,18 @@ __device__ __forceinline__ void lm_secant_update(LmState& L) {
     if (!ok) return;
 #pragma unroll
     for (int p = 0; p < 5; p++) {
-        const double sp = L.tn[p] - L.th[p];
         double fs = 0.0;
 #pragma unroll
-        for (int q = 0; q < 5; q++) fs += L.H[hidx(p, q)] * (L.tn[q] - L.th[q]);
-        sh[p] = D[p] * sp;
-        yh[p] = (L.gr[p] - L.gp[p] - fs) * iD[p];
+        for (int q = 0; q < 5; q++) fs += H[hidx(p, q)] * dth[q];
+        sh[p] = D[p] * dth[p];
+        yh[p] = (dg[p] - fs) * iD[p];
     }
     double Ssh[5], sSs = 0.0, sy = 0.0, ss = 0.0;
 #pragma unroll
     for (int p = 0; p < 5; p++) {
         double t = 0.0;
 #pragma unroll
-        for (int q = 0; q < 5; q++) t += L.S[hidx(p, q)] * iD[p] * iD[q] * sh[q];
+        for (int q = 0; q < 5; q++) t += S[hidx(p, q)] * iD[p] * iD[q] * sh[q];
         Ssh[p] = t;
         sSs += sh[p] * t;
         sy += sh[p] * yh[p];
@@ -421,7 +432,7 @@ __device__ __forceinline__ void lm_secant_update(LmState& L) {
     for (int p = 0; p < 5; p++)
 #pragma unroll
         for (int q = p; q < 5; q++) {
-            const double v = tau * L.S[hidx(p, q)] * iD[p] * iD[q] + (r[p] * sh[q] + sh[p] * r[q]) * iss -
+            const double v = tau * S[hidx(p, q)] * iD[p] * iD[q] + (r[p] * sh[q] + sh[p] * r[q]) * iss -
                              rs * sh[p] * sh[q] * iss * iss;
             L.S[hidx(p, q)] = v * D[p] * D[q];
         }
