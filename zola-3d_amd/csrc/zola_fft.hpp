@@ -150,6 +150,47 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// One DPP lane permutation within each row of 16 lanes (CTRL: a dpp_ctrl
+// code), for a 32-bit or 64-bit value (the halves permuted alike).
+template <int CTRL>
+__device__ __forceinline__ float dpp_perm(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_perm(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sums of K values over the wave, identical in every lane.  Within each row of
+// 16 lanes by DPP permutations (no LDS): partner lane ^1 and ^2 (quad_perm
+// [1,0,3,2], [2,3,0,1]), then the mirrored lane in the half row and in the row
+// (each pairs the two partial sums of the level below); across the four rows
+// by cross-lane exchanges with lane ^16 and ^32.  Every step adds a value to
+// its partner's, so both partners hold the same bits.  The K exchanges of a
+// step are in flight together.
+template <int K, class T>
+__device__ __forceinline__ void wave_sum_rows(T (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += dpp_perm<0xB1>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += dpp_perm<0x4E>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += dpp_perm<0x141>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += dpp_perm<0x140>(v[k]);
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+        T t[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) t[k] = __shfl_xor(v[k], o, 64);
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] += t[k];
+    }
+}
+
 // wave_sum of K values at once, butterfly step by butterfly step: every lane
 // performs exactly the additions of K separate wave_sum calls (same results),
 // but the K cross-lane exchanges of a step are in flight together instead of

@@ -204,15 +204,23 @@ struct Synth64 {
     template <int R>
     __device__ __forceinline__ static int pix(int lane, int, int t) { return ((lane / R) * RPL<R> + t) * R + lane % R; }
     struct Red {};
-    __device__ __forceinline__ static float sum(float v, Red&, int, int) { return wave_sum(v); }
-    __device__ __forceinline__ static double sum(double v, Red&, int, int) { return wave_sum(v); }
+    __device__ __forceinline__ static float sum(float v, Red&, int, int) {
+        float a[1] = {v};
+        wave_sum_rows<1>(a);
+        return a[0];
+    }
+    __device__ __forceinline__ static double sum(double v, Red&, int, int) {
+        double a[1] = {v};
+        wave_sum_rows<1>(a);
+        return a[0];
+    }
     template <int K>
     __device__ __forceinline__ static void sumv(double (&v)[K], Red&, int, int) {
-        wave_sum_v<K>(v);
+        wave_sum_rows<K>(v);
     }
     template <int K>
     __device__ __forceinline__ static void sumv(float (&v)[K], Red&, int, int) {
-        wave_sum_v<K>(v);
+        wave_sum_rows<K>(v);
     }
     static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
     static constexpr int kRefFloats = (int)(sizeof(WaveScratch64) / sizeof(float));
