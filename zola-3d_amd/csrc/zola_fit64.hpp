@@ -260,7 +260,7 @@ struct QuadWaves {
     template <int K>
     __device__ __forceinline__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
         static_assert(K <= 24, "reduction width");
-        wave_sum_v<K>(v);
+        wave_sum_rows<K>(v);
 #pragma unroll
         for (int k = 0; k < K; k++)
             if (lane == 0) r.d[wr][k] = v[k];
@@ -272,7 +272,7 @@ struct QuadWaves {
     template <int K>
     __device__ __forceinline__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
         static_assert(K <= 4, "reduction width");
-        wave_sum_v<K>(v);
+        wave_sum_rows<K>(v);
 #pragma unroll
         for (int k = 0; k < K; k++)
             if (lane == 0) r.f[wr][k] = v[k];
