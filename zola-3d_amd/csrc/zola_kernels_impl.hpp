@@ -121,8 +121,12 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
         sn += n[t];
         snn += (double)n[t] * n[t];
     }
-    sn = wave_sum(sn);
-    snn = wave_sum(snn);
+    {
+        double s2[2] = {sn, snn};
+        wave_sum_rows<2>(s2);
+        sn = s2[0];
+        snn = s2[1];
+    }
     const double P = (double)R * R;
     const double vn = snn - sn * sn / P;
     auto corr = [&](int r) -> double {
@@ -133,7 +137,11 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
             const double tv = tmpl[(size_t)r * R * R + (g * RPL + t) * R + j];
             stn += tv * n[t];
         }
-        stn = wave_sum(stn);
+        {
+            double a1[1] = {stn};
+            wave_sum_rows<1>(a1);
+            stn = a1[0];
+        }
         const double st = tstat[2 * r], vt = tstat[2 * r + 1];
         return (stn - st * sn / P) / sqrt(vt * vn);
     };
@@ -166,7 +174,14 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
         const float tv = tmpl[(size_t)best * R * R + (g * RPL + t) * R + j];
         if (tv < thr) { sb += n[t]; cb += 1; } else { sa += n[t]; ca += 1; }
     }
-    sb = wave_sum(sb); cb = wave_sum(cb); sa = wave_sum(sa); ca = wave_sum(ca);
+    {
+        double ab[4] = {sb, cb, sa, ca};
+        wave_sum_rows<4>(ab);
+        sb = ab[0];
+        cb = ab[1];
+        sa = ab[2];
+        ca = ab[3];
+    }
     const double Bv = cb != 0 ? sb / cb : 0.0;
     const double Av = sa - ca * Bv;
     if (lane == 0) {
