@@ -20,7 +20,9 @@ on the batch size (the FP64 oracle cannot fit a whole batch in test time):
   launch, bit for bit, and noise-free ROIs return their parameters;
 * 2,000 frames of 256x256 through detection and the acceptance walk: a few
   frames processed alone give the same candidate lists and records, bit for bit,
-  as inside the whole run.
+  as inside the whole run;
+* the calibration bead stack of 101 slices of 512x512 against the oracle
+  directly (it finishes in about a second there).
 """
 import numpy as np
 import pytest
@@ -306,3 +308,43 @@ def test_frames_2000_records_do_not_depend_on_the_batch(torch_cuda):
         if det is not None:
             det.close()
         ctx.close()
+
+
+def test_bead_preparation_full_stack_matches_oracle(torch_cuda):
+    """The benchmark's bead stack (101 slices of 512x512, 12 beads, 32 px
+    patches, 20 nm steps) against the FP64 oracle: refined seeds, centre slice
+    and background pixel count identical, in-frame patch pixels bit-identical,
+    background, B and A to 1e-9 relative."""
+    print("case: bead preparation 101 x 512 x 512", flush=True)
+    S, W, H, P, nb, zstep = 101, 512, 512, 32, 12, 0.02
+    rng = np.random.default_rng(0x20A1D3 + 5)
+    lam = np.full((S, W, H), 30.0)
+    beads = rng.uniform(40, 470, (nb, 2))
+    u = np.arange(-16, 17)
+    for z in range(S):
+        s = 1.2 + 0.02 * abs(z - S // 2)
+        for bx, by in beads:
+            ix, iy = int(bx) + u, int(by) + u
+            lam[z][np.ix_(ix, iy)] += 3e4 / (2 * np.pi * s * s) * np.exp(
+                -((ix[:, None] - bx) ** 2 + (iy[None, :] - by) ** 2) / (2 * s * s))
+    adu = (rng.poisson(lam) + 100).astype(np.float32)
+    del lam
+    xp, yp = beads[:, 0] + 0.7, beads[:, 1] - 0.6
+    ctx = Z.Context(Z.optics(), 21, Z.astigmatic_coeffs(28))
+    try:
+        got = ctx.bead_prepare(adu, xp, yp, P, zstep, offset=100.0)
+    finally:
+        ctx.close()
+    ref = O.bead_prepare(O.bead_convert(adu, 1.0, 1.0, 100.0), xp, yp, P, zstep)
+    assert ref["rc"] == 0
+    np.testing.assert_array_equal(got["x"], ref["x"])
+    np.testing.assert_array_equal(got["y"], ref["y"])
+    d = got["diag"]
+    assert (d["size_filt"], d["miniposition"], d["ndata"]) == (ref["size_filt"], ref["miniposition"], ref["ndata"])
+    np.testing.assert_allclose(got["background"], ref["poly"], rtol=1e-9)
+    np.testing.assert_allclose(got["B"], ref["B"], rtol=1e-9)
+    np.testing.assert_allclose(got["A"], ref["A"], rtol=1e-9, atol=1e-9 * np.abs(ref["A"]).max())
+    for p in range(nb):
+        x0, y0 = int(ref["x"][p]) - P // 2, int(ref["y"][p]) - P // 2
+        np.testing.assert_array_equal(got["patches"][p], ref["patches"][p])  # beads lie inside the frame
+        assert 0 <= x0 and x0 + P <= W and 0 <= y0 and y0 + P <= H
