@@ -109,15 +109,20 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
                                                          double zmin_t, double zstep_t,
                                                          double* __restrict__ init) {
     constexpr int RPL = RoiLayout<R>::RPL;
+    // templates are staged in LDS in chunks that the block's WPB ROIs share
+    constexpr int kChunk = 8192;  // floats per chunk (32 KB)
+    constexpr int NC = kChunk / (R * R) > 0 ? kChunk / (R * R) : 1;
+    __shared__ float ts[NC * R * R];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = blockIdx.x * WPB + w;
-    if (b >= B) return;
+    // waves past the batch end stay for the block barriers of the staging
+    const bool active = b < B;
     const int j = lane % R, g = lane / R;
     float n[RPL];
     double sn = 0, snn = 0;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
-        n[t] = Px<R>::valid(lane, t) ? pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
+        n[t] = active && Px<R>::valid(lane, t) ? pixels[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
         sn += n[t];
         snn += (double)n[t] * n[t];
     }
@@ -147,10 +152,33 @@ __global__ __launch_bounds__(64 * WPB) void k_fit_init64(int B, const float* __r
     };
     int best = 0;
     double cbest = -1e300;
-    for (int r = 0; r < nZ; r++) {
-        const double c = corr(r);
-        if (c > cbest) { cbest = c; best = r; }
+    for (int r0 = 0; r0 < nZ; r0 += NC) {
+        const int nc = nZ - r0 < NC ? nZ - r0 : NC;
+        __syncthreads();  // the previous chunk is no longer read
+        for (int k = threadIdx.x; k < nc * R * R; k += 64 * WPB) ts[k] = tmpl[(size_t)r0 * R * R + k];
+        __syncthreads();
+        if (!active) continue;
+        for (int rc = 0; rc < nc; rc++) {
+            // corr(r0 + rc) with the template read from LDS
+            double stn = 0;
+#pragma unroll
+            for (int t = 0; t < RPL; t++) {
+                if (!Px<R>::valid(lane, t)) continue;
+                const double tv = ts[rc * R * R + (g * RPL + t) * R + j];
+                stn += tv * n[t];
+            }
+            {
+                double a1[1] = {stn};
+                wave_sum_rows<1>(a1);
+                stn = a1[0];
+            }
+            const int r = r0 + rc;
+            const double st = tstat[2 * r], vt = tstat[2 * r + 1];
+            const double c = (stn - st * sn / P) / sqrt(vt * vn);
+            if (c > cbest) { cbest = c; best = r; }
+        }
     }
+    if (!active) return;
     double z0 = zmin_t + best * zstep_t;
     if (best > 1 && best < nZ - 1) {
         const double ya = corr(best - 1), yb = cbest, yc = corr(best + 1);
@@ -302,7 +330,7 @@ hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t
 template <int R>
 hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, const double* tstat,
                          double zmin_t, double zstep_t, double* init, hipStream_t st) {
-    constexpr int W = 4;
+    constexpr int W = 16;
     dim3 grid((B + W - 1) / W), block(64 * W);
     hipLaunchKernelGGL((k_fit_init64<R, W>), grid, block, 0, st, B, pixels, nZ, tmpl, tstat, zmin_t,
                        zstep_t, init);
