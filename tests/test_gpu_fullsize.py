@@ -14,6 +14,8 @@ on the batch size (the FP64 oracle cannot fit a whole batch in test time):
     within 5% of 1 (CRLB and fit checked together; the MLE is near-efficient
     there); fewer than 3% of the fits, mostly at |z| > 0.6 um, end in the
     other lobe of the astigmatic z response, where their template start lay;
+* the same batch with a per-pixel sCMOS map: noise-free recovery for every ROI
+  and batch independence;
 * 41x41 ROIs on the 128x128 pupil with a per-ROI focus term, 65,536 per launch:
   noise-free recovery for every ROI and a sample against the oracle;
 * 200,000 ROIs of mixed sides in one graph launch: every group equals its own
@@ -348,3 +350,37 @@ def test_bead_preparation_full_stack_matches_oracle(torch_cuda):
         x0, y0 = int(ref["x"][p]) - P // 2, int(ref["y"][p]) - P // 2
         np.testing.assert_array_equal(got["patches"][p], ref["patches"][p])  # beads lie inside the frame
         assert 0 <= x0 and x0 + P <= W and 0 <= y0 and y0 + P <= H
+
+
+def test_headline_batch_with_scmos_map_noise_free(torch_cuda, ctx21):
+    """65,536 noise-free 21x21 ROIs with a per-pixel sCMOS map v (mu = A psf +
+    B + v): every ROI returns the parameters it was made from (0.5 nm x/y,
+    1 nm z), and a subset fitted alone is bit-identical to the full batch."""
+    torch = torch_cuda
+    B = 65536
+    print(f"case: {B} noise-free 21x21 ROIs with an sCMOS map", flush=True)
+    px, truth = _synth(torch, ctx21, 21, B, 0x54, noise=False)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x55)
+    varg = (0.5 + 11.5 * torch.rand(B, 21, 21, device="cuda", generator=g)).contiguous()
+    px = (px + varg).contiguous()
+    init = _init(torch, ctx21, (px - varg).contiguous())
+    cfg = ctx21.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+
+    def fit(p, i, v):
+        o = _outs(torch, p.shape[0])
+        ctx21.fit_ext_dev(cfg, p, i, o["theta"], o["nll"], o["crlb"], o["iters"], o["status"], varg=v)
+        torch.cuda.synchronize()
+        return {k: x.cpu().numpy() for k, x in o.items()}
+
+    res = fit(px, init, varg)
+    t = truth.cpu().numpy()
+    d = np.abs(res["theta"][:, 2:] - t[:, 2:]) * 1000
+    print("max |d| nm", d.max(axis=0))
+    assert np.all((res["status"] & Z.ST_CONVERGED) != 0)
+    assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
+    idx = torch.from_numpy(np.sort(np.random.default_rng(6).choice(B, 512, replace=False))).cuda()
+    sub = fit(px[idx].contiguous(), init[idx].contiguous(), varg[idx].contiguous())
+    ii = idx.cpu().numpy()
+    for k in ("theta", "nll", "crlb", "iters", "status"):
+        np.testing.assert_array_equal(sub[k], res[k][ii])
