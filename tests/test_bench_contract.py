@@ -1,0 +1,16 @@
+"""bench.py's command-line contract on the CPU: more than one GPU is refused
+with a message and exit status 2 before any GPU work (one GPU's shard is
+what this build measures)."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "--gpus 2" in r.stderr and "one GPU" in r.stderr
+    assert r.stdout == ""
