@@ -164,13 +164,37 @@ __device__ __forceinline__ double dpp_perm(double v) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sums of K values over the wave, identical in every lane.  Within each row of
-// 16 lanes by DPP permutations (no LDS): partner lane ^1 and ^2 (quad_perm
-// [1,0,3,2], [2,3,0,1]), then the mirrored lane in the half row and in the row
-// (each pairs the two partial sums of the level below); across the four rows
-// by cross-lane exchanges with lane ^16 and ^32.  Every step adds a value to
-// its partner's, so both partners hold the same bits.  The K exchanges of a
-// step are in flight together.
+// Sum over rows 2k and 2k+1 (permlane16_swap) or over the two halves of the
+// wave (permlane32_swap, ROW32 true), in every lane: with both operands equal
+// to v, the swap returns the lower row's (half's) value in .x and the upper
+// one's in .y at each lane, so every lane adds the same two numbers.
+template <bool ROW32>
+__device__ __forceinline__ float row_swap_sum(float v) {
+    const int b = __builtin_bit_cast(int, v);
+    const auto p = ROW32 ? __builtin_amdgcn_permlane32_swap(b, b, false, false)
+                         : __builtin_amdgcn_permlane16_swap(b, b, false, false);
+    return __builtin_bit_cast(float, (int)p[0]) + __builtin_bit_cast(float, (int)p[1]);
+}
+template <bool ROW32>
+__device__ __forceinline__ double row_swap_sum(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+    const auto pl = ROW32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                          : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto ph = ROW32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                          : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double a = __builtin_bit_cast(double, ((long long)(int)ph[0] << 32) | (unsigned int)pl[0]);
+    const double c = __builtin_bit_cast(double, ((long long)(int)ph[1] << 32) | (unsigned int)pl[1]);
+    return a + c;
+}
+
+// Sums of K values over the wave, identical in every lane, with no LDS.
+// Within each row of 16 lanes by DPP permutations: partner lane ^1 and ^2
+// (quad_perm [1,0,3,2], [2,3,0,1]), then the mirrored lane in the half row and
+// in the row (each pairs the two partial sums of the level below); across rows
+// by the permlane swaps of row pairs, then of wave halves.  Every step adds a
+// value to its partner's (or the same two values in both), so all lanes end
+// with the same bits.
 template <int K, class T>
 __device__ __forceinline__ void wave_sum_rows(T (&v)[K]) {
 #pragma unroll
@@ -182,13 +206,9 @@ __device__ __forceinline__ void wave_sum_rows(T (&v)[K]) {
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] += dpp_perm<0x140>(v[k]);
 #pragma unroll
-    for (int o = 16; o <= 32; o <<= 1) {
-        T t[K];
+    for (int k = 0; k < K; k++) v[k] = row_swap_sum<false>(v[k]);
 #pragma unroll
-        for (int k = 0; k < K; k++) t[k] = __shfl_xor(v[k], o, 64);
-#pragma unroll
-        for (int k = 0; k < K; k++) v[k] += t[k];
-    }
+    for (int k = 0; k < K; k++) v[k] = row_swap_sum<true>(v[k]);
 }
 
 // wave_sum of K values at once, butterfly step by butterfly step: every lane
