@@ -47,6 +47,7 @@ def parse():
     p.add_argument("--no-frames", action="store_true")
     p.add_argument("--no-formats", action="store_true")
     p.add_argument("--no-beads", action="store_true")
+    p.add_argument("--no-host", action="store_true")
     p.add_argument("--n128-batch", type=int, default=65536)
     p.add_argument("--mixed-total", type=int, default=200000)
     p.add_argument("--frames", type=int, default=2000)
@@ -582,6 +583,26 @@ def main():
         lm_rate_kernel = B / ((np.mean(fit_ms) + np.mean(init_ms)) / 1e3)
         out["batched_vs_plain"] = round(lm_rate_kernel / (Bp / ((pms + np.mean(init_ms) * Bp / B) / 1e3)), 3)
 
+    if not args.no_host:
+        # The same ROIs through the host-array entry point (zola_fit_batch):
+        # pixels and starts copied in from pageable host memory, results
+        # copied out, inside the timed call.  Starts come from the device
+        # initialiser above.
+        px_h = px.cpu().numpy()
+        init_h = init.cpu().numpy()
+        ctx.fit(cfg, px_h, init_h)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            res_h = ctx.fit(cfg, px_h, init_h)
+        th_ms = (time.perf_counter() - t0) * 1e3 / reps
+        out["host_arrays"] = {
+            "rois": B, "call_ms": round(th_ms, 3), "value": round(B / (th_ms / 1e3), 1),
+            "unit": "localizations/s",
+            "in_bytes": int(px_h.nbytes + init_h.nbytes),
+            "same_theta_as_device_form": bool(np.array_equal(res_h["theta"], theta.cpu().numpy())),
+            "note": "zola_fit_batch on host arrays: PCIe copies in and out included, init not included",
+        }
     if not args.no_calib:
         out["calibration"] = calibration_bench(torch, Z, stream, cpu=not args.no_cpu)
     if not args.no_n128:
