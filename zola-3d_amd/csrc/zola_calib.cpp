@@ -61,28 +61,46 @@ int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const doub
     zola_get_stream(ctx, &sp);
     hipStream_t st = (hipStream_t)sp;
 
+    // One device block: [inputs: params, amp, bg | outputs: nll, dparams, damp,
+    // dbg, da | pixels].  The input and output spans are mirrored in pinned host
+    // memory with the same offsets, so each evaluation moves one copy each way.
     CalibBuffers d;
-    const size_t sz[] = {al((size_t)B * 4 * sizeof(float)), al(B * R2 * sizeof(float)),
-                         al((size_t)B * sizeof(double)), al((size_t)B * sizeof(double)),
-                         al((size_t)B * sizeof(double)), al((size_t)B * 4 * sizeof(float)),
-                         al((size_t)B * sizeof(double)), al((size_t)B * sizeof(double)),
-                         al((size_t)(J > 0 ? J : 1) * sizeof(double))};
-    size_t total = 0;
-    for (size_t v : sz) total += v;
+    const size_t jz1 = (size_t)(J > 0 ? J : 1);
+    const size_t o_par = 0, o_amp = o_par + al((size_t)B * 4 * sizeof(float)),
+                 o_bg = o_amp + al((size_t)B * sizeof(double)), in_bytes = o_bg + al((size_t)B * sizeof(double));
+    const size_t o_nll = 0, o_dp = o_nll + al((size_t)B * sizeof(double)),
+                 o_dA = o_dp + al((size_t)B * 4 * sizeof(float)), o_dB = o_dA + al((size_t)B * sizeof(double)),
+                 o_da = o_dB + al((size_t)B * sizeof(double)), out_bytes = o_da + al(jz1 * sizeof(double));
+    const size_t total = in_bytes + out_bytes + al(B * R2 * sizeof(float));
     if (hipMalloc(&d.base, total) != hipSuccess) return ZOLA_ENOMEM;
-    char* p = (char*)d.base;
-    d.params = (float*)p; p += sz[0];
-    d.pixels = (float*)p; p += sz[1];
-    d.amp = (double*)p; p += sz[2];
-    d.bg = (double*)p; p += sz[3];
-    d.nll = (double*)p; p += sz[4];
-    d.dparams = (float*)p; p += sz[5];
-    d.damp = (double*)p; p += sz[6];
-    d.dbg = (double*)p; p += sz[7];
-    d.da = (double*)p;
+    char* hin = nullptr;
+    char* hout = nullptr;
+    if (hipHostMalloc((void**)&hin, in_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&hout, out_bytes, hipHostMallocDefault) != hipSuccess) {
+        if (hin) (void)hipHostFree(hin);
+        (void)hipFree(d.base);
+        return ZOLA_ENOMEM;
+    }
+    char* din = (char*)d.base;
+    char* dout = din + in_bytes;
+    d.params = (float*)(din + o_par);
+    d.amp = (double*)(din + o_amp);
+    d.bg = (double*)(din + o_bg);
+    d.nll = (double*)(dout + o_nll);
+    d.dparams = (float*)(dout + o_dp);
+    d.damp = (double*)(dout + o_dA);
+    d.dbg = (double*)(dout + o_dB);
+    d.da = (double*)(dout + o_da);
+    d.pixels = (float*)(dout + out_bytes);
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        (void)hipHostFree(hin);
+        (void)hipHostFree(hout);
+        (void)hipFree(d.base);
+    };
     int rc = ZOLA_OK;
     if (hipMemcpy(d.pixels, pixels, B * R2 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(d.base);
+        release();
         return ZOLA_EHIP;
     }
 
@@ -104,9 +122,14 @@ int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const doub
     for (int k = 0; k < nc; k++) u[k] = coef[fit[k]] / scale[k];
     for (int k = 0; k < 5 * nb; k++) u[nc + k] = bd[k] / scale[nc + k];
 
-    std::vector<float> hpar((size_t)B * 4);
-    std::vector<double> hamp(B), hbg(B), hnll(B), hdA(B), hdB(B), hda(J > 0 ? J : 1);
-    std::vector<float> hdp((size_t)B * 4);
+    float* hpar = (float*)(hin + o_par);
+    double* hamp = (double*)(hin + o_amp);
+    double* hbg = (double*)(hin + o_bg);
+    const double* hnll = (const double*)(hout + o_nll);
+    const float* hdp = (const float*)(hout + o_dp);
+    const double* hdA = (const double*)(hout + o_dA);
+    const double* hdB = (const double*)(hout + o_dB);
+    const double* hda = (const double*)(hout + o_da);
     int evals = 0;
     // f(u) and its gradient
     auto evaluate = [&](const std::vector<double>& uu, std::vector<double>& g, double& f) -> int {
@@ -124,20 +147,15 @@ int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const doub
                 hamp[i] = bb[3] * scale[nc + 5 * b + 3];
                 hbg[i] = bb[4] * scale[nc + 5 * b + 4];
             }
-        if (hipMemcpyAsync(d.params, hpar.data(), hpar.size() * sizeof(float), hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(d.amp, hamp.data(), B * sizeof(double), hipMemcpyHostToDevice, st) ||
-            hipMemcpyAsync(d.bg, hbg.data(), B * sizeof(double), hipMemcpyHostToDevice, st) ||
-            hipMemsetAsync(d.da, 0, (size_t)(J > 0 ? J : 1) * sizeof(double), st))
+        // the previous evaluation ended with a stream synchronisation, so the
+        // pinned input span is free to be rewritten above
+        if (hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, st) ||
+            hipMemsetAsync(d.da, 0, jz1 * sizeof(double), st))
             return ZOLA_EHIP;
         r = zola_calib_eval_dev(ctx, B, d.params, d.pixels, d.amp, d.bg, d.nll, d.dparams, d.damp,
                                 d.dbg, d.da);
         if (r) return r;
-        if (hipMemcpyAsync(hnll.data(), d.nll, B * sizeof(double), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(hdp.data(), d.dparams, hdp.size() * sizeof(float), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(hdA.data(), d.damp, B * sizeof(double), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(hdB.data(), d.dbg, B * sizeof(double), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(hda.data(), d.da, (size_t)(J > 0 ? J : 1) * sizeof(double), hipMemcpyDeviceToHost, st) ||
-            hipStreamSynchronize(st))
+        if (hipMemcpyAsync(hout, dout, out_bytes, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st))
             return ZOLA_EHIP;
         evals++;
         f = 0.0;
@@ -239,7 +257,7 @@ int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const doub
     if (out_nll) *out_nll = f;
     if (out_iters) *out_iters = it;
     if (out_evals) *out_evals = evals;
-    (void)hipFree(d.base);
+    release();
     return rc;
 }
 
