@@ -253,6 +253,45 @@ def test_noise_free_fit_recovers_truth(fit):
         assert np.abs(r["theta"][2:] - th[2:]).max() < 1e-3
 
 
+def test_reference_descent_on_noisy_rois_is_monotone_and_stops_at_the_optimum():
+    """The reference's coordinate descent (cpu/Localization.localize) on Poisson
+    ROIs, pinned without a reference run by what the method fixes:
+    * its likelihood is the Poisson NLL sum(mu - n log mu), recomputed here in
+      numpy from the model (a dropped or mis-signed term fails);
+    * it accepts only improving moves: the final NLL is below the start's and
+      not below the FP64 LM optimum of the same ROI;
+    * it stops next to that optimum (its rule: a sweep moving x, y, z by less
+      than 0.1 nm; the slow z direction leaves up to 1 nm);
+    * started at the optimum it stays there (a wrong derivative sign or step
+      rule would walk away)."""
+    m = O.Model(O.optics(), 21, Z.astigmatic_coeffs(28))
+    rng = np.random.default_rng(3)
+
+    def nll(roi, th):
+        mu = th[0] * m.psf(th[2], th[3], 0.0, th[4]) + th[1]
+        return float(np.sum(mu - roi * np.log(mu)))
+
+    for k in range(8):
+        tr = np.array([rng.uniform(800, 4000), rng.uniform(5, 40), rng.uniform(-0.1, 0.1),
+                       rng.uniform(-0.1, 0.1), rng.uniform(-0.8, 0.8)])
+        roi = rng.poisson(tr[0] * m.psf(tr[2], tr[3], 0.0, tr[4]) + tr[1]).astype(np.float64)
+        start = tr + np.array([tr[0] * rng.uniform(-0.3, 0.3), tr[1] * rng.uniform(-0.3, 0.3),
+                               rng.uniform(-0.08, 0.08), rng.uniform(-0.08, 0.08), rng.uniform(-0.15, 0.15)])
+        r = m.localize(roi, start, iterMax=50)
+        assert r["ok"] == 1 and r["iters"] < 50
+        end = nll(roi, r["theta"])
+        assert abs(r["lik"] - end) <= 1e-9 * abs(end)
+        opt = m.lm_fit(roi, r["theta"], maxit=500, tol=1e-12)
+        n_opt = nll(roi, opt["theta"])
+        assert end < nll(roi, start)
+        assert end >= n_opt - 1e-9 * abs(n_opt)
+        d = np.abs(r["theta"][2:] - opt["theta"][2:]) * 1000.0
+        assert d[:2].max() <= 0.1 and d[2] <= 1.0, (k, d)
+        r2 = m.localize(roi, opt["theta"], iterMax=20)
+        np.testing.assert_allclose(r2["theta"][2:], opt["theta"][2:], atol=1e-9)
+        assert nll(roi, r2["theta"]) <= n_opt + 1e-9 * abs(n_opt)
+
+
 def test_fisher_crlb_of_noise_free_model():
     # the FD Fisher of the reference (with its A-derivative quirk) and the
     # analytic Fisher agree on the x, y, z block
