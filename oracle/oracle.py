@@ -84,6 +84,9 @@ def lib():
         L.zo_calib_nll.restype = C.c_double
         L.zo_frame_localize.argtypes = [vp, vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_double, _dp, C.c_int]
+        L.zo_frame_localize_iters.argtypes = [vp, vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                              C.c_double, _dp, C.c_int, _ip]
+        L.zo_set_psf_perturb.argtypes = [C.c_double]
         L.zo_frame_localize_scmos.argtypes = [vp, vp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                               C.c_int, C.c_double, _dp, C.c_int]
         L.zo_search_psf_center.restype = C.c_double
@@ -291,12 +294,23 @@ class Detector:
 
 
 def frame_localize(model, det, image, zfocus, minZ, maxZ, iter_max=20, photon_thr=500.0,
-                   cap=10000):
-    """LocalizationThread.run on one frame: (n, 12) records."""
+                   cap=10000, with_iters=False):
+    """LocalizationThread.run on one frame: (n, 12) records (and, with_iters, the
+    iteration count of each record's fit)."""
     out = np.zeros((cap, 12))
+    if with_iters:
+        its = np.zeros(cap, np.int32)
+        n = lib().zo_frame_localize_iters(model._h, det._h, _d(image), zfocus, minZ, maxZ, iter_max,
+                                          photon_thr, out, cap, its)
+        return out[:min(n, cap)], its[:min(n, cap)]
     n = lib().zo_frame_localize(model._h, det._h, _d(image), zfocus, minZ, maxZ, iter_max,
                                 photon_thr, out, cap)
     return out[:min(n, cap)]
+
+
+def set_psf_perturb(eps):
+    """Sensitivity probe: deterministic PSF perturbation of up to eps * peak (0 = off)."""
+    lib().zo_set_psf_perturb(float(eps))
 
 
 def frame_localize_scmos(model, det, det_image, image, varg, zfocus, minZ, maxZ, iter_max=20,

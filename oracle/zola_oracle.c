@@ -17,6 +17,7 @@
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
+#include <stdint.h>
 #include <string.h>
 
 #define ZO_PI 3.14159265358979323846
@@ -138,6 +139,33 @@ struct zo_model {
 
 static int psf_cache_enabled = 0;
 void zo_set_psf_cache(int on) { psf_cache_enabled = on; }
+
+/* Sensitivity probe (no reference counterpart): with eps > 0 every PSF gets a
+ * deterministic perturbation of up to eps * max(psf) per pixel, a hash of the
+ * pixel index and of the bits of (x, y, zoil, zwat), so the model stays a fixed
+ * function of its parameters as an FP32 synthesis is.  Used to measure how far an
+ * algorithm's end point moves under errors of the size of the GPU's FP32 PSF. */
+static double psf_perturb_eps = 0.0;
+void zo_set_psf_perturb(double eps) { psf_perturb_eps = eps; }
+static uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+static void psf_perturb(double* out, int n, double x, double y, double zoil, double zwat) {
+    double v[4] = {x, y, zoil, zwat}, mx = 0;
+    uint64_t h = 0x9e3779b97f4a7c15ULL;
+    for (int k = 0; k < 4; k++) {
+        uint64_t b;
+        memcpy(&b, &v[k], 8);
+        h = mix64(h ^ b);
+    }
+    for (int k = 0; k < n; k++) mx = fmax(mx, out[k]);
+    for (int k = 0; k < n; k++) {
+        double u = (double)(mix64(h + (uint64_t)k) >> 11) * (1.0 / 9007199254740992.0);
+        out[k] += psf_perturb_eps * mx * (2 * u - 1);
+    }
+}
 
 static void model_gauss(zo_model* m, double sigma) {
     int s2 = m->s2;
@@ -392,6 +420,7 @@ void zo_model_psf(zo_model* m, double x, double y, double zoil, double zwat, dou
             r += out[i * R + ii];
         }
     for (int k = 0; k < R * R; k++) out[k] /= r;
+    if (psf_perturb_eps > 0) psf_perturb(out, R * R, x, y, zoil, zwat);
     m->last[0] = x; m->last[1] = y; m->last[2] = zoil; m->last[3] = zwat;
     memcpy(m->last_psf, out, sizeof(double) * R * R);
     m->have_last = 1;
@@ -1491,18 +1520,24 @@ static void get_a_and_b_scmos(int npx, const double* patch, const double* psf, c
 
 static int frame_walk(zo_model* m, zo_detector* d, const double* det_image, const double* image,
                       const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
-                      double photonThr, double* out, int cap);
+                      double photonThr, double* out, int cap, int* iters);
 
 int zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double zfocus,
                       double minZ, double maxZ, int iterMax, double photonThr,
                       double* out, int cap) {
-    return frame_walk(m, d, image, image, NULL, zfocus, minZ, maxZ, iterMax, photonThr, out, cap);
+    return frame_walk(m, d, image, image, NULL, zfocus, minZ, maxZ, iterMax, photonThr, out, cap, NULL);
+}
+
+int zo_frame_localize_iters(zo_model* m, zo_detector* d, const double* image, double zfocus,
+                            double minZ, double maxZ, int iterMax, double photonThr,
+                            double* out, int cap, int* iters) {
+    return frame_walk(m, d, image, image, NULL, zfocus, minZ, maxZ, iterMax, photonThr, out, cap, iters);
 }
 
 int zo_frame_localize_scmos(zo_model* m, zo_detector* d, const double* det_image, const double* image,
                             const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
                             double photonThr, double* out, int cap) {
-    return frame_walk(m, d, det_image, image, varg, zfocus, minZ, maxZ, iterMax, photonThr, out, cap);
+    return frame_walk(m, d, det_image, image, varg, zfocus, minZ, maxZ, iterMax, photonThr, out, cap, NULL);
 }
 
 /* The walk on `image`, candidates from `det_image`; with an sCMOS map varg
@@ -1510,7 +1545,7 @@ int zo_frame_localize_scmos(zo_model* m, zo_detector* d, const double* det_image
  * (LocalizationPipeline.java:797-850). */
 static int frame_walk(zo_model* m, zo_detector* d, const double* det_image, const double* image,
                       const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
-                      double photonThr, double* out, int cap) {
+                      double photonThr, double* out, int cap, int* iters) {
     int W = d->W, H = d->H, R = m->R, n = R * R, h = R / 2;
     double dx = m->o.xystep;
     int ccap = 1 << 16;
@@ -1586,6 +1621,7 @@ static int frame_walk(zo_model* m, zo_detector* d, const double* det_image, cons
                             o[9] = 1000 * crlb[2];
                             o[10] = lik;
                             o[11] = k;
+                            if (iters) iters[nloc] = it;
                         }
                         nloc++;
                         for (int a = 0; a < R; a++)

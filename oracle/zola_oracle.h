@@ -101,6 +101,9 @@ int zo_localize(zo_model* m, const double* roi, double* theta, double zfocus,
                 int* nan_flag);
 
 void zo_set_psf_cache(int on);  /* reuse the PSF when (x,y,zoil,zwat) repeat */
+/* Sensitivity probe: deterministic perturbation of up to eps * max(psf) per PSF
+ * pixel (0 = off), a fixed function of the parameters like an FP32 synthesis. */
+void zo_set_psf_perturb(double eps);
 
 /* Same over a batch with nthreads POSIX threads (one model copy per thread).
  * rois: B*R*R, theta: B*5 in/out. */
@@ -158,6 +161,10 @@ int  zo_frame_localize(zo_model* m, zo_detector* d, const double* image, double 
  * ((adu - offset)/gain + var/gain^2 per pixel), varg the W x H map var/gain^2
  * used by getAandB and the fit (LocalizationPipeline.java:597-633, 797-850,
  * 1168-1211). */
+/* As zo_frame_localize, also writing each record's fit iteration count (cap ints). */
+int  zo_frame_localize_iters(zo_model* m, zo_detector* d, const double* image, double zfocus,
+                             double minZ, double maxZ, int iterMax, double photonThr,
+                             double* out, int cap, int* iters);
 int  zo_frame_localize_scmos(zo_model* m, zo_detector* d, const double* det_image, const double* image,
                              const double* varg, double zfocus, double minZ, double maxZ, int iterMax,
                              double photonThr, double* out, int cap);

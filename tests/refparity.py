@@ -1,0 +1,59 @@
+"""Comparison of the GPU reference-form fit (the coordinate descent of
+process/cpu/Localization.java:491-653) with its FP64 restatement, zo_localize,
+run from the same starts.
+
+Every ROI is compared, including those that stop at the iteration cap
+(Localization.java:595-618 with iterMax from LocalizationPipeline.java:77).
+Both sides run the same deterministic descent; the GPU synthesises the PSF in
+FP32 (errors up to ~7e-7 of the peak), so its likelihoods differ from FP64 by
+about 1e-9 relative.
+
+Converged ROIs (the descent stopped on its 0.1 nm rule) sit at a well-defined
+point, and there the bar is the north star's: x, y within 0.5 nm, z within 1 nm.
+
+A capped ROI stops while still creeping along a flat likelihood valley, so its
+end point is ill-conditioned: perturbing the FP64 oracle's own PSF by 1e-7 of its
+peak (a seventh of the FP32 error) moves capped end points by several nm while
+converged ones stay within 0.2 nm (tests/test_oracle_pins.py::
+test_capped_descent_end_points_are_ill_conditioned).  For capped ROIs the bar is
+therefore: the iteration cap reached on both sides, NLL within 1e-5 relative, x,
+y, z within CAPPED_NM (15 nm, the order of the CRLB at these photon counts) and
+photons within CAPPED_REL (1e-2) relative.
+
+On every ROI: NLL (sum mu - n ln mu at the returned theta) within 1e-5 relative
+and iteration counts within one sweep; equal counts on at least 80% of the ROIs.
+A count one sweep apart is a stop test (the sweep's move against 0.1 nm) taken
+within the FP32 noise of its threshold: 6 of 68 ROIs across the GPU tests.
+"""
+import numpy as np
+
+CAPPED_NM = 15.0
+CAPPED_REL = 1e-2
+
+
+def compare(res, ref, max_iter, label=""):
+    th_ref = np.array([r["theta"] for r in ref])
+    it_ref = np.array([r["iters"] for r in ref])
+    lik_ref = np.array([r["lik"] for r in ref])
+    d = np.abs(res["theta"][:, 2:] - th_ref[:, 2:]) * 1000.0
+    same_it = res["iters"] == it_ref
+    capped = it_ref >= max_iter
+    conv = ~capped
+    rel = np.abs(res["nll"] - lik_ref) / np.abs(lik_ref)
+    print("%s reference form: %d ROIs, %d capped (cpu), same iterations %.3f" %
+          (label, len(it_ref), int(capped.sum()), float(same_it.mean())))
+    if conv.any():
+        print("  |d| nm max converged x, y, z:", d[conv].max(axis=0))
+    if capped.any():
+        print("  |d| nm max capped x, y, z:", d[capped].max(axis=0))
+    print("  NLL rel max:", rel.max(), "iters gpu", res["iters"].tolist(), "cpu", it_ref.tolist())
+    assert same_it.mean() >= 0.8
+    assert np.abs(res["iters"] - it_ref).max() <= 1
+    assert rel.max() <= 1e-5
+    if conv.any():
+        assert d[conv][:, :2].max() <= 0.5 and d[conv][:, 2].max() <= 1.0
+    if capped.any():
+        assert np.all(res["iters"][capped] == max_iter)
+        assert d[capped].max() <= CAPPED_NM
+        assert np.all(np.abs(res["theta"][capped, 0] - th_ref[capped, 0]) <= CAPPED_REL * th_ref[capped, 0])
+    return d, same_it, capped

@@ -32,5 +32,5 @@ def test_backward_matches_oracle_adjoint(R, J):
     err_p = np.abs(dp - ref_dp).max(axis=0) / np.abs(ref_dp).max(axis=0)
     err_a = np.abs(da - ref_da).max() / np.abs(ref_da).max()
     print("rel err dparams", err_p, "da", err_a)
-    assert np.all(err_p < 1e-4)
-    assert err_a < 1e-4
+    assert np.all(err_p < 1e-6)
+    assert err_a < 1e-6

@@ -91,6 +91,17 @@ def test_flat_frame_has_no_candidates(models):
     det.close()
 
 
+def _score_and_nll(locs, ref):
+    """Column 6: scoreCompute's chi2 of the accepted model (LocalizationPipeline.java:
+    1099-1115); column 10: the fit's NLL.  Both depend on the fitted theta, which
+    agrees to a fraction of a nanometre."""
+    ds = np.abs(locs[:, 6] - ref[:, 6]) / np.abs(ref[:, 6])
+    dn = np.abs(locs[:, 10] - ref[:, 10]) / np.abs(ref[:, 10])
+    print("score rel max", ds.max(), "NLL rel max", dn.max())
+    assert ds.max() <= 1e-3
+    assert dn.max() <= 1e-5
+
+
 def _oracle_walk(om, frames, zlim):
     recs = []
     for f in range(frames.shape[0]):
@@ -120,6 +131,7 @@ def test_frame_walk_reference_form_matches_oracle(models):
     assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
     np.testing.assert_allclose(locs[:, 4], ref[:, 4], rtol=1e-3)
     np.testing.assert_allclose(locs[:, 7:10], ref[:, 7:10], rtol=2e-2)
+    _score_and_nll(locs, ref)
     det.close()
 
 
@@ -199,4 +211,5 @@ def test_frame_walk_scmos_matches_oracle(models):
     print("max |d| nm x, y, z:", d.max(axis=0))
     assert d[:, :2].max() <= 0.5 and d[:, 2].max() <= 1.0
     np.testing.assert_allclose(locs[:, 4:6], ref[:, 4:6], rtol=1e-3)
+    _score_and_nll(locs, ref)
     det.close()

@@ -4,13 +4,14 @@ every instantiated ROI side, against the FP64 oracle:
 * LM form: the FP64 Levenberg-Marquardt optimum (0.5 nm x/y, 1 nm z, 1e-5
   relative NLL);
 * reference form: the FP64 restatement of cpu/Localization.localize from the
-  same start (0.5 nm x/y, 1 nm z where both converge);
+  same start, on every ROI (tests/refparity.py);
 * the autotuner returns a valid layout and the fits that follow use it.
 """
 import numpy as np
 import pytest
 
 import oracle as O
+import refparity
 import simdata
 import zola_hip as Z
 
@@ -71,14 +72,7 @@ def test_reference_form_quad_layout_matches_restatement(R):
         ref = [m.localize(rois[b].astype(np.float64), init[b], minZ=-1.3, maxZ=1.3, iterMax=20) for b in range(B)]
     finally:
         O.lib().zo_set_psf_cache(0)
-    th_ref = np.array([r["theta"] for r in ref])
-    it_ref = np.array([r["iters"] for r in ref])
-    d = (res["theta"][:, 2:] - th_ref[:, 2:]) * 1000.0
-    both = (it_ref < 20) & (res["iters"] < 20)
-    print("ref-form |d| nm max:", np.abs(d[both]).max(axis=0), "iters gpu", res["iters"], "cpu", it_ref)
-    assert both.sum() >= B // 2
-    assert np.abs(d[both][:, :2]).max() <= 0.5
-    assert np.abs(d[both][:, 2]).max() <= 1.0
+    refparity.compare(res, ref, 20, "R %d quad" % R)
 
 
 def test_autotune_picks_a_layout_and_fits_follow_it():

@@ -7,6 +7,7 @@ import pytest
 
 import oracle as O
 import simdata
+import refparity
 import zola_hip as Z
 
 pytestmark = pytest.mark.gpu
@@ -78,19 +79,12 @@ def test_reference_form_scmos_matches_restatement(torch_cuda):
     th, nl, it, st = _fit(torch_cuda, ctx, cfg, rois, varg, init)
     ctx.close()
     O.lib().zo_set_psf_cache(1)
-    ref, itr = [], []
+    ref = []
     try:
         for b in range(B):
             O.set_scmos(varg[b].astype(np.float32).astype(np.float64))
-            r = m.localize(rois[b].astype(np.float64), init[b], minZ=-1.3, maxZ=1.3, iterMax=20)
-            ref.append(r["theta"])
-            itr.append(r["iters"])
+            ref.append(m.localize(rois[b].astype(np.float64), init[b], minZ=-1.3, maxZ=1.3, iterMax=20))
     finally:
         O.set_scmos(None)
         O.lib().zo_set_psf_cache(0)
-    ref, itr = np.array(ref), np.array(itr)
-    d = (th[:, 2:] - ref[:, 2:]) * 1000
-    both = (itr < 20) & (it < 20)
-    print("sCMOS ref-form |d| nm max:", np.abs(d[both]).max(axis=0), "iters gpu", it, "cpu", itr)
-    assert both.sum() >= B // 2
-    assert np.abs(d[both][:, :2]).max() <= 0.5 and np.abs(d[both][:, 2]).max() <= 1.0
+    refparity.compare(dict(theta=th, nll=nl, iters=it), ref, 20, "sCMOS")
