@@ -214,6 +214,43 @@ def optics(size_fft=64, xystep=0.11, zstep=0.05, wavelength=0.68, noil=1.518, nw
     return Optics(size_fft, xystep, zstep, wavelength, noil, nwat, na, sigma, with_apo, 0)
 
 
+_DT = {"f4": ("float32", "torch.float32"), "f8": ("float64", "torch.float64"), "i4": ("int32", "torch.int32")}
+
+
+def _shape_ok(shape, want):
+    return len(shape) == len(want) and all(w is None or int(s) == w for s, w in zip(shape, want))
+
+
+def _host(a, name, want, dt):
+    """Host array argument: converted to a C-contiguous array of dtype dt and
+    checked against the shape want (None entries are free).  Raises ValueError
+    before anything reaches the library, which trusts the sizes it is given."""
+    a = np.ascontiguousarray(a, dtype=_DT[dt][0])
+    if not _shape_ok(a.shape, want):
+        raise ValueError(f"{name}: shape {tuple(a.shape)}, expected {tuple(want)}")
+    return a
+
+
+def _dev(t, name, want, dt, optional=False):
+    """Device argument of a *_dev entry point: a torch tensor on the GPU,
+    C-contiguous, of dtype dt and shape want (None entries free)."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: a device tensor is required")
+    if not (hasattr(t, "data_ptr") and hasattr(t, "is_cuda")):
+        raise ValueError(f"{name}: expected a torch tensor in GPU memory, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: tensor is not in GPU memory")
+    if str(t.dtype) != _DT[dt][1]:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {_DT[dt][1]}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if not _shape_ok(tuple(t.shape), want):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(want)}")
+    return t
+
+
 def _ptr(a):
     """Data pointer of a numpy array or a torch tensor."""
     if a is None:
@@ -259,7 +296,7 @@ class Context:
             raise ZolaError(f"{what}: {lib().zola_last_error(self._h).decode()} (rc={rc})")
 
     def set_coefficients(self, coeffs):
-        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        c = _host(coeffs, "coeffs", (self.J,), "f8")
         self._check(lib().zola_set_coefficients(self._h, _ptr(c), len(c)), "set_coefficients")
 
     def set_sigma(self, sigma):
@@ -271,7 +308,7 @@ class Context:
     # -- PSF ---------------------------------------------------------------
     def psf_forward(self, params):
         """params: (B, 4) [x, y, z, zoil] um -> (B, R, R) float32 (host arrays)."""
-        p = np.ascontiguousarray(params, dtype=np.float32)
+        p = _host(params, "params", (None, 4), "f4")
         B = p.shape[0]
         out = np.zeros((B, self.R, self.R), np.float32)
         self._check(lib().zola_psf_forward(self._h, B, _ptr(p), _ptr(out)), "psf_forward")
@@ -279,13 +316,15 @@ class Context:
 
     def psf_forward_dev(self, params, out):
         """Device form: torch float32 tensors already on the GPU."""
+        B = _dev(params, "params", (None, 4), "f4").shape[0]
+        _dev(out, "out", (B, self.R, self.R), "f4")
         self._check(lib().zola_psf_forward_dev(self._h, params.shape[0], _ptr(params), _ptr(out)),
                     "psf_forward_dev")
 
     def psf_backward(self, params, w, with_coeffs=True):
-        p = np.ascontiguousarray(params, dtype=np.float32)
-        w = np.ascontiguousarray(w, dtype=np.float32)
+        p = _host(params, "params", (None, 4), "f4")
         B = p.shape[0]
+        w = _host(w, "w", (B, self.R, self.R), "f4")
         dp = np.zeros((B, 4), np.float32)
         da = np.zeros(self.J, np.float64) if with_coeffs else None
         self._check(lib().zola_psf_backward(self._h, B, _ptr(p), _ptr(w), _ptr(dp), _ptr(da)),
@@ -293,6 +332,10 @@ class Context:
         return dp, da
 
     def psf_backward_dev(self, params, w, dparams, da=None):
+        B = _dev(params, "params", (None, 4), "f4").shape[0]
+        _dev(w, "w", (B, self.R, self.R), "f4")
+        _dev(dparams, "dparams", (B, 4), "f4")
+        _dev(da, "da", (self.J,), "f8", optional=True)
         self._check(lib().zola_psf_backward_dev(self._h, params.shape[0], _ptr(params), _ptr(w),
                                                 _ptr(dparams), _ptr(da)), "psf_backward_dev")
 
@@ -304,9 +347,9 @@ class Context:
 
     def fit(self, cfg, pixels, init):
         """Host form. pixels (B, R, R) float32, init (B, 5) float64 [A, B, x, y, z]."""
-        px = np.ascontiguousarray(pixels, dtype=np.float32)
-        ini = np.ascontiguousarray(init, dtype=np.float64)
+        px = _host(pixels, "pixels", (None, self.R, self.R), "f4")
         B = px.shape[0]
+        ini = _host(init, "init", (B, 5), "f8")
         theta = np.zeros((B, 5)); nll = np.zeros(B); crlb = np.zeros((B, 3))
         iters = np.zeros(B, np.int32); status = np.zeros(B, np.int32)
         self._check(lib().zola_fit_batch(self._h, C.byref(cfg), B, _ptr(px), _ptr(ini),
@@ -349,8 +392,10 @@ class Context:
         """InitBackgroundAndPhotonNumber (CCD/EMCCD form).  adu (S, W, H) float32
         raw stack [z][i][ii]; xp, yp float seeds.  Returns refined seeds,
         patches (nb, S, P, P), B (nb), A (nb, S), background (W, H) and diag."""
-        a = np.ascontiguousarray(adu, dtype=np.float32)
+        a = _host(adu, "adu", (None, None, None), "f4")
         S, W, H = a.shape
+        if len(xp) != len(yp):
+            raise ValueError(f"xp, yp: {len(xp)} and {len(yp)} seeds")
         o = self._bead_out(S, W, H, P, xp, yp)
         d = BeadDiag()
         self._check(lib().zola_bead_prepare(self._h, S, W, H, _ptr(a), adu_factor, gain, offset, zstep, P,
@@ -363,9 +408,11 @@ class Context:
     def bead_prepare_scmos(self, adu, offset, gain, variance, xp, yp, P, zstep):
         """InitBackgroundAndPhotonNumber (sCMOS form) with float camera maps (W, H);
         adds scmos (nb, P, P) as the reference leaves it (last bead only)."""
-        a = np.ascontiguousarray(adu, dtype=np.float32)
+        a = _host(adu, "adu", (None, None, None), "f4")
         S, W, H = a.shape
-        maps = [np.ascontiguousarray(m, dtype=np.float32) for m in (offset, gain, variance)]
+        if len(xp) != len(yp):
+            raise ValueError(f"xp, yp: {len(xp)} and {len(yp)} seeds")
+        maps = [_host(m, n, (W, H), "f4") for m, n in ((offset, "offset"), (gain, "gain"), (variance, "variance"))]
         o = self._bead_out(S, W, H, P, xp, yp)
         o["scmos"] = np.zeros((len(o["x"]), P, P))
         d = BeadDiag()
@@ -377,8 +424,20 @@ class Context:
         o["diag"] = self._bead_diag(d)
         return o
 
+    def _fit_outputs(self, pixels, init, theta, nll, crlb, iters, status):
+        B = _dev(pixels, "pixels", (None, self.R, self.R), "f4").shape[0]
+        _dev(init, "init", (B, 5), "f8")
+        _dev(theta, "theta", (B, 5), "f8")
+        _dev(nll, "nll", (B,), "f8")
+        _dev(crlb, "crlb", (B, 3), "f8")
+        _dev(iters, "iters", (B,), "i4")
+        _dev(status, "status", (B,), "i4")
+        return B
+
     def fit_zoil_dev(self, cfg, pixels, init, zoil, theta, nll, crlb, iters, status):
         """Device form with a per-ROI focus term zoil (B floats)."""
+        B = self._fit_outputs(pixels, init, theta, nll, crlb, iters, status)
+        _dev(zoil, "zoil", (B,), "f4")
         self._check(lib().zola_fit_batch_zoil_dev(self._h, C.byref(cfg), pixels.shape[0], _ptr(pixels),
                                                   _ptr(init), _ptr(zoil), _ptr(theta), _ptr(nll), _ptr(crlb),
                                                   _ptr(iters), _ptr(status)), "fit_batch_zoil_dev")
@@ -386,6 +445,9 @@ class Context:
     def fit_ext_dev(self, cfg, pixels, init, theta, nll, crlb, iters, status, zoil=None, varg=None):
         """Device form with an optional per-ROI focus term zoil (B floats) and an
         optional sCMOS map varg (B x R x R floats, var/gain^2 added to mu)."""
+        B = self._fit_outputs(pixels, init, theta, nll, crlb, iters, status)
+        _dev(zoil, "zoil", (B,), "f4", optional=True)
+        _dev(varg, "varg", (B, self.R, self.R), "f4", optional=True)
         self._check(lib().zola_fit_batch_ext_dev(self._h, C.byref(cfg), pixels.shape[0], _ptr(pixels), _ptr(init),
                                                  _ptr(zoil) if zoil is not None else None,
                                                  _ptr(varg) if varg is not None else None, _ptr(theta),
@@ -396,6 +458,17 @@ class Context:
         """groups: list of dicts with keys R, pixels, init, theta, nll, crlb, iters, status
         (device tensors); one graph launch on the context's stream."""
         G = len(groups)
+        if not 1 <= G <= 8:
+            raise ValueError(f"groups: {G}, expected 1 to 8")
+        for g in groups:
+            R = int(g["R"])
+            B = _dev(g["pixels"], "pixels", (None, R, R), "f4").shape[0]
+            _dev(g["init"], "init", (B, 5), "f8")
+            _dev(g["theta"], "theta", (B, 5), "f8")
+            _dev(g["nll"], "nll", (B,), "f8")
+            _dev(g["crlb"], "crlb", (B, 3), "f8")
+            _dev(g["iters"], "iters", (B,), "i4")
+            _dev(g["status"], "status", (B,), "i4")
         Rs = (C.c_int * G)(*[int(g["R"]) for g in groups])
         Bs = (C.c_int * G)(*[int(g["pixels"].shape[0]) for g in groups])
         arr = {k: (C.c_void_p * G)(*[g[k].data_ptr() for g in groups])
@@ -405,6 +478,7 @@ class Context:
                                              arr["status"]), "fit_mixed_dev")
 
     def fit_dev(self, cfg, pixels, init, theta, nll, crlb, iters, status):
+        self._fit_outputs(pixels, init, theta, nll, crlb, iters, status)
         self._check(lib().zola_fit_batch_dev(self._h, C.byref(cfg), pixels.shape[0],
                                              _ptr(pixels), _ptr(init), _ptr(theta), _ptr(nll),
                                              _ptr(crlb), _ptr(iters), _ptr(status)),
@@ -412,28 +486,36 @@ class Context:
 
     # -- calibration -------------------------------------------------------
     def calib_eval_dev(self, params, pixels, amp, bg, nll, dparams, damp, dbg, da=None):
+        B = _dev(params, "params", (None, 4), "f4").shape[0]
+        _dev(pixels, "pixels", (B, self.R, self.R), "f4")
+        for name, t in (("amp", amp), ("bg", bg), ("nll", nll), ("damp", damp), ("dbg", dbg)):
+            _dev(t, name, (B,), "f8")
+        _dev(dparams, "dparams", (B, 4), "f4")
+        _dev(da, "da", (self.J,), "f8", optional=True)
         self._check(lib().zola_calib_eval_dev(self._h, params.shape[0], _ptr(params), _ptr(pixels),
                                               _ptr(amp), _ptr(bg), _ptr(nll), _ptr(dparams),
                                               _ptr(damp), _ptr(dbg), _ptr(da)), "calib_eval_dev")
 
     def calib_fit(self, stacks, dz, bead, coeffs, fit_mask=None, max_iter=100, gtol=1e-6):
         """ZernikePhaseRetrieval: stacks (nb, S, R, R), dz (S,), bead (nb, 5) [x, y, z0, A, bg]."""
-        px = np.ascontiguousarray(stacks, dtype=np.float32)
+        px = _host(stacks, "stacks", (None, None, self.R, self.R), "f4")
         nb, S = px.shape[:2]
-        bd = np.ascontiguousarray(bead, dtype=np.float64).copy()
-        co = np.ascontiguousarray(coeffs, dtype=np.float64).copy()
-        mask = None if fit_mask is None else np.ascontiguousarray(fit_mask, dtype=np.int32)
+        bd = _host(bead, "bead", (nb, 5), "f8").copy()
+        co = _host(coeffs, "coeffs", (self.J,), "f8").copy()
+        dz = _host(dz, "dz", (S,), "f8")
+        mask = None if fit_mask is None else _host(fit_mask, "fit_mask", (self.J,), "i4")
         nll = C.c_double()
         it = C.c_int()
         ev = C.c_int()
-        self._check(lib().zola_calib_fit(self._h, nb, S, _ptr(px),
-                                         _ptr(np.ascontiguousarray(dz, dtype=np.float64)),
+        self._check(lib().zola_calib_fit(self._h, nb, S, _ptr(px), _ptr(dz),
                                          _ptr(bd), _ptr(co), _ptr(mask), int(max_iter),
                                          float(gtol), C.byref(nll), C.byref(it), C.byref(ev)),
                     "calib_fit")
         return dict(bead=bd, coeffs=co, nll=nll.value, iters=it.value, evals=ev.value)
 
     def fit_init_dev(self, pixels, nZ, zmin_t, zstep_t, zfocus, init):
+        B = _dev(pixels, "pixels", (None, self.R, self.R), "f4").shape[0]
+        _dev(init, "init", (B, 5), "f8")
         self._check(lib().zola_fit_init_dev(self._h, pixels.shape[0], _ptr(pixels), int(nZ),
                                             float(zmin_t), float(zstep_t), float(zfocus),
                                             _ptr(init)), "fit_init_dev")
