@@ -94,6 +94,20 @@ int  zola_psf_backward(zola_ctx* ctx, int B, const float* params, const float* w
 int  zola_psf_backward_dev(zola_ctx* ctx, int B, const float* params, const float* w,
                            float* dparams, double* da);
 
+/* Allocation-free form of zola_psf_backward_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or calibration loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as zola_psf_backward_dev; the per-PSF
+ * scratch (dL/dphase on the pupil window, 1024 floats per PSF) lives in a fixed
+ * device array of ZOLA_NOALLOC_MAX_PSF x 1024 floats (8 MiB) that every GPU the
+ * library runs on holds until the process ends, instead of the context's
+ * grow-only buffer.  B above ZOLA_NOALLOC_MAX_PSF returns ZOLA_EINVAL.  All calls
+ * on one GPU share that array: calls that overlap on one GPU (two streams or two
+ * contexts at once) race. */
+#define ZOLA_NOALLOC_MAX_PSF 2048
+int  zola_psf_backward_noalloc_dev(zola_ctx* ctx, int B, const float* params, const float* w,
+                                   float* dparams, double* da);
+
 /* Fit modes */
 #define ZOLA_FIT_REFERENCE 0  /* reference coordinate descent (cpu/Localization.java:491-653) */
 #define ZOLA_FIT_LM        1  /* batched Poisson-MLE Levenberg-Marquardt with analytic tangents */

@@ -41,6 +41,7 @@ struct zola_ctx {
     float* d_qbar = nullptr;      // per-PSF dL/dphase for the adjoint, grow-only
     size_t qbar_bytes = 0;
     int* d_queue = nullptr;       // ROI counters of the persistent fits (one per concurrent group)
+    float* scratch = nullptr;     // device address of the fixed allocation-free scratch (zola_noalloc.cpp)
     // graph of the last mixed-size fit (one kernel node per ROI-size group)
     hipGraphExec_t mixed_exec = nullptr;
     hipStream_t side[8] = {};
@@ -50,6 +51,10 @@ struct zola_ctx {
     // (reference form, LM): ZOLA_LAYOUT_*
     int layout[4][2] = {};
 };
+
+// Looks up the device address of the fixed scratch array of the allocation-free
+// entry points for the current device (once per device) into c->scratch.
+int zola_scratch_setup(zola_ctx* c);
 
 // Four waves per ROI (1) or one wave per ROI (0) for a 64-grid fit.
 int zola_fit_quad(const zola_ctx* c, int R, int mode);

@@ -246,6 +246,10 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
 
     HIP_TRY(c, hipStreamCreate(&c->stream));
     c->own_stream = true;
+    {
+        int rs = zola_scratch_setup(c);
+        if (rs) return rs;
+    }
     HIP_TRY(c, hipMalloc((void**)&c->d_queue, 8 * sizeof(int)));
     if (N == zola::kN2) {
         HIP_TRY(c, hipMalloc((void**)&c->d_tab128, zola::kW2 * zola::kW2 * sizeof(float4)));

@@ -34,6 +34,7 @@ SYMBOLS = [
     "zola_java_double_string",
     "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
     "zola_formats_last_error", "zola_bead_prepare", "zola_bead_prepare_scmos",
+    "zola_psf_backward_noalloc_dev",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -92,7 +93,7 @@ def lib():
         L.zola_set_stream.argtypes = [vp, vp]
         for name in ("zola_psf_forward", "zola_psf_forward_dev"):
             getattr(L, name).argtypes = [vp, C.c_int, vp, vp]
-        for name in ("zola_psf_backward", "zola_psf_backward_dev"):
+        for name in ("zola_psf_backward", "zola_psf_backward_dev", "zola_psf_backward_noalloc_dev"):
             if hasattr(L, name):
                 getattr(L, name).argtypes = [vp, C.c_int, vp, vp, vp, vp]
         for name in ("zola_fit_batch", "zola_fit_batch_dev"):
