@@ -76,57 +76,78 @@ def synth(ctx, torch, B, seed):
 
 
 def calibration_bench(torch, Z, stream, cpu=True):
-    """PSF calibration from 5 bead z-stacks of 41 slices (50 nm, -1..+1 um), 31x31 px,
-    2e5 photons/slice, background 10, 45 Zernike modes ~ N(0, 0.15 rad) with
-    Noll 5 = 0.8 rad; all 42 non-registration coefficients plus per-bead
-    (x, y, z0, N, bg) fitted by L-BFGS on adjoint gradients from a zero start."""
+    """PSF calibration on the reference's full model (ZernikePhaseRetrieval.Model):
+    5 bead z-stacks of 41 slices (50 nm, -1..+1 um), 31x31 px, 2e5 photons per
+    slice on a per-slice curve, order-2 background around 10 photons/px, 45 Zernike
+    modes ~ N(0, 0.15 rad) with Noll 5 = 0.8 rad.  Fitted by L-BFGS on exact
+    adjoint gradients from a zero start: the 42 non-registration coefficients,
+    sigma, and per bead x, y, z0, the six background coefficients, fit_a and the
+    41 fit_a_each."""
     Rc, J, nb, S = 31, 45, 5, 41
     rng = np.random.default_rng(0x20A1D3 + 3)
     truth = rng.normal(0.0, 0.15, J)
     truth[:3] = 0.0
     truth[Z.nm_to_slot(*Z.noll_to_nm(5))] = 0.8
     dz = (np.arange(S) - S // 2) * 0.05
-    beads = np.stack([rng.uniform(-0.05, 0.05, nb), rng.uniform(-0.05, 0.05, nb),
-                      rng.uniform(-0.05, 0.05, nb), np.full(nb, 2e5), np.full(nb, 10.0)], 1)
+    xyz = np.stack([rng.uniform(-0.05, 0.05, nb), rng.uniform(-0.05, 0.05, nb), rng.uniform(-0.05, 0.05, nb)], 1)
+    A = 2e5 * (1.0 + 0.15 * np.sin(np.arange(S)[None, :] / 5.0 + rng.uniform(0, 6, (nb, 1))))
+    poly = np.zeros((nb, 6))
+    poly[:, 0] = 10.0 + rng.uniform(-2, 2, nb)
+    poly[:, 1] = rng.uniform(-0.05, 0.05, nb)
+    poly[:, 3] = rng.uniform(-0.05, 0.05, nb)
+    ii = np.arange(Rc, dtype=np.float64)
+    mono = np.stack([np.ones((Rc, Rc)), ii[:, None] * np.ones(Rc), (ii * ii)[:, None] * np.ones(Rc),
+                     np.ones(Rc)[:, None] * ii, ii[:, None] * ii, np.ones(Rc)[:, None] * ii * ii])
     gen = Z.Context(Z.optics(), Rc, truth)
     gen.set_stream(stream.cuda_stream)
     par = np.zeros((nb * S, 4), np.float32)
     for b in range(nb):
         for s_ in range(S):
-            par[b * S + s_] = [beads[b, 0], beads[b, 1], 0.0, dz[s_] + beads[b, 2]]
+            par[b * S + s_] = [xyz[b, 0], xyz[b, 1], 0.0, dz[s_] + xyz[b, 2]]
     P = torch.from_numpy(par).cuda()
     psf = torch.empty(nb * S, Rc, Rc, device="cuda")
     gen.psf_forward_dev(P, psf)
+    bgmap = torch.from_numpy(np.repeat(np.tensordot(poly, mono, 1), S, axis=0)).float().cuda()
+    amp_t = torch.from_numpy(A.reshape(-1)).float().cuda()
     g = torch.Generator(device="cuda")
     g.manual_seed(0x20A1D3 + 3)
-    stacks = torch.poisson(2e5 * psf + 10.0, generator=g).float()
+    stacks = torch.poisson(amp_t[:, None, None] * psf + bgmap, generator=g).float().contiguous()
     ctx = Z.Context(Z.optics(), Rc, np.zeros(J))
     ctx.set_stream(stream.cuda_stream)
-    # one forward+adjoint evaluation (device form), HIP events on the context stream
-    amp = torch.full((nb * S,), 2e5, dtype=torch.float64, device="cuda")
-    bg = torch.full((nb * S,), 10.0, dtype=torch.float64, device="cuda")
-    nll = torch.empty(nb * S, dtype=torch.float64, device="cuda")
-    dpar = torch.empty(nb * S, 4, device="cuda")
+    # one forward+adjoint evaluation of the full model (device form), HIP events on the context stream
+    B = nb * S
+    amp = torch.from_numpy(A.reshape(-1)).cuda()
+    pol = torch.from_numpy(poly).cuda()
+    nll = torch.empty(B, dtype=torch.float64, device="cuda")
+    dpar = torch.empty(B, 4, device="cuda")
     dA = torch.empty_like(nll)
-    dB = torch.empty_like(nll)
+    dpoly = torch.empty(B, 6, dtype=torch.float64, device="cuda")
+    dsig = torch.empty_like(nll)
     da = torch.zeros(J, dtype=torch.float64, device="cuda")
-    ctx.calib_eval_dev(P, stacks, amp, bg, nll, dpar, dA, dB, da)
+    args = (nb, S, P, stacks, amp, pol, nll, dpar, dA)
+    kw = dict(dpoly=dpoly, dsigma=dsig, da=da)
+    ctx.calib_eval_full_dev(*args, **kw)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     reps = 20
     for _ in range(reps):
-        ctx.calib_eval_dev(P, stacks, amp, bg, nll, dpar, dA, dB, da)
+        ctx.calib_eval_full_dev(*args, **kw)
     e1.record(stream)
     torch.cuda.synchronize()
     eval_ms = e0.elapsed_time(e1) / reps
-    start = beads.copy()
-    start[:, :3] = 0.0
+    # starting point as the plugin has it after the bead preparation: photon curves as
+    # patch sums minus a flat background level from the patch border
     host = stacks.cpu().numpy().reshape(nb, S, Rc, Rc)
-    start[:, 3] = host.reshape(nb, S, -1).sum(2).mean(1) - Rc * Rc * 10.0
+    c0 = np.array([np.median(np.concatenate([host[b, :, 0, :].ravel(), host[b, :, -1, :].ravel()]))
+                   for b in range(nb)])
+    A0 = host.reshape(nb, S, -1).sum(2) - Rc * Rc * c0[:, None]
+    poly0 = np.zeros((nb, 6))
+    poly0[:, 0] = c0
     mask = np.ones(J, np.int32)
     mask[:3] = 0
     t0 = time.perf_counter()
-    res = ctx.calib_fit(host, dz, start, np.zeros(J), fit_mask=mask, max_iter=500, gtol=1e-3)
+    res = ctx.calib_fit_full(host, dz, A0, np.zeros((nb, 3)), poly0, np.zeros(J), sigma=1.0, fit_mask=mask,
+                             max_iter=2000, gtol=1e-4)
     dt = time.perf_counter() - t0
     # algorithmic work of one evaluation: a forward and an adjoint pupil
     # transform per PSF plus the Zernike projection, 2 D J FLOP per PSF
@@ -134,37 +155,80 @@ def calibration_bench(torch, Z, stream, cpu=True):
     fft = 5 * 64 * 64 * np.log2(64 * 64)
     W = nb * S * (2 * fft + 2 * D * J)
     out = {
-        "workload": "5 beads x 41 slices x 31x31 px, 45 Zernike modes, L-BFGS on adjoint "
-                    "gradients (forward+adjoint per evaluation), zero start",
+        "workload": "full calibration model: 5 beads x 41 slices x 31x31 px, 45 Zernike modes, per-slice photon "
+                    "curves, order-2 backgrounds, sigma; L-BFGS on adjoint gradients (forward+adjoint per "
+                    "evaluation), zero start",
         "iterations": res["iters"], "evaluations": res["evals"],
         "iters_per_s": round(res["iters"] / dt, 2),
         "evals_per_s": round(res["evals"] / dt, 2),
         "eval_kernel_ms": round(eval_ms, 4),
         "max_coeff_err_rad": round(float(np.abs(res["coeffs"] - truth)[3:].max()), 5),
+        "sigma": round(res["sigma"], 5),
+        "max_xyz_err_nm": round(float(np.abs(res["xyz"] - xyz).max() * 1000), 3),
         "roofline": {"bound": "valu", "achieved": round(W / (eval_ms * 1e-3) / 1e12, 4), "peak": 157.3,
                      "unit": "TFLOP/s", "frac": round(W / (eval_ms * 1e-3) / 1e12 / 157.3, 5),
                      "algorithmic_flops_per_eval": float(W)},
     }
     if cpu:
-        # the reference CPU calibration re-evaluates the whole stack about 6 times
-        # per fitted coefficient per iteration (finite differences and the gamma
-        # line search, ZernikePhaseRetrieval.java:1091-1187): 6 x 42 evaluations
+        # one iteration of the reference calibration loop (ZernikePhaseRetrieval.
+        # phase_retrieve_zernike: registration, sigma, background polynomial, photon
+        # offsets and the 42 Zernike coordinates by finite differences), FP64 C
+        # restatement, the slices of a bead on all host cores, from the same start
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        m = O.Model(O.optics(), Rc, np.zeros(J))
-        reps = 2
+        threads = host_cores()
+        ref = O.CalibRef(O.optics(), np.zeros(J), host.astype(np.float64), 0.05, A0, c0, sigma=1.0,
+                         nthreads=threads)
         t0 = time.perf_counter()
-        for _ in range(reps):
-            for b in range(nb):
-                O.calib_nll(m, host[b].astype(np.float64), dz, np.full(S, beads[b, 0]), np.full(S, beads[b, 1]),
-                            np.full(S, beads[b, 2]), np.full(S, 2e5), np.full(Rc * Rc, 10.0))
-        t_eval = (time.perf_counter() - t0) / reps
-        per_iter = 6 * (J - 3)
+        ref.iteration()
+        t_it = time.perf_counter() - t0
         out["cpu_baseline"] = {
-            "value": round(1.0 / (per_iter * t_eval), 5), "unit": "iterations/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} full-stack NLL evaluations (FP64 C restatement, 5 beads x 41 slices) on one "
-                      f"core, {t_eval * 1e3:.1f} ms each, x {per_iter} evaluations per reference iteration"}
+            "value": round(1.0 / t_it, 5), "unit": "iterations/s", "cores": threads, "kind": "port",
+            "sample": f"one iteration of the reference calibration loop (FP64 C restatement of "
+                      f"ZernikePhaseRetrieval.phase_retrieve_zernike, same stacks and start), {t_it:.1f} s "
+                      f"on {threads} threads"}
+    gen.close()
+    ctx.close()
     return out
+
+
+def _cgroup_cpus():
+    """CPUs granted by the cgroup v2 quota (cpu.max), or None without a quota."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        return None if q == "max" else max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        return None
+
+
+def host_cores():
+    """Host cores this process may use: the CPU affinity set, capped by the
+    cgroup quota (the CPU legs run one thread per core)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    q = _cgroup_cpus()
+    return min(n, q) if q else n
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"cores_used": host_cores(), "affinity_cpus": aff, "cgroup_cpus": _cgroup_cpus(),
+            "cpu_count": os.cpu_count(), "cpu_model": model}
 
 
 def _events(torch, n=2):
@@ -628,7 +692,7 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_cores()
         S = args.cpu_sample or threads * 6
         S = min(S, B)
         rois = px[:S].double().cpu().numpy()
@@ -645,6 +709,7 @@ def main():
             "sample": f"first {S} ROIs of the same batch, same initial values; FP64 C "
                       f"restatement of cpu/Localization.localize + PSFPhaseMany (no PSF cache), "
                       f"{dt:.1f} s on {threads} threads",
+            "host": host_info(),
         }
     print(json.dumps(out))
 

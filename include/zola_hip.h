@@ -243,6 +243,66 @@ int  zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const dou
                     double gtol, double* out_nll, int* out_iters, int* out_evals);
 
 /* ---------------------------------------------------------------------------
+ * The reference's full calibration model (process/cpu/ZernikePhaseRetrieval.java:
+ * 1842-1879; GPU twin process/gpu/ZernikePhaseRetrieval_.java:362-474):
+ *   mu(b, s, i, ii) = psf_bs(i, ii) * amp(b, s) + background_b(i, ii) [+ scmos_b(i, ii)],
+ * replaced by 1e-8 where <= 0, NLL = sum mu - n ln mu, with
+ *   psf_bs = PSF at (x_b, y_b), z_oil = dz_s + z0_b, z_wat = 0 (computePSF 1830-1835),
+ *   amp(b, s) = A[b][s] + fit_a[b] + fit_a_each[b][s] (the bead preparation's photon
+ *     curve plus the fitted offsets, 1859-1867),
+ *   background_b = c0 + c1 i + c2 i^2 + c3 ii + c4 i ii + c5 ii^2 in patch
+ *     coordinates (PolynomialBackground of order 2, 1518-1630, same monomial order),
+ *   and the Gaussian blur width sigma (px) a free parameter (updateSigmaGaussianKernel).
+ * PSF b*S + s is slice s of bead b. */
+
+/* One forward + adjoint evaluation of the full model (device pointers, context
+ * stream).  params nb*S x 4 [x, y, z, zoil] (slices: z = 0, zoil = dz_s + z0_b),
+ * pixels nb*S x R x R, amp nb*S, poly nb x 6, offset nb x R x R or NULL.
+ * Writes nll nb*S, dparams nb*S x 4 (dNLL/dx, dy, dz, dzoil), damp nb*S, dpoly
+ * nb*S x 6 (may be NULL), dsigma nb*S (dNLL/dsigma, may be NULL) and adds
+ * dNLL/da_j to da (J, may be NULL).  Replaces the per-parameter finite
+ * differences of updatePhaseZernike, updateRegistrationStacks, updatePhotonA,
+ * updatePhotonAeach, updatePhotonBpoly and updateSigmaGaussianKernel
+ * (ZernikePhaseRetrieval.java:673-1205). */
+int  zola_calib_eval_full_dev(zola_ctx* ctx, int nb, int S, const float* params, const float* pixels,
+                              const double* amp, const double* poly, const double* offset, double* nll,
+                              float* dparams, double* damp, double* dpoly, double* dsigma, double* da);
+
+/* Parameters fitted by zola_calib_fit_full besides the Zernike coefficients */
+#define ZOLA_CALIB_FIT_REGISTRATION 1   /* bead x, y, z0 (updateRegistrationStacks)        */
+#define ZOLA_CALIB_FIT_SIGMA        2   /* blur width (updateSigmaGaussianKernel)           */
+#define ZOLA_CALIB_FIT_BACKGROUND   4   /* background polynomial (updatePhotonBpoly)        */
+#define ZOLA_CALIB_FIT_PHOTONS      8   /* fit_a, fit_a_each (updatePhotonA, ...Aeach)      */
+#define ZOLA_CALIB_FIT_ALL          15
+
+/* Bead stacks and their per-bead parameters (host arrays) as the reference holds
+ * them after InitBackgroundAndPhotonNumber (paramImage) and in its fit state. */
+typedef struct zola_calib_beads {
+    int nb, S;
+    const float* pixels;   /* nb x S x R x R photons (paramImage.image)                  */
+    const double* dz;      /* S slice defocus, um (deltaZ, ZernikePhaseRetrieval.java:131-140) */
+    const double* A;       /* nb x S photon curves (paramImage.A)                         */
+    const double* offset;  /* nb x R x R sCMOS var/gain^2 (paramImage.scmos), or NULL      */
+    double* xyz;           /* nb x 3 in/out: registration x, y, z0 (um)                    */
+    double* poly;          /* nb x 6 in/out: background (start: c0 = paramImage.B, rest 0) */
+    double* fit_a;         /* nb in/out                                                    */
+    double* fit_a_each;    /* nb x S in/out                                                */
+} zola_calib_beads;
+
+/* Phase retrieval on the full model, replacing ZernikePhaseRetrieval.run /
+ * phase_retrieve_zernike (process/cpu/ZernikePhaseRetrieval.java:227-363; GPU
+ * process/gpu/ZernikePhaseRetrieval_.java:362-474): the coefficients with
+ * fit_mask[j] = 1 (NULL: all), sigma (in/out, when ZOLA_CALIB_FIT_SIGMA) and the
+ * bead parameters selected by flags, by FP64 L-BFGS on the exact gradients of
+ * zola_calib_eval_full_dev.  Stops at max_iter iterations, when the largest
+ * scaled gradient component is below gtol, or when the relative NLL decrease
+ * falls below 1e-13.  The context's coefficients and sigma are left at the
+ * result. */
+int  zola_calib_fit_full(zola_ctx* ctx, zola_calib_beads* beads, double* sigma, double* coeffs,
+                         const int* fit_mask, int flags, int max_iter, double gtol, double* out_nll,
+                         int* out_iters, int* out_evals);
+
+/* ---------------------------------------------------------------------------
  * Frame detector: normalised FFT cross-correlation of whole frames against a
  * z-stack of PSF templates, local maxima and parabola refinement, in FP64.
  * Replaces process/cpu/Predetection.java:63-593 (the CPU path of the plugin;

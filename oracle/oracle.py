@@ -92,6 +92,15 @@ def lib():
         L.zo_search_psf_center.restype = C.c_double
         L.zo_search_psf_center.argtypes = [vp, C.c_double, C.c_double]
         L.zo_calib_nll.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]
+        L.zo_calib_ref_create.restype = vp
+        L.zo_calib_ref_create.argtypes = [C.POINTER(Optics), C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp,
+                                          C.c_double, _dp, C.c_void_p, _dp, C.c_double, C.c_int]
+        L.zo_calib_ref_free.argtypes = [vp]
+        L.zo_calib_ref_likelihood.restype = C.c_double
+        L.zo_calib_ref_likelihood.argtypes = [vp]
+        L.zo_calib_ref_iteration.restype = C.c_double
+        L.zo_calib_ref_iteration.argtypes = [vp]
+        L.zo_calib_ref_state.argtypes = [vp] + [C.c_void_p] * 6
         _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
         L.zo_bead_convert.argtypes = [C.c_long, _dp, C.c_double, C.c_double, C.c_double]
         L.zo_bead_convert_scmos.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
@@ -372,3 +381,36 @@ def bead_prepare(image, xp, yp, P, zstep, vargain=None):
                 poly=poly, coef=coef, stats=stats, size_filt=int(info[0]), miniposition=int(info[1]),
                 ndata=int(info[2]))
 
+
+
+class CalibRef:
+    """The reference CPU calibration loop (ZernikePhaseRetrieval.phase_retrieve_zernike)
+    restated in FP64 C: stacks (nb, S, R, R), A (nb, S) photon curves, B0 (nb,)
+    background levels, scmos (nb, R, R) or None; slices on nthreads threads."""
+
+    def __init__(self, opt, coeffs, stacks, zstep, A, B0, sigma=1.0, scmos=None, nthreads=1):
+        self._keep = [_d(coeffs), _d(stacks), _d(A), None if scmos is None else _d(scmos), _d(B0)]
+        self.nb, self.S, self.R = stacks.shape[0], stacks.shape[1], stacks.shape[2]
+        self.J = len(coeffs)
+        self.opt = opt
+        co, st, a, sc, b0 = self._keep
+        self._h = lib().zo_calib_ref_create(C.byref(opt), self.R, co, self.J, self.nb, self.S, st, zstep, a,
+                                            None if sc is None else sc.ctypes.data, b0, sigma, int(nthreads))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.zo_calib_ref_free(self._h)
+            self._h = None
+
+    def likelihood(self):
+        return lib().zo_calib_ref_likelihood(self._h)
+
+    def iteration(self):
+        return lib().zo_calib_ref_iteration(self._h)
+
+    def state(self):
+        co = np.zeros(self.J); reg = np.zeros((3, self.nb)); poly = np.zeros((self.nb, 6))
+        fa = np.zeros(self.nb); fae = np.zeros((self.nb, self.S)); sg = C.c_double()
+        lib().zo_calib_ref_state(self._h, co.ctypes.data, reg.ctypes.data, poly.ctypes.data, fa.ctypes.data,
+                                 fae.ctypes.data, C.addressof(sg))
+        return dict(coeffs=co, reg=reg, poly=poly, fit_a=fa, fit_a_each=fae, sigma=sg.value)

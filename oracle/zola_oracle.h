@@ -197,4 +197,17 @@ int  zo_bead_prepare(int S, int W, int H, const double* image, const double* var
 #ifdef __cplusplus
 }
 #endif
+/* FP64 restatement of the reference CPU calibration loop (zola_calib_ref.c):
+ * stacks nb x S x R x R photons, A nb x S photon curves, scmos nb x R x R or
+ * NULL, B0 nb background levels; slices on nthreads threads. */
+typedef struct zo_calib_ref zo_calib_ref;
+zo_calib_ref* zo_calib_ref_create(const zo_optics* o, int R, const double* coeffs, int J, int nb, int S,
+                                  const double* stacks, double zstep, const double* A, const double* scmos,
+                                  const double* B0, double sigma, int nthreads);
+void   zo_calib_ref_free(zo_calib_ref* c);
+double zo_calib_ref_likelihood(zo_calib_ref* c);
+double zo_calib_ref_iteration(zo_calib_ref* c);
+void   zo_calib_ref_state(const zo_calib_ref* c, double* coeffs, double* reg, double* poly, double* fa,
+                          double* fae, double* sigma);
+
 #endif

@@ -746,3 +746,61 @@ def test_scmos_frame_walk_subtracts_the_map_in_the_fit():
     for k, dxy, dz, ra in _match(locs, em):
         assert dxy < 40 and dz < 80 and 0.85 < ra < 1.15
     assert np.all(np.abs(locs[:, 5] - 20.0) < 2.0)   # background without the variance term
+
+
+def test_full_calibration_gradient_matches_central_differences():
+    """tests/calibdata.full_nll_grad, the FP64 reference of the GPU calibration
+    model, against central differences of its own NLL in every kind of variable:
+    a Zernike coefficient (h = 1e-4 rad), bead x, y, z0 (1e-6 um), each background
+    coefficient, fit_a and one fit_a_each (relative steps); a negative background
+    corner makes the clamp act.  The sigma derivative, which the GPU forms
+    analytically, is checked against the same differences in the GPU test."""
+    import calibdata as cd
+    a = np.random.default_rng(3).normal(0, 0.15, 15)
+    a[:3] = 0.0
+    a[3] = 0.8
+    m = O.Model(O.optics(), 21, a)
+    st, dz, tr = cd.full_stacks(m, 2, 5, 0.1, seed=4, photons=5e4)
+    poly = tr["poly"].copy()
+    fa, fae = np.array([300.0, -200.0]), np.random.default_rng(5).normal(0, 500, (2, 5))
+    xyz = tr["xyz"] + 0.01
+    off = np.random.default_rng(6).uniform(0.5, 3.0, (2, 21, 21))
+
+    def F(a_=a, xyz_=xyz, poly_=poly, fa_=fa, fae_=fae):
+        m.set_coeffs(a_)
+        return cd.full_nll_grad(m, st, dz, tr["A"], xyz_, poly_, fa_, fae_, off)
+
+    f0, g = F()
+    checks = []
+    for j in (3, 7, 12):
+        h = 1e-4
+        ap, am = a.copy(), a.copy()
+        ap[j] += h
+        am[j] -= h
+        checks.append((g["a"][j], (F(a_=ap)[0] - F(a_=am)[0]) / (2 * h)))
+    m.set_coeffs(a)
+    for b, k in ((0, 0), (1, 1), (0, 2)):
+        h = 1e-6
+        xp, xm = xyz.copy(), xyz.copy()
+        xp[b, k] += h
+        xm[b, k] -= h
+        checks.append((g["xyz"][b, k], (F(xyz_=xp)[0] - F(xyz_=xm)[0]) / (2 * h)))
+    for c in range(6):
+        h = 1e-4 / 21 ** [0, 1, 2, 1, 2, 2][c]
+        pp, pm = poly.copy(), poly.copy()
+        pp[1, c] += h
+        pm[1, c] -= h
+        checks.append((g["poly"][1, c], (F(poly_=pp)[0] - F(poly_=pm)[0]) / (2 * h)))
+    h = 1e-2
+    checks.append((g["fa"][0], (F(fa_=fa + [h, 0])[0] - F(fa_=fa - [h, 0])[0]) / (2 * h)))
+    e = np.zeros_like(fae)
+    e[1, 3] = h
+    checks.append((g["fae"][1, 3], (F(fae_=fae + e)[0] - F(fae_=fae - e)[0]) / (2 * h)))
+    for an, fd in checks:
+        assert abs(an - fd) <= 1e-5 * max(abs(fd), 1.0), (an, fd)
+    pc = poly.copy()
+    pc[0, 0] = -1e6
+    f1, g1 = F(poly_=pc)
+    for k in ("xyz", "poly"):
+        assert np.all(g1[k][0] == 0.0) and np.all(g1[k][1] == g[k][1])
+    assert g1["fa"][0] == 0.0 and np.all(g1["fae"][0] == 0.0)

@@ -25,10 +25,19 @@ struct AdjArgs {
     // calibration form: w = dNLL/dpsf is formed from the data
     const float* pixels;   // B x R x R photons
     const double* amp;     // B photons per PSF
-    const double* bg;      // B background per pixel
+    const double* bg;      // B background per pixel (used when poly is null)
     double* nll;           // B  sum(mu - n ln mu), mu = amp psf + bg (clamped at 1e-8)
     double* damp;          // B  dNLL/damp = sum psf (1 - n/mu)
-    double* dbg;           // B  dNLL/dbg  = sum (1 - n/mu)
+    double* dbg;           // B  dNLL/dbg  = sum (1 - n/mu), may be null
+    // full model of ZernikePhaseRetrieval.Model (1859-1867): PSF b belongs to
+    // bead b / S; its background is that bead's order-2 polynomial in the patch
+    // coordinates, c0 + c1 i + c2 i^2 + c3 ii + c4 i ii + c5 ii^2, plus an
+    // optional per-bead additive map (sCMOS var/gain^2)
+    int S;                 // slices per bead (bead index b / S)
+    const double* poly;    // nb x 6, or null
+    const double* offset;  // nb x R x R, or null
+    double* dpoly;         // B x 6: sum (1 - n/mu) x monomial, or null
+    double* dsig;          // B: dNLL/dsigma (the SIG kernels only)
 };
 
 // Stage 2 of E keeping E (row a = lane) plus the forward row blur.
@@ -55,15 +64,25 @@ __device__ __forceinline__ void stage2_E_keep64(float h1reg, const WaveScratch64
     });
 }
 
-template <int R, int WPB, bool CALIB>
+// SIG (calibration form only): also dNLL/dsigma.  The blur kernel is the
+// separable h1 (x) h1 with h1 = IDFT(G1), G1 a Gaussian of width sigma px, so
+// d(Bl)/dsigma = (hs (x) h1 + h1 (x) hs) I with hs = dh1/dsigma, and
+//   dNLL/dsigma = sum_{a,b} I(a,b) sum_j [us(a,j) h1(j - b) + u(a,j) hs(j - b)]
+// with u, us the column adjoints of dL/dBl through h1 and hs.
+template <int R, int WPB, bool CALIB, bool SIG = false>
 __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A) {
+    static_assert(CALIB || !SIG, "the sigma derivative is formed in the calibration form");
     struct Lds {
         BlockTables64 T;
+        float hs[64];
         WaveScratch64 S[WPB];
     };
     __shared__ Lds lds;
     BlockTables64& T = lds.T;
     load_tables64(T, M);
+    if constexpr (SIG) {
+        for (int t = threadIdx.x; t < 64; t += blockDim.x) lds.hs[t] = M.hs[t];
+    }
     __syncthreads();
     constexpr int G = RoiLayout<R>::G, RPL = RoiLayout<R>::RPL;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -96,33 +115,56 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
     }
     ssum = wave_sum(ssum);
     if constexpr (CALIB) {
-        // ZernikePhaseRetrieval.Model.computeLikelihood (1859-1867): mu = psf A + bg,
-        // clamped at 1e-8; w = dNLL/dpsf = A (1 - n/mu)
-        const double av = A.amp[b], bv = A.bg[b];
-        double nl = 0.0, da = 0.0, db = 0.0;
+        // ZernikePhaseRetrieval.Model.computeLikelihood (1859-1867): mu = psf A +
+        // background (+ sCMOS map), replaced by 1e-8 where it is <= 0 (there the
+        // likelihood term is constant, so its gradient is 0); w = dNLL/dpsf = A (1 - n/mu)
+        const double av = A.amp[b];
+        const int bead = (A.poly || A.offset) ? b / A.S : 0;
+        double pc[6] = {A.poly ? 0.0 : A.bg[b], 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (A.poly) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) pc[c] = A.poly[bead * 6 + c];
+        }
+        const double jd = (double)j;
+        double nl = 0.0, da = 0.0, dm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < RPL; t++) {
             const bool ok = g < G && g * RPL + t < R;
             wv[t] = 0.f;
             if (ok) {
+                const int i = g * RPL + t;
+                const double id = (double)i;
                 const double ps = (double)(fabsf(bl[t]) / ssum);
-                const double n = A.pixels[(size_t)b * R * R + (g * RPL + t) * R + j];
-                double mu = ps * av + bv;
-                if (mu <= 0) mu = 1e-8;
+                const double n = A.pixels[(size_t)b * R * R + i * R + j];
+                // monomial order of PolynomialBackground (base 3, exponents of (i, ii) with sum < 3)
+                const double mono[6] = {1.0, id, id * id, jd, id * jd, jd * jd};
+                double bgp = 0.0;
+#pragma unroll
+                for (int c = 0; c < 6; c++) bgp += pc[c] * mono[c];
+                if (A.offset) bgp += A.offset[((size_t)bead * R + i) * R + j];
+                double mu = ps * av + bgp;
+                const bool clamped = mu <= 0;
+                if (clamped) mu = 1e-8;
                 nl += mu - n * log(mu);
-                const double r = 1.0 - n / mu;
+                const double r = clamped ? 0.0 : 1.0 - n / mu;
                 da += ps * r;
-                db += r;
+#pragma unroll
+                for (int c = 0; c < 6; c++) dm[c] += r * mono[c];
                 wv[t] = (float)(av * r);
             }
         }
         nl = wave_sum(nl);
         da = wave_sum(da);
-        db = wave_sum(db);
+#pragma unroll
+        for (int c = 0; c < 6; c++) dm[c] = wave_sum(dm[c]);
         if (lane == 0) {
             A.nll[b] = nl;
             A.damp[b] = da;
-            A.dbg[b] = db;
+            if (A.dbg) A.dbg[b] = dm[0];
+            if (A.dpoly) {
+#pragma unroll
+                for (int c = 0; c < 6; c++) A.dpoly[(size_t)b * 6 + c] = dm[c];
+            }
         }
     } else {
 #pragma unroll
@@ -149,19 +191,29 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
     }
     wave_lds_fence();
     // column adjoint for this lane's grid row a: u(j) = sum_i h1[(i - R/2 - a) & 63] Bt[i][j]
-    float u[R];
+    float u[R], us[SIG ? R : 1];
     {
-        float c[R];
+        float c[R], cs[SIG ? R : 1];
 #pragma unroll
-        for (int i = 0; i < R; i++) c[i] = T.h3[64 + ((i - R / 2 - lane) & 63)];
+        for (int i = 0; i < R; i++) {
+            c[i] = T.h3[64 + ((i - R / 2 - lane) & 63)];
+            if constexpr (SIG) cs[i] = lds.hs[(i - R / 2 - lane) & 63];
+        }
 #pragma unroll
         for (int jj = 0; jj < R; jj++) {
-            float acc = 0.f;
+            float acc = 0.f, accs = 0.f;
 #pragma unroll
-            for (int i = 0; i < R; i++) acc += c[i] * Bt[i * R + jj];
+            for (int i = 0; i < R; i++) {
+                const float bt = Bt[i * R + jj];
+                acc += c[i] * bt;
+                if constexpr (SIG) accs += cs[i] * bt;
+            }
             u[jj] = acc;
+            if constexpr (SIG) us[jj] = accs;
         }
     }
+    const float hsreg = SIG ? lds.hs[lane] : 0.f;
+    float gsig = 0.f;
     // row adjoint along b, dL/dE = 2 dL/dI E, then the adjoint of the
     // stage-2 IDFT: S1bar(a, n) = sum_b dL/dE(a,b) exp(-2 pi i n b / 64)
     float sr[32], si[32];
@@ -176,13 +228,19 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
         static_for<0, 32>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             constexpr int bcol = 2 * m + HP;
-            float ib = 0.f;
+            float ib = 0.f, ibs = 0.f;
             static_for<0, R>([&](auto jc) {
                 constexpr int jj = decltype(jc)::value;
                 const float hc = __int_as_float(
                     __builtin_amdgcn_readlane(__float_as_int(h1reg), (jj - R / 2 - bcol) & 63));
                 ib += hc * u[jj];
+                if constexpr (SIG) {
+                    const float hsc = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(hsreg), (jj - R / 2 - bcol) & 63));
+                    ibs += hsc * u[jj] + hc * us[jj];
+                }
             });
+            if constexpr (SIG) gsig += ibs * (Er[HP][m] * Er[HP][m] + Ei[HP][m] * Ei[HP][m]);
             er[m] = 2.f * ib * Er[HP][m];
             ei[m] = 2.f * ib * Ei[HP][m];
         });
@@ -256,6 +314,10 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
         A.dparams[b * 4 + 1] = gy;
         A.dparams[b * 4 + 2] = gz;
         A.dparams[b * 4 + 3] = go;
+    }
+    if constexpr (SIG) {
+        gsig = wave_sum(gsig);
+        if (lane == 0) A.dsig[b] = (double)gsig;
     }
 }
 

@@ -66,11 +66,24 @@ static int compute_h1(zola_ctx* c) {
         double kk = (double)((k + s2 / 2) % s2) - (double)(s2 / 2);
         G[k] = std::exp(-.5 * kk * kk / sigpow);
     }
-    std::vector<double> h1(s2), hd(s2);
+    // cos and sin of 2 pi m / s2, m = k d mod s2 (the calibration sets sigma on every evaluation)
+    std::vector<double> ct(s2), st(s2);
+    for (int m = 0; m < s2; m++) {
+        ct[m] = std::cos(2 * PI * (double)m / s2);
+        st[m] = std::sin(2 * PI * (double)m / s2);
+    }
+    std::vector<double> h1(s2), hd(s2), hs(s2);
     for (int d = 0; d < s2; d++) {
-        double acc = 0;
-        for (int k = 0; k < s2; k++) acc += G[k] * std::cos(2 * PI * (double)k * d / s2);
+        double acc = 0, accs = 0;
+        for (int k = 0; k < s2; k++) {
+            // dG1/dsigma = -G1 kk^2 / (sigF^2 sigma)  (sigF = s2/(2 pi sigma))
+            double kk = (double)((k + s2 / 2) % s2) - (double)(s2 / 2);
+            const double cs = ct[(k * d) % s2];
+            acc += G[k] * cs;
+            accs -= G[k] * kk * kk / (sigpow * c->o.sigma) * cs;
+        }
         h1[d] = acc / s2;
+        hs[d] = accs / s2;
     }
     // Spectral derivative of the blurred image along one axis, per um of
     // emitter shift.  Shifting the emitter by x moves the (band-limited,
@@ -81,7 +94,7 @@ static int compute_h1(zola_ctx* c) {
         for (int k = 0; k < s2; k++) {
             int ks = k < s2 / 2 ? k : k - s2;
             if (k == s2 / 2) ks = 0;
-            acc -= G[k] * (2 * PI * ks / s2) * std::sin(2 * PI * (double)k * d / s2);
+            acc -= G[k] * (2 * PI * ks / s2) * st[(k * d) % s2];
         }
         hd[d] = acc / s2 / c->o.xystep;
     }
@@ -90,6 +103,7 @@ static int compute_h1(zola_ctx* c) {
         for (int d = 0; d < s2; d++) {
             c->model.h1[d] = (float)h1[d];
             c->model.hd[d] = (float)hd[d];
+            c->model.hs[d] = (float)hs[d];
         }
         return ZOLA_OK;
     }
@@ -466,6 +480,48 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
     A.nll = nll;
     A.damp = damp;
     A.dbg = dbg;
+    A.S = 1;
+    HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    return ZOLA_OK;
+}
+
+int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, const float* pixels,
+                             const double* amp, const double* poly, const double* offset, double* nll,
+                             float* dparams, double* damp, double* dpoly, double* dsigma, double* da) {
+    if (!c) return ZOLA_EINVAL;
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
+    if (nb < 0 || S <= 0 || (long)nb * S > (1L << 30)) return set_err(c, ZOLA_EINVAL, "bad batch");
+    const int B = nb * S;
+    if (B == 0) return ZOLA_OK;
+    if (!params || !pixels || !amp || !poly || !nll || !dparams || !damp)
+        return set_err(c, ZOLA_EINVAL, "null buffer");
+    float* qbar = nullptr;
+    if (da && c->J > 0) {
+        const size_t need = (size_t)B * 1024 * sizeof(float);
+        if (need > c->qbar_bytes) {
+            if (c->d_qbar) HIP_TRY(c, hipFree(c->d_qbar));
+            c->d_qbar = nullptr;
+            c->qbar_bytes = 0;
+            HIP_TRY(c, hipMalloc((void**)&c->d_qbar, need));
+            c->qbar_bytes = need;
+        }
+        qbar = c->d_qbar;
+    }
+    zola::AdjArgs A{};
+    A.B = B;
+    A.params = params;
+    A.dparams = dparams;
+    A.qbar = qbar;
+    A.pixels = pixels;
+    A.amp = amp;
+    A.nll = nll;
+    A.damp = damp;
+    A.S = S;
+    A.poly = poly;
+    A.offset = offset;
+    A.dpoly = dpoly;
+    A.dsig = dsigma;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
     if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
     return ZOLA_OK;

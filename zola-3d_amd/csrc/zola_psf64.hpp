@@ -58,6 +58,7 @@ struct Model64 {
     float kappa;       // 2 pi / (dx N): kx = kappa (n_u + 1/2)
     float h1[64];      // blur kernel over signed offset d (index d & 63)
     float hd[64];      // d/dt of the blur kernel (spectral), divided by dx
+    float hs[64];      // d h1 / d sigma (sigma in camera pixels)
 };
 
 template <int R>

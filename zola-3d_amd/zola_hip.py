@@ -34,7 +34,7 @@ SYMBOLS = [
     "zola_java_double_string",
     "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
     "zola_formats_last_error", "zola_bead_prepare", "zola_bead_prepare_scmos",
-    "zola_psf_backward_noalloc_dev",
+    "zola_psf_backward_noalloc_dev", "zola_calib_eval_full_dev", "zola_calib_fit_full",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -60,6 +60,16 @@ class FitConfig(C.Structure):
 class BeadDiag(C.Structure):
     _fields_ = [("mean0", C.c_double), ("std0", C.c_double), ("mean", C.c_double), ("std", C.c_double),
                 ("coef", C.c_double * 6), ("size_filt", C.c_int), ("miniposition", C.c_int), ("ndata", C.c_int)]
+
+
+CALIB_FIT_REGISTRATION, CALIB_FIT_SIGMA, CALIB_FIT_BACKGROUND, CALIB_FIT_PHOTONS = 1, 2, 4, 8
+CALIB_FIT_ALL = 15
+
+
+class CalibBeads(C.Structure):
+    _fields_ = [("nb", C.c_int), ("S", C.c_int), ("pixels", C.c_void_p), ("dz", C.c_void_p), ("A", C.c_void_p),
+                ("offset", C.c_void_p), ("xyz", C.c_void_p), ("poly", C.c_void_p), ("fit_a", C.c_void_p),
+                ("fit_a_each", C.c_void_p)]
 
 
 class AcceptConfig(C.Structure):
@@ -104,6 +114,10 @@ def lib():
             L.zola_calib_eval_dev.argtypes = [vp, C.c_int] + [vp] * 9
             L.zola_calib_fit.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_int,
                                          C.c_double, vp, vp, vp]
+        if hasattr(L, "zola_calib_fit_full"):
+            L.zola_calib_eval_full_dev.argtypes = [vp, C.c_int, C.c_int] + [vp] * 11
+            L.zola_calib_fit_full.argtypes = [vp, C.POINTER(CalibBeads), vp, vp, vp, C.c_int, C.c_int, C.c_double,
+                                              vp, vp, vp]
         if hasattr(L, "zola_fit_init_dev"):
             L.zola_fit_init_dev.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double, C.c_double,
                                             C.c_double, vp]
@@ -513,6 +527,52 @@ class Context:
                                          float(gtol), C.byref(nll), C.byref(it), C.byref(ev)),
                     "calib_fit")
         return dict(bead=bd, coeffs=co, nll=nll.value, iters=it.value, evals=ev.value)
+
+    def calib_eval_full_dev(self, nb, S, params, pixels, amp, poly, nll, dparams, damp, offset=None, dpoly=None,
+                            dsigma=None, da=None):
+        """The reference's full calibration model (zola_calib_eval_full_dev), device tensors;
+        PSF b*S + s is slice s of bead b."""
+        B = nb * S
+        _dev(params, "params", (B, 4), "f4")
+        _dev(pixels, "pixels", (B, self.R, self.R), "f4")
+        for name, t in (("amp", amp), ("nll", nll), ("damp", damp)):
+            _dev(t, name, (B,), "f8")
+        _dev(poly, "poly", (nb, 6), "f8")
+        _dev(dparams, "dparams", (B, 4), "f4")
+        _dev(offset, "offset", (nb, self.R, self.R), "f8", optional=True)
+        _dev(dpoly, "dpoly", (B, 6), "f8", optional=True)
+        _dev(dsigma, "dsigma", (B,), "f8", optional=True)
+        _dev(da, "da", (self.J,), "f8", optional=True)
+        self._check(lib().zola_calib_eval_full_dev(self._h, nb, S, _ptr(params), _ptr(pixels), _ptr(amp), _ptr(poly),
+                                                   _ptr(offset), _ptr(nll), _ptr(dparams), _ptr(damp), _ptr(dpoly),
+                                                   _ptr(dsigma), _ptr(da)), "calib_eval_full_dev")
+
+    def calib_fit_full(self, stacks, dz, A, xyz, poly, coeffs, sigma=1.0, fit_a=None, fit_a_each=None,
+                       offset=None, fit_mask=None, flags=CALIB_FIT_ALL, max_iter=200, gtol=1e-6):
+        """Phase retrieval on the reference's full model (ZernikePhaseRetrieval.run):
+        stacks (nb, S, R, R), dz (S,), A (nb, S) photon curves, xyz (nb, 3), poly (nb, 6),
+        offset (nb, R, R) sCMOS map or None.  Returns the fitted values."""
+        px = _host(stacks, "stacks", (None, None, self.R, self.R), "f4")
+        nb, S = px.shape[:2]
+        dz = _host(dz, "dz", (S,), "f8")
+        A = _host(A, "A", (nb, S), "f8")
+        xyz = _host(xyz, "xyz", (nb, 3), "f8").copy()
+        poly = _host(poly, "poly", (nb, 6), "f8").copy()
+        fa = np.zeros(nb) if fit_a is None else _host(fit_a, "fit_a", (nb,), "f8").copy()
+        fae = np.zeros((nb, S)) if fit_a_each is None else _host(fit_a_each, "fit_a_each", (nb, S), "f8").copy()
+        off = None if offset is None else _host(offset, "offset", (nb, self.R, self.R), "f8")
+        co = _host(coeffs, "coeffs", (self.J,), "f8").copy()
+        mask = None if fit_mask is None else _host(fit_mask, "fit_mask", (self.J,), "i4")
+        beads = CalibBeads(nb, S, px.ctypes.data, dz.ctypes.data, A.ctypes.data,
+                           off.ctypes.data if off is not None else None, xyz.ctypes.data, poly.ctypes.data,
+                           fa.ctypes.data, fae.ctypes.data)
+        sg = C.c_double(float(sigma))
+        nll, it, ev = C.c_double(), C.c_int(), C.c_int()
+        self._check(lib().zola_calib_fit_full(self._h, C.byref(beads), C.byref(sg), _ptr(co), _ptr(mask), int(flags),
+                                              int(max_iter), float(gtol), C.byref(nll), C.byref(it), C.byref(ev)),
+                    "calib_fit_full")
+        return dict(coeffs=co, sigma=sg.value, xyz=xyz, poly=poly, fit_a=fa, fit_a_each=fae, nll=nll.value,
+                    iters=it.value, evals=ev.value)
 
     def fit_init_dev(self, pixels, nZ, zmin_t, zstep_t, zfocus, init):
         B = _dev(pixels, "pixels", (None, self.R, self.R), "f4").shape[0]
