@@ -54,7 +54,11 @@ typedef struct zola_optics {
 typedef struct zola_ctx zola_ctx;
 
 /* Creates a context for ROI side R (sizeoutput) with J Zernike coefficients
- * in the reference's order (utils/Zernike.java:26-79).  Replaces
+ * in the reference's order (utils/Zernike.java:26-79).  Pupil grid N =
+ * pow2(size_fft rounded down to even) (PhaseParameters.java:278-296); blur grid
+ * s2 = pow2(min(2R, N)).  Covered: N = 64 with R in {17, 21, 31, 41}; N = 128
+ * with R in {16, 20, 32, 41} (s2 = 32, 64, 64, 128).  Other shapes return
+ * ZOLA_EUNSUPPORTED.  Replaces
  * DataPhase_(sizeFFT, json) + setSizeoutput + setMany
  * (process/gpu/DataPhase_.java:275-297) and MyCudaStream.init. */
 int  zola_ctx_create(const zola_optics* optics, int R, const double* coeffs, int J,

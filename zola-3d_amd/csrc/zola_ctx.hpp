@@ -22,7 +22,7 @@ struct zola_ctx {
     float* d_tab = nullptr;  // 4 x 1024 floats: phi, amp, kz', kzo'
     zola::Model128 model128{};
     float4* d_tab128 = nullptr;  // 64 x 64 window {phi, amp, kz', kzo'} (N = 128)
-    float* d_taps128 = nullptr;  // h1[128], hd[128] (N = 128)
+    float* d_taps128 = nullptr;  // h1[128], hd[128], kwin[64][128] (N = 128)
     std::vector<float> taps128;  // host copy of d_taps128
     hipStream_t stream = nullptr;
     bool own_stream = false;

@@ -112,17 +112,30 @@ hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const flo
     return hipGetLastError();
 }
 
-bool zola_kernels_support_roi128(int R) { return R == 41; }
+// 128 grid: the plugin's default patch (32, blur grid 64), smaller even patches
+// 16 and 20 (blur grids 32 and 64) and 41 (blur grid 128)
+#define ZOLA_R128_DISPATCH(R, CALL) \
+    switch (R) {                    \
+        case 16: return CALL(16);   \
+        case 20: return CALL(20);   \
+        case 32: return CALL(32);   \
+        case 41: return CALL(41);   \
+        default: return hipErrorInvalidValue; \
+    }
+
+bool zola_kernels_support_roi128(int R) { return R == 16 || R == 20 || R == 32 || R == 41; }
 
 hipError_t zola_launch_psf_forward128(const Model128& M, int R, int B, const float* params, float* out,
                                       hipStream_t st) {
-    if (R != 41) return hipErrorInvalidValue;
-    return launch_psf_forward128_r<41>(M, B, params, out, st);
+#define CALL(RR) launch_psf_forward128_r<RR>(M, B, params, out, st)
+    ZOLA_R128_DISPATCH(R, CALL)
+#undef CALL
 }
 
 hipError_t zola_launch_fit128(const Model128& M, int R, int mode, const FitArgs& A, hipStream_t st) {
-    if (R != 41) return hipErrorInvalidValue;
-    return launch_fit128_r<41>(M, mode, A, st);
+#define CALL(RR) launch_fit128_r<RR>(M, mode, A, st)
+    ZOLA_R128_DISPATCH(R, CALL)
+#undef CALL
 }
 
 namespace {

@@ -99,7 +99,7 @@ static int compute_h1(zola_ctx* c) {
         hd[d] = acc / s2 / c->o.xystep;
     }
     c->model_ver++;
-    if (s2 == zola::kN) {
+    if (c->N == zola::kN) {
         for (int d = 0; d < s2; d++) {
             c->model.h1[d] = (float)h1[d];
             c->model.hd[d] = (float)hd[d];
@@ -107,11 +107,37 @@ static int compute_h1(zola_ctx* c) {
         }
         return ZOLA_OK;
     }
-    // s2 = 128: all 128 taps of both kernels, uploaded by upload_taps128
-    c->taps128.resize(2 * s2);
+    // 128 grid: the s2 taps of both kernels at offsets 0 and 128 (s2 = 32, 64 or
+    // 128; the unused tail stays 0), then for s2 < 128 the crop-derivative kernel
+    // kwin[c][g] = sum_{o in [-s2/2, s2/2)} h1(c - R/2 - o) dN(o - g) (c < R, rows
+    // padded to 64 with zeros), dN the 128-periodic spectral derivative with its
+    // Nyquist bin dropped, per um; all uploaded by upload_taps128
+    const int N = zola::kN2;
+    c->taps128.assign(2 * N + 64 * N, 0.f);
     for (int d = 0; d < s2; d++) {
         c->taps128[d] = (float)h1[d];
-        c->taps128[s2 + d] = (float)hd[d];
+        c->taps128[N + d] = (float)hd[d];
+    }
+    if (s2 < N) {
+        std::vector<double> dN(N);
+        for (int d = 0; d < N; d++) {
+            double acc = 0;
+            for (int k = 0; k < N; k++) {
+                int ks = k < N / 2 ? k : k - N;
+                if (k == N / 2) ks = 0;
+                acc -= (2 * PI * ks / N) * std::sin(2 * PI * (double)((k * d) % N) / N);
+            }
+            dN[d] = acc / N / c->o.xystep;
+        }
+        const int R = c->R;
+        for (int cc = 0; cc < R; cc++)
+            for (int g = 0; g < N; g++) {
+                const int goff = g < N / 2 ? g : g - N;
+                double acc = 0;
+                for (int o = -s2 / 2; o < s2 / 2; o++)
+                    acc += h1[(((cc - R / 2 - o) % s2) + s2) % s2] * dN[(((o - goff) % N) + N) % N];
+                c->taps128[2 * N + cc * N + g] = (float)acc;
+            }
     }
     return ZOLA_OK;
 }
@@ -189,7 +215,8 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
     *out = c;
     if (N != zola::kN && N != zola::kN2)
         return set_err(c, ZOLA_EUNSUPPORTED, "pupil grid N=%d: this build covers N=64 and N=128", N);
-    if (c->s2 != N)
+    // the 64-grid kernels blur on the whole grid; the 128-grid ones on s2 = 32, 64 or 128
+    if (N == zola::kN && c->s2 != N)
         return set_err(c, ZOLA_EUNSUPPORTED, "blur grid s2=%d must equal N=%d (R >= N/4)", c->s2, N);
     if (N == zola::kN ? !zola_kernels_support_roi(R) : !zola_kernels_support_roi128(R))
         return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not instantiated for N=%d", R, N);
@@ -267,10 +294,11 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
     HIP_TRY(c, hipMalloc((void**)&c->d_queue, 8 * sizeof(int)));
     if (N == zola::kN2) {
         HIP_TRY(c, hipMalloc((void**)&c->d_tab128, zola::kW2 * zola::kW2 * sizeof(float4)));
-        HIP_TRY(c, hipMalloc((void**)&c->d_taps128, 2 * zola::kN2 * sizeof(float)));
+        HIP_TRY(c, hipMalloc((void**)&c->d_taps128, (2 + 64) * zola::kN2 * sizeof(float)));
         c->model128.tab = c->d_tab128;
         c->model128.h1 = c->d_taps128;
         c->model128.hd = c->d_taps128 + zola::kN2;
+        c->model128.kwin = c->d_taps128 + 2 * zola::kN2;
         c->model128.kappa = (float)c->kappa;
         int rc = upload_tables(c);
         if (rc) return rc;

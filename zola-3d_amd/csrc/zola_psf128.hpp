@@ -1,5 +1,7 @@
 // zola_psf128.hpp -- PSF synthesis on a 128x128 pupil grid (gfx950), ROI side
-// R in [33, 44] (blur grid s2 = 128), four waves (one block) per PSF.
+// R in [9, 44], four waves (one block) per PSF.  The blur grid is
+// s2 = pow2(min(2R, 128)) (process/cpu/PSFPhaseMany.java:102-108): 32 for R <= 16,
+// 64 for R <= 32, else 128.
 //
 // Same model as zola_psf64.hpp (process/cpu/PSFPhaseMany.java:553-622):
 // E = IDFT_128x128 of the fftshifted pupil, I = |E|^2, Gaussian blur (the exact
@@ -17,7 +19,10 @@
 //    quarters Q (each lane two quarters): the 64 inputs folded to 32 after the
 //    twiddle exp(2 pi i v Q / 128), then a 32-point IDFT.  I and
 //    2 Re(conj(E) dE/dz) over the whole grid go back to LDS.
-//  * Row blur (exact, 128 taps) only at the R crop columns: lane = row p,
+//  * For s2 < 128 the reference crops |E|^2 to the s2 x s2 window of offsets
+//    [-s2/2, s2/2) around the centre (PSFPhaseMany.java:575-582) and blurs it
+//    s2-periodically; the blurs below read only that window, with s2 taps.
+//  * Row blur (exact, s2 taps) only at the R crop columns: lane = row p,
 //    waves split the columns; the taps are broadcast from two VGPRs with
 //    readlane, grouped by tap so each coefficient is read once per chunk.
 //  * Column blur (exact) of the R crop columns: lane = ROI column j, wave w
@@ -37,8 +42,12 @@ constexpr int kI2 = 129;   // row stride of the real intensity / blur buffers
 
 struct Model128 {
     const float4* tab;   // 64 x 64 window [wu][wv]: {phi, amp, kz - kz0, kzoil - kzoil0}
-    const float* h1;     // 128 blur taps h1[d], d = 0..127 (periodic)
-    const float* hd;     // 128 taps of the spectral-derivative blur (includes 1/dx)
+    const float* h1;     // s2 blur taps h1[d], d = 0..s2-1 (periodic)
+    const float* hd;     // s2 taps of the spectral-derivative blur (includes 1/dx), s2 = 128
+    // s2 < 128: R x 128, the x (or y) tangent of the blurred crop at crop index c
+    // per unit intensity at grid index g: sum over the window offsets o of
+    // h1_s2(c - R/2 - o) dN(o - g), dN the 128-periodic spectral derivative (1/dx)
+    const float* kwin;
     float kappa;         // 2 pi / (dx N): kx = kappa (n_u + 1/2)
 };
 
@@ -51,10 +60,24 @@ struct alignas(16) Buf128 {
 
 struct NoShared {};
 
+__host__ __device__ constexpr int pow2_at_least(int x) { return x <= 1 ? 1 : 2 * pow2_at_least((x + 1) / 2); }
+
 template <int R>
 struct Roi128 {
     static constexpr int RQ = (R + 3) / 4;   // ROI rows per wave
-    static_assert(R >= 33 && R <= 4 * 11, "the 128 grid covers ROI sides 33..44");
+    static constexpr int S2 = pow2_at_least(2 * R < 128 ? 2 * R : 128);   // blur grid
+    static_assert(R >= 9 && R <= 4 * 11, "the 128 grid covers ROI sides 9..44");
+    static_assert(S2 == 32 || S2 == 64 || S2 == 128, "blur grid");
+};
+
+// First offset of chunk k (8 grid columns or rows) of the blur window, and its
+// grid index: the window is [-S2/2, S2/2) around the centre (offset o at index
+// o mod 128); for S2 = 128 the chunks simply run over the grid.  With S2/2 a
+// multiple of 8 no chunk wraps.
+template <int S2, int K>
+struct WinChunk {
+    static constexpr int o0 = S2 == 128 ? 8 * K : 8 * K - S2 / 2;
+    static constexpr int g0 = (o0 + 128) % 128;
 };
 
 __device__ __forceinline__ constexpr int slot64_n(int j) { return j < 32 ? j : j - 64; }
@@ -124,33 +147,38 @@ __device__ __forceinline__ void stage2_quarter(const Buf128& S, int p, float (&r
 // NCH = 1: (h1, I);  NCH = 3: (h1, I), (hd, I), (h1, Iz).
 template <int R, int J0, int NJ, int NCH>
 __device__ __forceinline__ void row_blur_full(const float* Ib, const float* Izb, int p, const float (&h1)[2],
-                                              const float (&hd)[2], float (&acc)[NCH][NJ]) {
+                                              const float (&hd)[2], const float* __restrict__ kwin,
+                                              float (&acc)[NCH][NJ]) {
+    constexpr int S2 = Roi128<R>::S2;
+    // the derivative channel goes through hd when the window is the whole grid
+    constexpr bool HD = NCH == 3 && S2 == 128;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
 #pragma unroll
         for (int jj = 0; jj < NJ; jj++) acc[c][jj] = 0.f;
-    static_for<0, 16>([&](auto qcc) {
-        constexpr int q0 = decltype(qcc)::value * 8;
+    static_for<0, S2 / 8>([&](auto qcc) {
+        constexpr int q0 = WinChunk<S2, decltype(qcc)::value>::o0;   // column offset
+        constexpr int g0 = WinChunk<S2, decltype(qcc)::value>::g0;   // its grid column
         float iv[8], izv[NCH == 3 ? 8 : 1];
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            iv[t] = Ib[p * kI2 + q0 + t];
-            if constexpr (NCH == 3) izv[t] = Izb[p * kI2 + q0 + t];
+            iv[t] = Ib[p * kI2 + g0 + t];
+            if constexpr (NCH == 3) izv[t] = Izb[p * kI2 + g0 + t];
         }
         constexpr int dmin = J0 - R / 2 - q0 - 7;
         static_for<0, NJ + 7>([&](auto ddc) {
             constexpr int d = dmin + decltype(ddc)::value;
-            constexpr int dm = ((d % 128) + 128) % 128;
+            constexpr int dm = ((d % S2) + S2) % S2;
             const float c1 = tap<dm>(h1);
             float cd = 0.f;
-            if constexpr (NCH == 3) cd = tap<dm>(hd);
+            if constexpr (HD) cd = tap<dm>(hd);
             static_for<0, 8>([&](auto tc) {
                 constexpr int t = decltype(tc)::value;
                 constexpr int jj = d + R / 2 + q0 + t - J0;
                 if constexpr (jj >= 0 && jj < NJ) {
                     acc[0][jj] += c1 * iv[t];
                     if constexpr (NCH == 3) {
-                        acc[1][jj] += cd * iv[t];
+                        if constexpr (HD) acc[1][jj] += cd * iv[t];
                         acc[2][jj] += c1 * izv[t];
                     }
                 }
@@ -158,6 +186,20 @@ __device__ __forceinline__ void row_blur_full(const float* Ib, const float* Izb,
             __builtin_amdgcn_sched_barrier(0);
         });
     });
+    if constexpr (NCH == 3 && !HD) {
+        // derivative of the crop: every grid column of the row through kwin
+        static_for<0, 16>([&](auto qcc) {
+            constexpr int g0 = decltype(qcc)::value * 8;
+            float iv[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) iv[t] = Ib[p * kI2 + g0 + t];
+#pragma unroll
+            for (int jj = 0; jj < NJ; jj++)
+#pragma unroll
+                for (int t = 0; t < 8; t++) acc[1][jj] += kwin[(J0 + jj) * kN2 + g0 + t] * iv[t];
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    }
 }
 
 // Column blur of ROI column j at ROI rows I0 .. I0+NI-1:
@@ -166,37 +208,41 @@ __device__ __forceinline__ void row_blur_full(const float* Ib, const float* Izb,
 // NO = 4: (h1, Rb0), (hd, Rb0), (h1, Rb1), (h1, Rb2).
 template <int R, int I0, int NI, int NO>
 __device__ __forceinline__ void column_blur_full(const float* Rb, int j, const float (&h1)[2], const float (&hd)[2],
-                                                 float (&o0)[NI], float (&o1)[NI], float (&o2)[NI], float (&o3)[NI]) {
+                                                 const float* __restrict__ kwin, float (&o0)[NI], float (&o1)[NI],
+                                                 float (&o2)[NI], float (&o3)[NI]) {
+    constexpr int S2 = Roi128<R>::S2;
+    constexpr bool HD = NO == 4 && S2 == 128;
 #pragma unroll
     for (int ii = 0; ii < NI; ii++) {
         o0[ii] = 0.f;
         if constexpr (NO == 4) o1[ii] = o2[ii] = o3[ii] = 0.f;
     }
-    static_for<0, 16>([&](auto pcc) {
-        constexpr int p0 = decltype(pcc)::value * 8;
+    static_for<0, S2 / 8>([&](auto pcc) {
+        constexpr int p0 = WinChunk<S2, decltype(pcc)::value>::o0;   // row offset
+        constexpr int g0 = WinChunk<S2, decltype(pcc)::value>::g0;   // its grid row
         float r0[8], r1[NO == 4 ? 8 : 1], r2[NO == 4 ? 8 : 1];
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            r0[t] = Rb[(0 * R + j) * kI2 + p0 + t];
+            r0[t] = Rb[(0 * R + j) * kI2 + g0 + t];
             if constexpr (NO == 4) {
-                r1[t] = Rb[(1 * R + j) * kI2 + p0 + t];
-                r2[t] = Rb[(2 * R + j) * kI2 + p0 + t];
+                r1[t] = Rb[(1 * R + j) * kI2 + g0 + t];
+                r2[t] = Rb[(2 * R + j) * kI2 + g0 + t];
             }
         }
         constexpr int dmin = I0 - R / 2 - p0 - 7;
         static_for<0, NI + 7>([&](auto ddc) {
             constexpr int d = dmin + decltype(ddc)::value;
-            constexpr int dm = ((d % 128) + 128) % 128;
+            constexpr int dm = ((d % S2) + S2) % S2;
             const float c1 = tap<dm>(h1);
             float cd = 0.f;
-            if constexpr (NO == 4) cd = tap<dm>(hd);
+            if constexpr (HD) cd = tap<dm>(hd);
             static_for<0, 8>([&](auto tc) {
                 constexpr int t = decltype(tc)::value;
                 constexpr int ii = d + R / 2 + p0 + t - I0;
                 if constexpr (ii >= 0 && ii < NI) {
                     o0[ii] += c1 * r0[t];
                     if constexpr (NO == 4) {
-                        o1[ii] += cd * r0[t];
+                        if constexpr (HD) o1[ii] += cd * r0[t];
                         o2[ii] += c1 * r1[t];
                         o3[ii] += c1 * r2[t];
                     }
@@ -205,6 +251,20 @@ __device__ __forceinline__ void column_blur_full(const float* Rb, int j, const f
             __builtin_amdgcn_sched_barrier(0);
         });
     });
+    if constexpr (NO == 4 && !HD) {
+        // x tangent: every grid row of the row-blurred value through kwin
+        static_for<0, 16>([&](auto pcc) {
+            constexpr int g0 = decltype(pcc)::value * 8;
+            float r0[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) r0[t] = Rb[(0 * R + j) * kI2 + g0 + t];
+#pragma unroll
+            for (int ii = 0; ii < NI; ii++)
+#pragma unroll
+                for (int t = 0; t < 8; t++) o1[ii] += kwin[(I0 + ii) * kN2 + g0 + t] * r0[t];
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    }
 }
 
 __device__ __forceinline__ void load_taps128(const Model128& M, int lane, float (&h1)[2], float (&hd)[2]) {
@@ -282,8 +342,8 @@ __device__ __forceinline__ void eval128(const Model128& M, Buf128 (&S)[2], int l
     // ---- row blur at the crop columns: waves 0,1 columns [0, JA), waves 2,3 [JA, R)
     constexpr int JA = (R + 1) / 2, JB = R - JA;
     float ra[NCH][JA], rb[NCH][JB];
-    if ((wr >> 1) == 0) row_blur_full<R, 0, JA, NCH>(Ib, Izb, p, h1, hd, ra);
-    else row_blur_full<R, JA, JB, NCH>(Ib, Izb, p, h1, hd, rb);
+    if ((wr >> 1) == 0) row_blur_full<R, 0, JA, NCH>(Ib, Izb, p, h1, hd, M.kwin, ra);
+    else row_blur_full<R, JA, JB, NCH>(Ib, Izb, p, h1, hd, M.kwin, rb);
     __syncthreads();  // all intensity reads are done
     float* Rb = S[0].re;  // [channel][column][row]
 #pragma unroll
@@ -301,10 +361,10 @@ __device__ __forceinline__ void eval128(const Model128& M, Buf128 (&S)[2], int l
     const int j = lane < R ? lane : 0;
     constexpr int NO = TANG ? 4 : 1;
     switch (wr) {
-        case 0: column_blur_full<R, 0 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
-        case 1: column_blur_full<R, 1 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
-        case 2: column_blur_full<R, 2 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
-        default: column_blur_full<R, 3 * RQ, RQ, NO>(Rb, j, h1, hd, vB, vX, vY, vZ); break;
+        case 0: column_blur_full<R, 0 * RQ, RQ, NO>(Rb, j, h1, hd, M.kwin, vB, vX, vY, vZ); break;
+        case 1: column_blur_full<R, 1 * RQ, RQ, NO>(Rb, j, h1, hd, M.kwin, vB, vX, vY, vZ); break;
+        case 2: column_blur_full<R, 2 * RQ, RQ, NO>(Rb, j, h1, hd, M.kwin, vB, vX, vY, vZ); break;
+        default: column_blur_full<R, 3 * RQ, RQ, NO>(Rb, j, h1, hd, M.kwin, vB, vX, vY, vZ); break;
     }
     __syncthreads();  // the buffers are reused by the next evaluation
 }
