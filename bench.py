@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--no-formats", action="store_true")
     p.add_argument("--no-beads", action="store_true")
     p.add_argument("--no-host", action="store_true")
-    p.add_argument("--n128-batch", type=int, default=65536)
+    p.add_argument("--n128-batch", type=int, default=1_000_000)
     p.add_argument("--mixed-total", type=int, default=200000)
     p.add_argument("--frames", type=int, default=2000)
     p.add_argument("--cpu-sample", type=int, default=0, help="ROIs for the CPU baseline (0: auto)")
@@ -264,7 +264,9 @@ def depth128_bench(torch, Z, stream, B):
     it = torch.empty(B, dtype=torch.int32, device="cuda")
     st = torch.empty(B, dtype=torch.int32, device="cuda")
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-3.3, zmax=3.3, tol=1e-4)
-    ctx.fit_zoil_dev(cfg, px, init, zoil, th, nl, cr, it, st)
+    # warm-up on a slice (kernel load), then the whole batch in one persistent pass
+    Bw = min(B, 8192)
+    ctx.fit_zoil_dev(cfg, px[:Bw], init[:Bw], zoil[:Bw], th[:Bw], nl[:Bw], cr[:Bw], it[:Bw], st[:Bw])
     e = _events(torch)
     e[0].record(stream)
     ctx.fit_zoil_dev(cfg, px, init, zoil, th, nl, cr, it, st)
@@ -283,7 +285,7 @@ def depth128_bench(torch, Z, stream, B):
         "median_abs_err_vs_truth_nm": [round(float(v), 3) for v in np.median(err, axis=0)],
         "median_crlb_nm": [round(float(v), 3) for v in np.median(cr.cpu().numpy() * 1000, axis=0)],
     }
-    Bp = max(1, B // 8)
+    Bp = max(1, min(B // 8, 8192))
     cfgp = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=-3.3, zmax=3.3, photon_threshold=0.0)
     args = (px[:Bp].contiguous(), init[:Bp].contiguous(), zoil[:Bp].contiguous(), th[:Bp], nl[:Bp], cr[:Bp],
             it[:Bp], st[:Bp])
@@ -405,11 +407,19 @@ def frames_bench(torch, Z, stream, F):
     t0 = time.perf_counter()
     lists, cnt = det.detect(host, cap=8192)
     t_det = time.perf_counter() - t0
+    # detection roofline: per frame one forward transform of the padded frame and
+    # one inverse per template, FX x FY complex FP64, 5 N log2 N flops each
+    # (windowed statistics, products and local maxima excluded)
+    fl_det = (1 + det.nZ) * 5.0 * det.FX * det.FY * np.log2(det.FX * det.FY)
+    ach = fl_det * F / t_det / 1e12
     out = {"workload": f"{F} frames 256x256, 0.5 emitters/um^2 ({n} emitters), tetrapod 45 modes, 128x128 "
                        f"pupil, {det.nZ} templates at 100 nm, 41x41 patches",
            "frames": F, "emitters": n, "templates": det.nZ,
            "detect_ms_per_frame": round(t_det / F * 1e3, 3),
-           "candidates_per_frame": round(float(np.mean(cnt)), 1)}
+           "candidates_per_frame": round(float(np.mean(cnt)), 1),
+           "detect_roofline": {"bound": "valu-fp64", "achieved": round(ach, 4), "peak": 78.6, "unit": "TFLOP/s",
+                               "frac": round(ach / 78.6, 5), "algorithmic_flops_per_frame": float(fl_det),
+                               "note": "host-array call, frame upload and candidate download included"}}
     acc = det.accept_config(photon_threshold=500.0, min_z=-3.0, max_z=3.0)
     for mode, name in ((Z.FIT_LM, "lm"), (Z.FIT_REFERENCE, "plain")):
         fit = ctx.fit_config(mode=mode, max_iter=30 if mode == Z.FIT_LM else 20, zmin=-3.3, zmax=3.3, tol=1e-4,
