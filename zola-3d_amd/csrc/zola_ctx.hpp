@@ -56,6 +56,13 @@ struct zola_ctx {
 // entry points for the current device (once per device) into c->scratch.
 int zola_scratch_setup(zola_ctx* c);
 
+// Launches without argument checks, on stream st (which may be capturing): the
+// PSF synthesis of zola_psf_forward_dev and the fit of zola_fit_batch_ext_dev.
+int zola_psf_launch(zola_ctx* c, int B, const float* params, float* out, hipStream_t st);
+int zola_fit_launch(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels, const double* init,
+                    const float* zoil, const float* varg, double* theta, double* nll, double* crlb, int* iters,
+                    int* status, hipStream_t st);
+
 // Four waves per ROI (1) or one wave per ROI (0) for a 64-grid fit.
 int zola_fit_quad(const zola_ctx* c, int R, int mode);
 

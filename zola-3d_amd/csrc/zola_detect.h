@@ -68,5 +68,8 @@ hipError_t zola_launch_model_stats(int B, int R, const float* psf, const float* 
 hipError_t zola_launch_gather_map(int W, int H, int R, int B, const int* roi, const float* map, float* out,
                                   hipStream_t st);
 
+// PSF parameters (B x 4 floats [x, y, z, zfocus]) of fitted theta (B x 5 doubles).
+hipError_t zola_launch_theta_params(int B, const double* th, double zfocus, float* par, hipStream_t st);
+
 // FP64 twiddle table W_4096^a = (cos, sin)(2 pi a / 4096), a = 0..4095.
 void zola_twiddle_table(double2* host4096);

@@ -523,6 +523,25 @@ hipError_t zola_launch_gather_map(int W, int H, int R, int B, const int* roi, co
     return hipGetLastError();
 }
 
+namespace {
+// PSF parameters [x, y, z, zoil] (float) of fitted theta [A, B, x, y, z]: the
+// model of the last likelihood (Localization.getPSF) at the fit's focus term.
+__global__ void k_theta_params(int B, const double* __restrict__ th, float zfocus, float* __restrict__ par) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    par[b * 4 + 0] = (float)th[b * 5 + 2];
+    par[b * 4 + 1] = (float)th[b * 5 + 3];
+    par[b * 4 + 2] = (float)th[b * 5 + 4];
+    par[b * 4 + 3] = zfocus;
+}
+}  // namespace
+
+hipError_t zola_launch_theta_params(int B, const double* th, double zfocus, float* par, hipStream_t st) {
+    if (B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_theta_params, dim3((B + 255) / 256), dim3(256), 0, st, B, th, (float)zfocus, par);
+    return hipGetLastError();
+}
+
 void zola_twiddle_table(double2* t) {
     const double pi = 3.14159265358979323846;
     for (int a = 0; a < kTwN; a++) {

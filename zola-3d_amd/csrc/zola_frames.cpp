@@ -147,6 +147,17 @@ struct zola_detector {
     int locf_fb = 0;
     float* d_vmap = nullptr;
     float* d_vpx = nullptr;
+    // one fit round as a graph (captured on cap_st, replayed on the context stream)
+    // and its pinned host staging: roi/init in, theta/nll/crlb/status/model
+    // statistics/mask bits out
+    hipStream_t cap_st = nullptr;
+    hipGraphExec_t round_exec = nullptr;
+    int* h_roi = nullptr;
+    double* h_init = nullptr;
+    double* h_res = nullptr;      // per fit: theta 5, nll, crlb 3
+    int* h_st = nullptr;
+    double* h_mstat = nullptr;
+    uint32_t* h_bits = nullptr;
 };
 
 namespace {
@@ -169,10 +180,22 @@ void free_cand(zola_detector* d) {
     d->cap_fb = 0;
 }
 
+template <class T>
+void hfree(T*& p) {
+    if (p) (void)hipHostFree((void*)p);
+    p = nullptr;
+}
+
 void free_fit(zola_detector* d) {
+    if (d->round_exec) {
+        (void)hipStreamSynchronize(d->ctx->stream);
+        (void)hipGraphExecDestroy(d->round_exec);
+        d->round_exec = nullptr;
+    }
     dfree(d->d_roi); dfree(d->d_init); dfree(d->d_th); dfree(d->d_nll); dfree(d->d_cr);
     dfree(d->d_it); dfree(d->d_st); dfree(d->d_px); dfree(d->d_par); dfree(d->d_psf);
     dfree(d->d_mstat); dfree(d->d_mbits); dfree(d->d_vpx);
+    hfree(d->h_roi); hfree(d->h_init); hfree(d->h_res); hfree(d->h_st); hfree(d->h_mstat); hfree(d->h_bits);
     d->fit_cap = 0;
 }
 
@@ -350,6 +373,7 @@ int zola_detector_destroy(zola_detector* d) {
     dfree(d->d_norm);
     dfree(d->d_locf);
     dfree(d->d_vmap);
+    if (d->cap_st) (void)hipStreamDestroy(d->cap_st);
     delete d;
     return ZOLA_OK;
 }
@@ -700,6 +724,14 @@ int ensure_fit(zola_detector* d, int B) {
     HIP_TRY(c, hipMalloc((void**)&d->d_mstat, (size_t)B * 2 * sizeof(double)));
     HIP_TRY(c, hipMalloc((void**)&d->d_mbits, (size_t)B * ((R2 + 31) / 32) * sizeof(uint32_t)));
     HIP_TRY(c, hipMalloc((void**)&d->d_vpx, (size_t)B * R2 * sizeof(float)));
+    HIP_TRY(c, hipHostMalloc((void**)&d->h_roi, (size_t)B * 3 * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(c, hipHostMalloc((void**)&d->h_init, (size_t)B * 5 * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(c, hipHostMalloc((void**)&d->h_res, (size_t)B * 9 * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(c, hipHostMalloc((void**)&d->h_st, (size_t)B * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(c, hipHostMalloc((void**)&d->h_mstat, (size_t)B * 2 * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(c, hipHostMalloc((void**)&d->h_bits, (size_t)B * ((R2 + 31) / 32) * sizeof(uint32_t),
+                             hipHostMallocDefault));
+    if (!d->cap_st) HIP_TRY(c, hipStreamCreateWithFlags(&d->cap_st, hipStreamNonBlocking));
     d->fit_cap = B;
     return ZOLA_OK;
 }
@@ -729,44 +761,77 @@ int fit_queue(zola_detector* d, Pipe& P, std::vector<FrameRun>& runs) {
         init[b * 5 + 3] = -cv[6];
         init[b * 5 + 4] = d->range[cand_r(fr, k)] + cv[4];
     }
-    HIP_TRY(c, hipMemcpyAsync(d->d_roi, roi.data(), roi.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    HIP_TRY(c, hipMemcpyAsync(d->d_init, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_TRY(c, zola_launch_gather_rois(P.W, P.H, P.R, B, d->d_roi, P.vmap ? d->d_locf : d->d_frames, d->d_px, st));
-    if (P.vmap) HIP_TRY(c, zola_launch_gather_map(P.W, P.H, P.R, B, d->d_roi, d->d_vmap, d->d_vpx, st));
-    rc = zola_fit_batch_ext_dev(c, P.fit, B, d->d_px, d->d_init, nullptr, P.vmap ? d->d_vpx : nullptr, d->d_th,
-                                d->d_nll, d->d_cr, d->d_it, d->d_st);
-    if (rc) return rc;
-    std::vector<double> th((size_t)B * 5), nll(B), cr((size_t)B * 3);
-    std::vector<int> stt(B);
-    HIP_TRY(c, hipMemcpyAsync(th.data(), d->d_th, th.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipMemcpyAsync(nll.data(), d->d_nll, nll.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipMemcpyAsync(cr.data(), d->d_cr, cr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipMemcpyAsync(stt.data(), d->d_st, stt.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipStreamSynchronize(st));
-    // model PSF at the fitted parameters (Localization.getPSF: the last likelihood's model)
-    std::vector<float> par((size_t)B * 4);
-    for (int b = 0; b < B; b++) {
-        par[b * 4 + 0] = (float)th[b * 5 + 2];
-        par[b * 4 + 1] = (float)th[b * 5 + 3];
-        par[b * 4 + 2] = (float)th[b * 5 + 4];
-        par[b * 4 + 3] = (float)P.fit->zfocus;
-    }
-    HIP_TRY(c, hipMemcpyAsync(d->d_par, par.data(), par.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    rc = zola_psf_forward_dev(c, B, d->d_par, d->d_psf);
-    if (rc) return rc;
-    // only the model statistics the walk uses come back, not the PSFs
+    std::memcpy(d->h_roi, roi.data(), roi.size() * sizeof(int));
+    std::memcpy(d->h_init, init.data(), init.size() * sizeof(double));
+    // The round's device work as one graph, captured on a private stream and
+    // replayed on the context stream: ROI centres and starts in, patch gathers,
+    // the fit, the model at the fitted parameters (Localization.getPSF: the last
+    // likelihood's model) and the statistics the walk needs (max, score, mask
+    // bits; not the PSFs), results out.  One synchronisation per round.
     const int nw = (int)((R2 + 31) / 32);
-    HIP_TRY(c, zola_launch_model_stats(B, P.R, d->d_psf, d->d_px, P.vmap ? d->d_vpx : nullptr, d->d_th,
-                                       P.acc->model_mask_frac, d->d_mstat,
-                                       d->d_mstat + B, d->d_mbits, st));
-    std::vector<double> mstat((size_t)B * 2);
+    hipStream_t cs = d->cap_st;
+    HIP_TRY(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    auto record = [&]() -> int {
+        HIP_TRY(c, hipMemcpyAsync(d->d_roi, d->h_roi, (size_t)B * 3 * sizeof(int), hipMemcpyHostToDevice, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->d_init, d->h_init, (size_t)B * 5 * sizeof(double), hipMemcpyHostToDevice, cs));
+        HIP_TRY(c, zola_launch_gather_rois(P.W, P.H, P.R, B, d->d_roi, P.vmap ? d->d_locf : d->d_frames, d->d_px, cs));
+        if (P.vmap) HIP_TRY(c, zola_launch_gather_map(P.W, P.H, P.R, B, d->d_roi, d->d_vmap, d->d_vpx, cs));
+        int r = zola_fit_launch(c, P.fit, B, d->d_px, d->d_init, nullptr, P.vmap ? d->d_vpx : nullptr, d->d_th,
+                                d->d_nll, d->d_cr, d->d_it, d->d_st, cs);
+        if (r) return r;
+        HIP_TRY(c, zola_launch_theta_params(B, d->d_th, P.fit->zfocus, d->d_par, cs));
+        r = zola_psf_launch(c, B, d->d_par, d->d_psf, cs);
+        if (r) return r;
+        HIP_TRY(c, zola_launch_model_stats(B, P.R, d->d_psf, d->d_px, P.vmap ? d->d_vpx : nullptr, d->d_th,
+                                           P.acc->model_mask_frac, d->d_mstat, d->d_mstat + B, d->d_mbits, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->h_res, d->d_th, (size_t)B * 5 * sizeof(double), hipMemcpyDeviceToHost, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->h_res + (size_t)B * 5, d->d_nll, (size_t)B * sizeof(double),
+                                  hipMemcpyDeviceToHost, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->h_res + (size_t)B * 6, d->d_cr, (size_t)B * 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->h_st, d->d_st, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->h_mstat, d->d_mstat, (size_t)B * 2 * sizeof(double), hipMemcpyDeviceToHost, cs));
+        HIP_TRY(c, hipMemcpyAsync(d->h_bits, d->d_mbits, (size_t)B * nw * sizeof(uint32_t), hipMemcpyDeviceToHost, cs));
+        return ZOLA_OK;
+    };
+    rc = record();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &graph);
+    if (rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc ? rc : set_err(c, ZOLA_EHIP, "fit-round capture: %s", hipGetErrorString(ec));
+    }
+    // the previous round has been synchronised: reuse its executable when only
+    // the parameters changed, else instantiate
+    bool updated = false;
+    if (d->round_exec) {
+        hipGraphExecUpdateResult res;
+        hipGraphNode_t bad = nullptr;
+        updated = hipGraphExecUpdate(d->round_exec, graph, &bad, &res) == hipSuccess &&
+                  res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGraphExecDestroy(d->round_exec);
+            d->round_exec = nullptr;
+        }
+    }
+    hipError_t eg = hipSuccess;
+    if (!updated) eg = hipGraphInstantiate(&d->round_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (eg != hipSuccess) {
+        d->round_exec = nullptr;
+        return set_err(c, ZOLA_EHIP, "fit-round graph: %s", hipGetErrorString(eg));
+    }
+    HIP_TRY(c, hipGraphLaunch(d->round_exec, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    const double* th = d->h_res;
+    const double* nll = d->h_res + (size_t)B * 5;
+    const double* cr = d->h_res + (size_t)B * 6;
+    const int* stt = d->h_st;
+    const double* mstat = d->h_mstat;
     const size_t base = P.bits.size();
     P.nw = nw;
     P.bits.resize(base + (size_t)B * nw);
-    HIP_TRY(c, hipMemcpyAsync(mstat.data(), d->d_mstat, mstat.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipMemcpyAsync(P.bits.data() + base, d->d_mbits, (size_t)B * nw * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, st));
-    HIP_TRY(c, hipStreamSynchronize(st));
+    std::memcpy(P.bits.data() + base, d->h_bits, (size_t)B * nw * sizeof(uint32_t));
     for (int b = 0; b < B; b++) {
         FitRes r;
         std::memcpy(r.th, &th[b * 5], 5 * sizeof(double));

@@ -188,6 +188,40 @@ int zola_fit_quad(const zola_ctx* c, int R, int mode) {
     return R >= 31 ? 1 : 0;  // one wave holds ceil(R/3) or more ROI rows per lane above 21
 }
 
+int zola_psf_launch(zola_ctx* c, int B, const float* params, float* out, hipStream_t st) {
+    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_psf_forward128(c->model128, c->R, B, params, out, st));
+    else HIP_TRY(c, zola_launch_psf_forward64(c->model, c->R, B, params, out, st));
+    return ZOLA_OK;
+}
+
+int zola_fit_launch(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels, const double* init,
+                    const float* zoil, const float* varg, double* theta, double* nll, double* crlb, int* iters,
+                    int* status, hipStream_t st) {
+    zola::FitArgs A{};
+    A.B = B;
+    A.max_iter = cfg->max_iter;
+    A.xystep = (float)c->o.xystep;
+    A.zmin = cfg->zmin;
+    A.zmax = cfg->zmax;
+    A.zfocus = cfg->zfocus;
+    A.thr = cfg->photon_threshold;
+    A.tol = cfg->tol;
+    A.noise_rel = 1e-8;
+    A.pixels = pixels;
+    A.init = init;
+    A.zoil = zoil;
+    A.varg = varg;
+    A.queue = c->d_queue;
+    A.theta = theta;
+    A.nll = nll;
+    A.crlb = crlb;
+    A.iters = iters;
+    A.status = status;
+    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, st));
+    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, zola_fit_quad(c, c->R, cfg->mode), A, st));
+    return ZOLA_OK;
+}
+
 extern "C" {
 
 int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zola_ctx** out) {
@@ -390,9 +424,7 @@ int zola_psf_forward_dev(zola_ctx* c, int B, const float* params, float* out) {
     if (!c) return ZOLA_EINVAL;
     if (B < 0 || (B > 0 && (!params || !out))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
-    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_psf_forward128(c->model128, c->R, B, params, out, c->stream));
-    else HIP_TRY(c, zola_launch_psf_forward64(c->model, c->R, B, params, out, c->stream));
-    return ZOLA_OK;
+    return zola_psf_launch(c, B, params, out, c->stream);
 }
 
 int zola_psf_forward(zola_ctx* c, int B, const float* params, float* out) {
@@ -586,29 +618,7 @@ int zola_fit_batch_ext_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const
         return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", cfg->mode);
     if (cfg->max_iter < 0 || !(cfg->zmax >= cfg->zmin))
         return set_err(c, ZOLA_EINVAL, "bad iteration count or z range");
-    zola::FitArgs A{};
-    A.B = B;
-    A.max_iter = cfg->max_iter;
-    A.xystep = (float)c->o.xystep;
-    A.zmin = cfg->zmin;
-    A.zmax = cfg->zmax;
-    A.zfocus = cfg->zfocus;
-    A.thr = cfg->photon_threshold;
-    A.tol = cfg->tol;
-    A.noise_rel = 1e-8;
-    A.pixels = pixels;
-    A.init = init;
-    A.zoil = zoil;
-    A.varg = varg;
-    A.queue = c->d_queue;
-    A.theta = theta;
-    A.nll = nll;
-    A.crlb = crlb;
-    A.iters = iters;
-    A.status = status;
-    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, c->stream));
-    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, zola_fit_quad(c, c->R, cfg->mode), A, c->stream));
-    return ZOLA_OK;
+    return zola_fit_launch(c, cfg, B, pixels, init, zoil, varg, theta, nll, crlb, iters, status, c->stream);
 }
 
 int zola_set_fit_layout(zola_ctx* c, int R, int mode, int layout) {
