@@ -153,12 +153,16 @@ def test_frame_walk_lm_form_finds_same_emitters(models):
     det.close()
 
 
-def test_search_psf_center_matches_oracle(torch_cuda):
-    print("case: focus centre, 64 grid, astigmatic", flush=True)
+@pytest.mark.parametrize("N,Rc,cases", [(64, R, ((0.0, 1.0), (1.5, 0.8))), (128, 32, ((0.0, 1.0), (0.6, 0.8)))])
+def test_search_psf_center_matches_oracle(torch_cuda, N, Rc, cases):
+    """SearchPSFcenter: the FP64 device Fisher matrices of every scan, inverted
+    on the host, pick the same depth as the oracle's host search (an argmin over
+    hundreds of CRLB values, so the whole chain must agree)."""
+    print(f"case: focus centre, {N} grid, astigmatic", flush=True)
     coef = Z.astigmatic_coeffs(28)
-    ctx = Z.Context(Z.optics(size_fft=64, nwat=1.33), R, coef)
-    m = O.Model(O.optics(size_fft=64, nwat=1.33), 64, coef)
-    for zf, ax in ((0.0, 1.0), (1.5, 0.8)):
+    ctx = Z.Context(Z.optics(size_fft=N, nwat=1.33), Rc, coef)
+    m = O.Model(O.optics(size_fft=N, nwat=1.33), N, coef)
+    for zf, ax in cases:
         c = ctx.search_psf_center(zf, ax)
         ref = O.search_psf_center(m, zf, ax)
         print("centre", zf, ax, c, ref)

@@ -1,5 +1,5 @@
 // zola_center.cpp -- the focus centre of the template range
-// (process/cpu/SearchPSFcenter.java:37-81, 86-214, 219-351), FP64 on the host.
+// (process/cpu/SearchPSFcenter.java:37-81, 86-214, 219-351), in FP64.
 //
 // The reference picks the z (depth) at which the lateral CRLB of a standard
 // emitter (3000 photons, background 50, PSF cropped to the full FFT size) is
@@ -8,10 +8,10 @@
 // 1e-5 um, then the start of the window of width range/2 with the smallest
 // summed CRLB, scanned at one pixel.  Each CRLB is the reference's FD Fisher
 // matrix (h = 5 nm; the photon derivative taken on the model with step
-// 100 h, as computeFisher does), inverted by cofactors.  Setup-time work: the
-// PSFs of one scan are computed in parallel on the host cores.
+// 100 h, as computeFisher does), inverted by cofactors.  The seven FP64 PSFs
+// per depth and the Fisher matrices of a whole scan are one batch on the GPU
+// (zola_center_dev.hip); the cofactor inverses and the scan logic stay here.
 #include <cmath>
-#include <thread>
 #include <vector>
 
 #include "../../include/zola_hip.h"
@@ -54,56 +54,28 @@ void inverse5(const double* a, double* inv) {
 }
 
 struct Search {
-    const zola_ctx* c;
+    zola_ctx* c;
     int R;
     double zfocus, photons = 3000, bg = 50;
+    int rc = ZOLA_OK;
 
-    // sqrt(CRLB_x^2 + CRLB_y^2) at depth z (computeFisher(0, 0, z, .005)).
-    double lateral_crlb(double z) const {
-        const double h = .005;
-        const int n = R * R;
-        std::vector<double> f1(n), p2(n), p0(n), d[5];
-        zola_psf_f64_host_r(c, R, 0, 0, zfocus, z, f1.data());
-        for (auto& v : f1) v = v * photons + bg;
-        for (auto& v : d) v.resize(n);
-        const double xs[3][2][3] = {{{h, 0, 0}, {-h, 0, 0}}, {{0, h, 0}, {0, -h, 0}}, {{0, 0, h}, {0, 0, -h}}};
-        for (int p = 0; p < 3; p++) {
-            zola_psf_f64_host_r(c, R, xs[p][0][0], xs[p][0][1], zfocus, z + xs[p][0][2], p2.data());
-            zola_psf_f64_host_r(c, R, xs[p][1][0], xs[p][1][1], zfocus, z + xs[p][1][2], p0.data());
-            for (int k = 0; k < n; k++)
-                d[p][k] = ((p2[k] * photons + bg) - (p0[k] * photons + bg)) / (2 * std::fabs(h));
+    // sqrt(CRLB_x^2 + CRLB_y^2) at every depth of a scan (computeFisher(0, 0, z, .005)).
+    std::vector<double> scan(const std::vector<double>& zs) {
+        std::vector<double> out(zs.size(), INFINITY), F(zs.size() * 25);
+        if (rc) return out;
+        rc = zola_center_fisher_dev(c, (int)zs.size(), zs.data(), zfocus, photons, bg, .005, F.data());
+        if (rc) return out;
+        for (size_t q = 0; q < zs.size(); q++) {
+            double inv[25];
+            inverse5(&F[q * 25], inv);
+            const double sx = std::sqrt(inv[0]), sy = std::sqrt(inv[6]);
+            out[q] = std::sqrt(sx * sx + sy * sy);
         }
-        for (int k = 0; k < n; k++) {
-            d[3][k] = (f1[k] * (photons + h * 100) + bg - (f1[k] * (photons - h * 100) + bg)) / (2 * std::fabs(h));
-            d[4][k] = (f1[k] * photons + (bg + h) - (f1[k] * photons + (bg - h))) / (2 * std::fabs(h));
-        }
-        double I[25], inv[25];
-        for (int p = 0; p < 5; p++)
-            for (int q = 0; q < 5; q++) {
-                double s = 0;
-                for (int k = 0; k < n; k++) s += (1 / f1[k]) * d[p][k] * d[q][k];
-                I[p * 5 + q] = s;
-            }
-        inverse5(I, inv);
-        const double sx = std::sqrt(inv[0]), sy = std::sqrt(inv[6]);
-        return std::sqrt(sx * sx + sy * sy);
-    }
-
-    // CRLB at every z of a scan, in parallel.
-    std::vector<double> scan(const std::vector<double>& zs) const {
-        std::vector<double> out(zs.size());
-        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> th;
-        for (unsigned t = 0; t < nt; t++)
-            th.emplace_back([&, t] {
-                for (size_t i = t; i < zs.size(); i += nt) out[i] = lateral_crlb(zs[i]);
-            });
-        for (auto& x : th) x.join();
         return out;
     }
 
     // getMinPositionCRLB (SearchPSFcenter.java:198-214): first strict minimum.
-    double min_position(double minZ, double maxZ, double stepZ) const {
+    double min_position(double minZ, double maxZ, double stepZ) {
         std::vector<double> zs;
         for (double u = minZ; u < maxZ; u += stepZ) zs.push_back(u);
         const std::vector<double> cr = scan(zs);
@@ -117,7 +89,7 @@ struct Search {
     }
 
     // getMinRangeCRLB (SearchPSFcenter.java:86-143) with its halved range.
-    double min_range(double minZ, double maxZ, double stepZ, double axial) const {
+    double min_range(double minZ, double maxZ, double stepZ, double axial) {
         axial /= 2;
         std::vector<double> zs;
         for (double u = minZ - axial / 2; u <= maxZ + axial / 2; u += stepZ) zs.push_back(u);
@@ -162,5 +134,5 @@ extern "C" int zola_search_psf_center(zola_ctx* c, double zfocus, double axial_r
         old = position;
     }
     *center = S.min_range(position - axial_range * 2, position + axial_range * 2, c->o.xystep, axial_range);
-    return ZOLA_OK;
+    return S.rc;
 }

@@ -63,6 +63,11 @@ int zola_fit_launch(zola_ctx* c, const zola_fit_config* cfg, int B, const float*
                     const float* zoil, const float* varg, double* theta, double* nll, double* crlb, int* iters,
                     int* status, hipStream_t st);
 
+// FD Fisher matrices (nz x 25) of the focus-centre search at depths zs, FP64 on
+// the device (zola_center_dev.hip).
+int zola_center_fisher_dev(zola_ctx* c, int nz, const double* zs, double zfocus, double photons, double bg,
+                           double h, double* fisher);
+
 // Four waves per ROI (1) or one wave per ROI (0) for a 64-grid fit.
 int zola_fit_quad(const zola_ctx* c, int R, int mode);
 
