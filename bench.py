@@ -333,12 +333,13 @@ def mixed_bench(torch, Z, stream, total):
                            status=torch.empty(Bg, dtype=torch.int32, device="cuda")))
         ctxs.append(c)
     ctx = ctxs[1]
-    # autotuner: per ROI side, the faster of one wave and four waves per ROI
+    # autotuner: per ROI side, the fastest of one, two (R <= 21) and four waves per ROI
     layouts = {}
+    names = {Z.LAYOUT_WAVE: "wave", Z.LAYOUT_QUAD: "quad", Z.LAYOUT_PAIR: "pair"}
     for Rg in sizes:
-        lay, t_wave, t_quad = ctx.autotune_fit(Rg, Z.FIT_LM, B=8192)
-        layouts[str(Rg)] = {"layout": "quad" if lay == Z.LAYOUT_QUAD else "wave",
-                            "ms_wave": round(t_wave, 3), "ms_quad": round(t_quad, 3)}
+        lay, t_wave, t_quad, t_pair = ctx.autotune_fit(Rg, Z.FIT_LM, B=8192)
+        layouts[str(Rg)] = {"layout": names.get(lay, str(lay)), "ms_wave": round(t_wave, 3),
+                            "ms_quad": round(t_quad, 3), "ms_pair": round(t_pair, 3) if t_pair else None}
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
     ctx.fit_mixed_dev(cfg, groups)
     e = _events(torch)
@@ -536,6 +537,9 @@ def main():
     iters = torch.empty(B, dtype=torch.int32, device="cuda")
     status = torch.empty(B, dtype=torch.int32, device="cuda")
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    # the default layout of the LM form at 21 px: two waves (one block) per ROI
+    ctx.set_fit_layout(R, Z.FIT_LM, Z.LAYOUT_PAIR)
+    kernel_name = "k_fit_lm64p<21>"
 
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     fit_ms = []
@@ -580,7 +584,7 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
-        if t.get("batch") == B:
+        if t.get("batch") == B and t.get("kernel") == kernel_name:
             traffic = t.get("bytes_per_launch")
 
     out = {
@@ -605,7 +609,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "k_fit_lm64<21,4>",
+            "kernel": kernel_name,
             "achieved": round(achieved / 1e12, 4),
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
@@ -708,9 +712,29 @@ def main():
         rois = px[:S].double().cpu().numpy()
         ini = init[:S].cpu().numpy()
         t0 = time.perf_counter()
-        O.localize_batch(O.optics(), R, coeffs, rois, ini, minZ=-1.3, maxZ=1.3, iterMax=20,
-                         thr=0.0, nthreads=threads)
+        cpu = O.localize_batch(O.optics(), R, coeffs, rois, ini, minZ=-1.3, maxZ=1.3,
+                               iterMax=20, thr=0.0, nthreads=threads)
         dt = time.perf_counter() - t0
+        # Agreement of the batched LM fit with the CPU path on the same ROIs
+        # and starts: the CPU path is the reference's coordinate descent
+        # (cpu/Localization.java:491-653), capped at iterMax = 20.
+        d_nm = np.abs(th[:S, 2:] - cpu["theta"][:, 2:]) * 1000.0
+        close = (d_nm[:, 0] <= 0.5) & (d_nm[:, 1] <= 0.5) & (d_nm[:, 2] <= 1.0)
+        capped = cpu["iters"] >= 20
+        dnll = nll[:S].cpu().numpy() - cpu["lik"]
+        out["lm_vs_cpu_path"] = {
+            "rois": S,
+            "within_0.5_0.5_1nm": round(float(close.mean()), 4),
+            "within_among_cpu_converged": round(float(close[~capped].mean()), 4)
+            if (~capped).any() else None,
+            "cpu_hit_iter_max": round(float(capped.mean()), 4),
+            "median_abs_diff_nm": [round(float(v), 4) for v in np.median(d_nm, axis=0)],
+            "nll_lm_minus_cpu": {"median": round(float(np.median(dnll)), 6),
+                                 "min": round(float(dnll.min()), 6),
+                                 "max": round(float(dnll.max()), 6)},
+            "note": "LM results vs the FP64 restatement of the CPU path (iterMax 20); a "
+                    "negative NLL difference means the LM fit reached a lower Poisson NLL",
+        }
         out["cpu_baseline"] = {
             "value": round(S / dt, 3),
             "unit": "localizations/s",

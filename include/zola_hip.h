@@ -20,7 +20,12 @@
  *  - Units: um for positions, photons for intensities.  Array layouts are
  *    row-major [i][ii] with the first index the ImageJ x column, as in the
  *    reference (process/cpu/LocalizationPipeline.java:592-596).
- *  - A context is used by one host thread at a time.
+ *  - Threads: every entry point locks its context (a detector locks the
+ *    context it was created on), so host threads may share a handle and their
+ *    calls run one after another, each on the context's stream.
+ *    zola_last_error returns the calling thread's copy of the context's last
+ *    message.  zola_ctx_destroy and zola_detector_destroy must not overlap
+ *    other calls on the same handle.
  */
 #ifndef ZOLA_HIP_H
 #define ZOLA_HIP_H
@@ -129,11 +134,14 @@ int  zola_psf_backward_noalloc_dev(zola_ctx* ctx, int B, const float* params, co
  * whose GPU batch shape is fixed by the 100-ROI slots of
  * process/gpu/Modelmany_double_.java:271-387):
  * ZOLA_LAYOUT_WAVE one wave per ROI; ZOLA_LAYOUT_QUAD four waves (one block)
- * per ROI, ceil(R/4) ROI rows per lane; ZOLA_LAYOUT_AUTO (the default) one
- * wave up to R = 21, four above. */
+ * per ROI, ceil(R/4) ROI rows per lane; ZOLA_LAYOUT_PAIR two waves (one block)
+ * per ROI, LM form at R <= 21 only; ZOLA_LAYOUT_AUTO (the default) four waves
+ * above R = 21, below that two waves for the LM form and one for the
+ * reference form. */
 #define ZOLA_LAYOUT_AUTO 0
 #define ZOLA_LAYOUT_WAVE 1
 #define ZOLA_LAYOUT_QUAD 2
+#define ZOLA_LAYOUT_PAIR 3
 
 typedef struct zola_fit_config {
     int    mode;             /* ZOLA_FIT_*                                    */
@@ -166,16 +174,18 @@ int  zola_fit_batch_zoil_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
                              double* theta, double* nll, double* crlb, int* iters, int* status);
 
 /* Sets the work layout (ZOLA_LAYOUT_*) of the 64-grid fits of ROI side R in
- * fit mode `mode` for this context.  Results of the two layouts agree to FP32
- * rounding of the blur sums (a different summation order). */
+ * fit mode `mode` for this context.  Results of the layouts agree to FP32
+ * rounding of the blur sums (a different summation order).  A layout that does
+ * not exist for (R, mode) returns ZOLA_EUNSUPPORTED. */
 int  zola_set_fit_layout(zola_ctx* ctx, int R, int mode, int layout);
 
 /* Autotuner of the fit layout: fits a synthetic batch of B ROIs of side R
  * (noise-free model images, started off the truth) with each layout on the
- * context's stream, keeps the faster one for (R, mode) and returns it in
- * *layout, with the two kernel times in ms[0] (one wave per ROI) and ms[1]
- * (four waves per ROI).  Synchronises the stream; not for use under graph
- * capture. */
+ * context's stream, keeps the fastest for (R, mode) and returns it in
+ * *layout, with the kernel times in ms[0] (one wave per ROI), ms[1] (four
+ * waves per ROI) and ms[2] (two waves per ROI; 0 where that layout does not
+ * exist): ms holds three doubles.  Synchronises the stream; not for use under
+ * graph capture. */
 int  zola_autotune_fit(zola_ctx* ctx, int R, int mode, int B, int* layout, double* ms);
 
 /* As zola_fit_batch_dev for an sCMOS camera: varg holds B x R x R floats, the
@@ -208,6 +218,20 @@ int  zola_fit_mixed_dev(zola_ctx* ctx, const zola_fit_config* cfg, int G, const 
                         const float* const* pixels, const double* const* init, double* const* theta,
                         double* const* nll, double* const* crlb, int* const* iters,
                         int* const* status);
+
+/* Host-array fit of ROIs of several sides in one call (ragged batch): ROI b is
+ * the roi_sizes[b] x roi_sizes[b] block of pixels starting at
+ * pixels[roi_offsets[b]] (row-major, n_pixels floats in all); init, theta, nll,
+ * crlb, iters and status are B-row arrays in ROI order.  The ROIs are grouped
+ * by side (at most 8 sides), copied in once, fitted by zola_fit_mixed_dev and
+ * copied back in ROI order; the results equal those of zola_fit_batch on each
+ * side's ROIs.  The reference fixes one patch side per model object and
+ * pipeline (sizeImage in process/gpu/Modelmany_double_.java:271-300, sizePatch
+ * in process/gpu/LocalizationPipeline_.java:61,109); this form lets one host
+ * call carry patches of several sides. */
+int  zola_fit_ragged(zola_ctx* ctx, const zola_fit_config* cfg, int B, const long long* roi_offsets,
+                     const int* roi_sizes, const float* pixels, long long n_pixels, const double* init,
+                     double* theta, double* nll, double* crlb, int* iters, int* status);
 
 /* Per-ROI initial values as the reference pipeline forms them for an isolated
  * ROI: z from the best zero-shift normalised correlation against nZ templates

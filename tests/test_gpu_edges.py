@@ -33,7 +33,7 @@ def test_empty_batch_is_a_no_op(env):
         assert res["theta"].shape == (0, 5)
 
 
-@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD])
+@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD, Z.LAYOUT_PAIR])
 def test_single_roi_matches_oracle(env, layout):
     ctx, m, a = env
     print(f"case: one ROI, layout {layout}", flush=True)

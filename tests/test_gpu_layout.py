@@ -1,5 +1,5 @@
-"""64-grid fits in both work layouts (one wave per ROI, four waves per ROI) at
-every instantiated ROI side, against the FP64 oracle:
+"""64-grid fits in every work layout (one wave, two waves (LM form, R <= 21) and
+four waves per ROI) at every instantiated ROI side, against the FP64 oracle:
 
 * LM form: the FP64 Levenberg-Marquardt optimum (0.5 nm x/y, 1 nm z, 1e-5
   relative NLL);
@@ -35,9 +35,11 @@ def setup(R):
     return _cache[R]
 
 
-@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD])
+@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD, Z.LAYOUT_PAIR])
 @pytest.mark.parametrize("R", [17, 21, 31, 41])
 def test_lm_layout_matches_fp64_optimum(R, layout):
+    if layout == Z.LAYOUT_PAIR and R > 21:
+        pytest.skip("the two-wave layout covers R <= 21")
     m, a, rois, init, opt = setup(R)
     print(f"R {R} layout {layout}", flush=True)
     ctx = Z.Context(Z.optics(), R, a)
@@ -75,17 +77,29 @@ def test_reference_form_quad_layout_matches_restatement(R):
     refparity.compare(res, ref, 20, "R %d quad" % R)
 
 
-def test_autotune_picks_a_layout_and_fits_follow_it():
-    m, a, rois, init, opt = setup(31)
-    ctx = Z.Context(Z.optics(), 31, a)
-    lay, t_wave, t_quad = ctx.autotune_fit(31, Z.FIT_LM, B=2048)
-    print("autotune R 31:", lay, t_wave, t_quad)
-    assert lay in (Z.LAYOUT_WAVE, Z.LAYOUT_QUAD)
+@pytest.mark.parametrize("R", [21, 31])
+def test_autotune_picks_a_layout_and_fits_follow_it(R):
+    m, a, rois, init, opt = setup(R)
+    ctx = Z.Context(Z.optics(), R, a)
+    lay, t_wave, t_quad, t_pair = ctx.autotune_fit(R, Z.FIT_LM, B=2048)
+    print(f"autotune R {R}:", lay, t_wave, t_quad, t_pair)
+    times = {Z.LAYOUT_WAVE: t_wave, Z.LAYOUT_QUAD: t_quad, Z.LAYOUT_PAIR: t_pair}
     assert t_wave > 0 and t_quad > 0
-    assert (lay == Z.LAYOUT_QUAD) == (t_quad < t_wave)
+    assert (t_pair > 0) == (R <= 21)
+    assert lay == min((t, k) for k, t in times.items() if t > 0)[1]
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
     got = ctx.fit(cfg, rois, init)
-    ctx.set_fit_layout(31, Z.FIT_LM, lay)
+    ctx.set_fit_layout(R, Z.FIT_LM, lay)
     again = ctx.fit(cfg, rois, init)
     ctx.close()
     np.testing.assert_array_equal(got["theta"], again["theta"])
+
+
+def test_pair_layout_is_lm_only_and_small_sides_only():
+    m, a, rois, init, opt = setup(31)
+    ctx = Z.Context(Z.optics(), 31, a)
+    with pytest.raises(Z.ZolaError):
+        ctx.set_fit_layout(31, Z.FIT_LM, Z.LAYOUT_PAIR)
+    with pytest.raises(Z.ZolaError):
+        ctx.set_fit_layout(21, Z.FIT_REFERENCE, Z.LAYOUT_PAIR)
+    ctx.close()

@@ -50,3 +50,51 @@ def test_mixed_groups_equal_separate_launches(torch_cuda, mode):
             np.testing.assert_array_equal(g["status"].cpu().numpy(), r["status"])
             g["theta"].zero_()
     ctx.close()
+
+
+def test_ragged_host_batch_equals_per_side_fits(torch_cuda):
+    """zola_fit_ragged on interleaved ROIs of four sides: each ROI's results
+    equal those of zola_fit_batch on its side's ROIs (same order within a side);
+    an offset past the pixel array is refused before any device work."""
+    print("case: ragged host batch", SIZES, flush=True)
+    coef = Z.astigmatic_coeffs(28)
+    opt = Z.optics(size_fft=64)
+    ctx = Z.Context(opt, 21, coef)
+    cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4)
+    per = {}
+    for k, (R, B) in enumerate(zip(SIZES, COUNTS)):
+        m = O.Model(O.optics(size_fft=64), R, coef)
+        rois, truth = simdata.isolated_rois(m, B, seed=900 + k, photons=(200, 20000))
+        init = truth.copy()
+        init[:, 4] += 0.05
+        cR = Z.Context(opt, R, coef)
+        per[R] = (rois, init, cR.fit(cR.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=-1.3, zmax=1.3, tol=1e-4),
+                                      rois, init))
+        cR.close()
+    # interleave the sides: ROI b takes the next ROI of side SIZES[b % 4] while any is left
+    order, nxt = [], {R: 0 for R in SIZES}
+    while any(nxt[R] < B for R, B in zip(SIZES, COUNTS)):
+        for R, B in zip(SIZES, COUNTS):
+            if nxt[R] < B:
+                order.append((R, nxt[R]))
+                nxt[R] += 1
+    rois = [per[R][0][i] for R, i in order]
+    init = np.stack([per[R][1][i] for R, i in order])
+    got = ctx.fit_ragged(cfg, rois, init)
+    for b, (R, i) in enumerate(order):
+        ref = per[R][2]
+        assert np.array_equal(got["theta"][b], ref["theta"][i]), (b, R, i)
+        assert got["nll"][b] == ref["nll"][i] and got["iters"][b] == ref["iters"][i]
+        assert got["status"][b] == ref["status"][i]
+        assert np.array_equal(got["crlb"][b], ref["crlb"][i])
+    # a ROI reaching past the pixel array is an argument error
+    import ctypes as C
+    lib = Z.lib()
+    offs = np.array([0, 441], np.int64)
+    sizes = np.array([21, 21], np.int32)
+    px = np.zeros(2 * 21 * 21 - 1, np.float32)
+    out = [np.zeros((2, 5)), np.zeros(2), np.zeros((2, 3)), np.zeros(2, np.int32), np.zeros(2, np.int32)]
+    rc = lib.zola_fit_ragged(ctx._h, C.byref(cfg), 2, Z._ptr(offs), Z._ptr(sizes), Z._ptr(px), px.size,
+                             Z._ptr(init[:2].copy()), *[Z._ptr(a) for a in out])
+    assert rc == -1 and b"outside" in lib.zola_last_error(ctx._h)
+    ctx.close()

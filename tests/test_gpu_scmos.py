@@ -43,7 +43,7 @@ def _fit(torch, ctx, cfg, rois, varg, init):
     return th.cpu().numpy(), nl.cpu().numpy(), it.cpu().numpy(), st.cpu().numpy()
 
 
-@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD])
+@pytest.mark.parametrize("layout", [Z.LAYOUT_WAVE, Z.LAYOUT_QUAD, Z.LAYOUT_PAIR])
 def test_lm_scmos_matches_fp64_optimum(torch_cuda, layout):
     print(f"case: sCMOS LM R {R} layout {layout}", flush=True)
     a, m, rois, varg, init = _data()

@@ -629,6 +629,7 @@ extern "C" {
 int zola_bead_prepare(zola_ctx* ctx, int S, int W, int H, const float* adu, double adu_factor, double gain,
                       double offset, double zstep, int P, int nb, float* xp, float* yp, double* patches, double* B,
                       double* A, double* background, zola_bead_diag* diag) {
+    CtxLock lk_(ctx);
     if (!ctx) return ZOLA_EINVAL;
     if (!(gain > 0) || !(adu_factor > 0))
         return set_err(ctx, ZOLA_EINVAL, "bead_prepare: gain and adu must be positive");
@@ -642,6 +643,7 @@ int zola_bead_prepare_scmos(zola_ctx* ctx, int S, int W, int H, const float* adu
                             const float* gain, const float* variance, double zstep, int P, int nb, float* xp,
                             float* yp, double* patches, double* B, double* A, double* scmos_patches,
                             double* background, zola_bead_diag* diag) {
+    CtxLock lk_(ctx);
     if (!ctx) return ZOLA_EINVAL;
     if (!offset || !gain || !variance) return set_err(ctx, ZOLA_EINVAL, "bead_prepare_scmos: NULL camera map");
     return bead_prepare(ctx, S, W, H, adu, 0.0, 0.0, offset, gain, variance, zstep, P, nb, xp, yp, patches, B, A,

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/zola_hip.h"
+#include "zola_ctx.hpp"
 
 namespace {
 
@@ -135,6 +136,7 @@ extern "C" {
 int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const double* dz,
                    double* bead, double* coeffs, const int* fit_mask, int max_iter, double gtol,
                    double* out_nll, int* out_iters, int* out_evals) {
+    CtxLock lk_(ctx);
     if (!ctx || nb <= 0 || S <= 0 || !pixels || !dz || !bead || !coeffs || max_iter < 0)
         return ZOLA_EINVAL;
     int info[5];
@@ -293,6 +295,7 @@ int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const doub
 // fit_a_each (0.1 of that slice's A).  sigma is kept >= 0.05 px.
 int zola_calib_fit_full(zola_ctx* ctx, zola_calib_beads* bd, double* sigma, double* coeffs, const int* fit_mask,
                         int flags, int max_iter, double gtol, double* out_nll, int* out_iters, int* out_evals) {
+    CtxLock lk_(ctx);
     if (!ctx || !bd || !coeffs || !sigma || max_iter < 0 || flags < 0 || flags > ZOLA_CALIB_FIT_ALL)
         return ZOLA_EINVAL;
     const int nb = bd->nb, S = bd->S;

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,26 @@ struct zola_ctx {
     // work layout of the 64-grid fits per ROI side (17, 21, 31, 41) and mode
     // (reference form, LM): ZOLA_LAYOUT_*
     int layout[4][2] = {};
+    // Entry points on one context run one at a time (CtxLock): host threads may
+    // share a handle, their calls are serialised.
+    mutable std::recursive_mutex mu;
+};
+
+// Holds a context's lock for the duration of an entry point (no-op for a
+// null context, which the entry point then rejects).
+class CtxLock {
+  public:
+    explicit CtxLock(const zola_ctx* c) : m_(c ? &c->mu : nullptr) {
+        if (m_) m_->lock();
+    }
+    ~CtxLock() {
+        if (m_) m_->unlock();
+    }
+    CtxLock(const CtxLock&) = delete;
+    CtxLock& operator=(const CtxLock&) = delete;
+
+  private:
+    std::recursive_mutex* m_;
 };
 
 // Looks up the device address of the fixed scratch array of the allocation-free
@@ -68,8 +89,9 @@ int zola_fit_launch(zola_ctx* c, const zola_fit_config* cfg, int B, const float*
 int zola_center_fisher_dev(zola_ctx* c, int nz, const double* zs, double zfocus, double photons, double bg,
                            double h, double* fisher);
 
-// Four waves per ROI (1) or one wave per ROI (0) for a 64-grid fit.
-int zola_fit_quad(const zola_ctx* c, int R, int mode);
+// Kernel layout of a 64-grid fit: 0 one wave per ROI, 1 four waves per ROI,
+// 2 two waves per ROI (LM form, R <= 21).
+int zola_fit_layout(const zola_ctx* c, int R, int mode);
 
 inline int set_err(zola_ctx* c, int code, const char* fmt, ...) {
     if (c) {

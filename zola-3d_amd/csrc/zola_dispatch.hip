@@ -16,6 +16,8 @@ hipError_t launch_adjoint_r(const Model64& M, const AdjArgs& A, hipStream_t st);
 template <int R>
 hipError_t launch_fit64q_r(const Model64& M, int mode, const FitArgs& A, hipStream_t st);
 template <int R>
+hipError_t launch_fit64p_r(const Model64& M, const FitArgs& A, hipStream_t st);
+template <int R>
 hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params, float* out, hipStream_t st);
 template <int R>
 hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st);
@@ -74,8 +76,14 @@ hipError_t zola_launch_psf_forward64(const Model64& M, int R, int B, const float
 }
 
 
-hipError_t zola_launch_fit64(const Model64& M, int R, int mode, int quad, const FitArgs& A, hipStream_t st) {
-    if (quad) {
+hipError_t zola_launch_fit64(const Model64& M, int R, int mode, int layout, const FitArgs& A, hipStream_t st) {
+    if (layout == 2) {
+        if (mode != 1) return hipErrorInvalidValue;
+        if (R == 17) return launch_fit64p_r<17>(M, A, st);
+        if (R == 21) return launch_fit64p_r<21>(M, A, st);
+        return hipErrorInvalidValue;
+    }
+    if (layout == 1) {
 #define CALL(RR) launch_fit64q_r<RR>(M, mode, A, st)
         ZOLA_R_DISPATCH(R, CALL)
 #undef CALL

@@ -22,6 +22,7 @@
 #include "zola_psf64.hpp"
 #include "zola_psf128.hpp"
 #include "zola_psf64q.hpp"
+#include "zola_psf64p.hpp"
 
 namespace zola {
 
@@ -187,6 +188,12 @@ __device__ __forceinline__ void lm_eval64(const Model64& M, const BlockTables64&
     column_pass2_64<R>(T.h3, T.hd3, scr, lane, vB, vX);
     column_pass64<R>(T.h3, scr + R * kTS, lane, vY);
     column_pass64<R>(T.h3, scr + 2 * R * kTS, lane, vZ);
+    // Pin the results here: the tangents are only read on accepted trials, and
+    // without this the compiler sinks their arithmetic past the reductions that
+    // follow, keeping every loaded row-blur value live across them.
+#pragma unroll
+    for (int t = 0; t < RoiLayout<R>::RPL; t++)
+        __asm__ __volatile__("" : "+v"(vB[t]), "+v"(vX[t]), "+v"(vY[t]), "+v"(vZ[t]));
     wave_lds_fence();
 }
 
@@ -334,6 +341,80 @@ struct Synth64q : QuadWaves {
     }
 };
 
+// Two waves per ROI (zola_psf64p.hpp): lane = (ROI column j, row group g),
+// wave wr owns ROI rows [wr RH, wr RH + RH) (RH = ceil(R/2)), RPL rows per lane.
+// Reductions are summed over the two waves through LDS, wave 0's part first,
+// so both waves see identical values and take identical branches.
+struct PairWaves {
+    static constexpr int kWaves = 2;
+    template <int R>
+    static constexpr int RPL = Roi64p<R>::RPL;
+    template <int R>
+    __device__ __forceinline__ static bool valid(int lane, int wr, int t) {
+        const int g = lane / R;
+        const int rows = wr == 0 ? Roi64p<R>::RH : R - Roi64p<R>::RH;
+        return g < Roi64p<R>::G && g * RPL<R> + t < rows;
+    }
+    template <int R>
+    __device__ __forceinline__ static int pix(int lane, int wr, int t) {
+        return (wr * Roi64p<R>::RH + (lane / R) * RPL<R> + t) * R + lane % R;
+    }
+    struct Red {
+        double d[2][24];
+        float f[2][4];
+    };
+    template <int K>
+    __device__ __forceinline__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
+        static_assert(K <= 24, "reduction width");
+        wave_sum_rows<K>(v);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (lane == 0) r.d[wr][k] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = r.d[0][k] + r.d[1][k];
+        __syncthreads();
+    }
+    template <int K>
+    __device__ __forceinline__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
+        static_assert(K <= 4, "reduction width");
+        wave_sum_rows<K>(v);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (lane == 0) r.f[wr][k] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = r.f[0][k] + r.f[1][k];
+        __syncthreads();
+    }
+    __device__ __forceinline__ static double sum(double v, Red& r, int lane, int wr) {
+        double a[1] = {v};
+        sumv<1>(a, r, lane, wr);
+        return a[0];
+    }
+    __device__ __forceinline__ static float sum(float v, Red& r, int lane, int wr) {
+        float a[1] = {v};
+        sumv<1>(a, r, lane, wr);
+        return a[0];
+    }
+};
+
+// The 64 grid by two waves per ROI (LM form only): blur taps in LDS, pupil
+// tables read from global memory.
+struct Synth64p : PairWaves {
+    using Model = Model64;
+    using Shared = BlurTaps64;
+    static constexpr int kLmFloats = 2 * (int)(sizeof(WaveScratch64) / sizeof(float));
+    static constexpr int kRefFloats = kLmFloats;
+    __device__ __forceinline__ static void load(Shared& T, const Model& M) { load_taps64(T, M); }
+    template <int R>
+    __device__ __forceinline__ static void lm_eval(const Model& M, const Shared& T, float* scr, int lane, int wr, float x, float y,
+                                   float z, float zoil, float (&vB)[Roi64p<R>::RPL], float (&vX)[Roi64p<R>::RPL],
+                                   float (&vY)[Roi64p<R>::RPL], float (&vZ)[Roi64p<R>::RPL]) {
+        eval64p<R>(M, T, scr, lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
+    }
+};
+
 // Trial point of an LM step d from th, clamped as the fit clamps every trial:
 // x, y by at most one pixel, z by at most 0.2 um, A >= 1, B >= 0.01,
 // zmin <= z <= zmax.
@@ -450,11 +531,21 @@ __device__ __forceinline__ void lm_secant_update(LmState& L) {
 // addresses plus immediate offsets reach them), then per-wave state and scratch.
 // WPB waves per block, WPB / SY::kWaves ROIs; the fit state is private to
 // each wave (the waves of a ROI keep identical copies).
+// The step of a ROI fitted by several waves, solved by its first wave and read
+// by the others: trial point, predicted decrease, and flags (bit 0: no trial
+// to evaluate, bit 1: converged before evaluating).
+struct LmShare {
+    double tn[5];
+    double pred;
+    int flag;
+};
+
 template <class SY, int WPB>
 struct LmLds {
     typename SY::Shared T;
     LmState L[WPB];
     typename SY::Red red[WPB / SY::kWaves];
+    LmShare share[WPB / SY::kWaves];
     int q[WPB / SY::kWaves];
     alignas(16) float S[WPB / SY::kWaves][SY::kLmFloats];
 };
@@ -499,6 +590,7 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
     while (b < A.B) {
         float* scr = lds.S[slot];
         auto& red = lds.red[slot];
+        LmShare& share = lds.share[slot];
         LmState& L = lds.L[w];
         const float* pix = A.pixels + (size_t)b * R * R;
         const float* vroi = A.varg ? A.varg + (size_t)b * R * R : nullptr;
@@ -531,7 +623,11 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         bool first = true;
         while (true) {
             bool eval_now = true, stop_noise = false, pre_conv = false;
-            if (!first) {
+            // The step is solved by the ROI's first wave.  With several waves per
+            // ROI the others take its trial point, predicted decrease and
+            // decision from LDS; they keep no Fisher matrix, secant term or
+            // gradient of their own (only the first wave solves).
+            if (!first && wr == 0) {
                 double Am[25], Lc[25], d[5];
     #pragma unroll
                 for (int p = 0; p < 5; p++) {
@@ -606,6 +702,31 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                     for (int p = 0; p < 5; p++) L.tn[p] = tn[p];
                 }
                 wave_lds_fence();
+                if constexpr (SY::kWaves > 1) {
+                    if (lane == 0) {
+                        share.flag = (eval_now ? 0 : 1) | (pre_conv ? 2 : 0);
+                        share.pred = L.pred;
+    #pragma unroll
+                        for (int p = 0; p < 5; p++) share.tn[p] = L.tn[p];
+                    }
+                }
+            }
+            if constexpr (SY::kWaves > 1) {
+                if (!first) {
+                    __syncthreads();
+                    const int f = share.flag;
+                    eval_now = (f & 1) == 0;
+                    pre_conv = (f & 2) != 0;
+                    if (wr != 0) {
+                        L.pred = share.pred;
+    #pragma unroll
+                        for (int p = 0; p < 5; p++) L.tn[p] = share.tn[p];
+                    }
+                    // without an evaluation (and its barriers) before the next
+                    // solve, the first wave would rewrite the step too early
+                    if (!eval_now) __syncthreads();
+                    wave_lds_fence();
+                }
             }
             bool accepted = false;
             if (eval_now) {
@@ -685,15 +806,17 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                         }
                     }
                     SY::template sumv<20>(acc, red, lane, wr);
+                    if (wr == 0) {
     #pragma unroll
-                    for (int p = 0; p < 5; p++) {
-                        L.gp[p] = L.gr[p];
-                        L.gr[p] = acc[p];
+                        for (int p = 0; p < 5; p++) {
+                            L.gp[p] = L.gr[p];
+                            L.gr[p] = acc[p];
+                        }
+    #pragma unroll
+                        for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
+                        wave_lds_fence();
+                        if (!first) lm_secant_update(L);
                     }
-    #pragma unroll
-                    for (int k = 0; k < 15; k++) L.H[k] = acc[5 + k];
-                    wave_lds_fence();
-                    if (!first) lm_secant_update(L);
     #pragma unroll
                     for (int p = 0; p < 5; p++) L.th[p] = L.tn[p];
                 }
@@ -712,9 +835,9 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
                 if (conv || it >= A.max_iter) break;
             }
         }
-        // CRLB of x, y, z from the Fisher matrix at the optimum
-        double cr[3];
-        {
+        // CRLB of x, y, z from the Fisher matrix at the optimum (first wave)
+        double cr[3] = {0.0, 0.0, 0.0};
+        if (wr == 0) {
             double Am[25], Lc[25];
     #pragma unroll
             for (int p = 0; p < 5; p++)
@@ -754,6 +877,13 @@ template <int R, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fit_lm128(Model128 M, FitArgs A) {
     __shared__ LmLds<Synth128, WPB> lds;
     fit_lm_body<Synth128, R, WPB>(M, A, lds);
+}
+
+// Two waves per SIMD (four blocks per CU): at most 256 registers per lane.
+template <int R>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fit_lm64p(Model64 M, FitArgs A) {
+    __shared__ LmLds<Synth64p, 2> lds;
+    fit_lm_body<Synth64p, R, 2>(M, A, lds);
 }
 
 template <int R>

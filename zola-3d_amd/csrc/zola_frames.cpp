@@ -290,6 +290,7 @@ extern "C" {
 
 int zola_detector_create(zola_ctx* c, int W, int H, double zmin, double zmax, double zstep,
                          double threshold, double zfocus, zola_detector** out) {
+    CtxLock lk_(c);
     if (!c || !out) return ZOLA_EINVAL;
     *out = nullptr;
     if (W <= 0 || H <= 0 || !(zstep > 0) || !(zmax > zmin))
@@ -363,6 +364,7 @@ int zola_detector_create(zola_ctx* c, int W, int H, double zmin, double zmax, do
 }
 
 int zola_detector_destroy(zola_detector* d) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d) return ZOLA_OK;
     free_batch(d);
     free_cand(d);
@@ -379,24 +381,28 @@ int zola_detector_destroy(zola_detector* d) {
 }
 
 int zola_detector_info(const zola_detector* d, int* info) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d || !info) return ZOLA_EINVAL;
     info[0] = d->g.nZ; info[1] = d->g.FX; info[2] = d->g.FY; info[3] = d->g.W; info[4] = d->g.H;
     return ZOLA_OK;
 }
 
 int zola_detector_range(const zola_detector* d, double* range) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d || !range) return ZOLA_EINVAL;
     std::memcpy(range, d->range.data(), d->range.size() * sizeof(double));
     return ZOLA_OK;
 }
 
 int zola_detector_templates(const zola_detector* d, double* psf) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d || !psf) return ZOLA_EINVAL;
     std::memcpy(psf, d->psf.data(), d->psf.size() * sizeof(double));
     return ZOLA_OK;
 }
 
 int zola_detector_correlation(zola_detector* d, const float* frame, double* corr) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d) return ZOLA_EINVAL;
     zola_ctx* c = d->ctx;
     if (!frame || !corr) return set_err(c, ZOLA_EINVAL, "null buffer");
@@ -414,6 +420,7 @@ int zola_detector_correlation(zola_detector* d, const float* frame, double* corr
 }
 
 int zola_detect_frames(zola_detector* d, int F, const float* frames, int cap, double* cand, int* counts) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d) return ZOLA_EINVAL;
     zola_ctx* c = d->ctx;
     if (F < 0 || cap < 0 || (F > 0 && (!frames || !counts || (cap > 0 && !cand))))
@@ -961,6 +968,7 @@ int localize_impl(zola_detector* d, const zola_fit_config* fit, const zola_accep
 extern "C" int zola_localize_frames(zola_detector* d, const zola_fit_config* fit, const zola_accept_config* acc,
                                     int F, const float* frames, int cap, double* locs, int* n_locs,
                                     int* per_frame, int* rounds) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d) return ZOLA_EINVAL;
     if (!fit || !acc || F < 0 || cap < 0 || !n_locs || (F > 0 && !frames) || (cap > 0 && !locs))
         return set_err(d->ctx, ZOLA_EINVAL, "bad frame-pipeline arguments");
@@ -971,6 +979,7 @@ extern "C" int zola_localize_frames_scmos(zola_detector* d, const zola_fit_confi
                                           const zola_accept_config* acc, int F, const float* adu,
                                           const float* offset, const float* gain, const float* variance, int cap,
                                           double* locs, int* n_locs, int* per_frame, int* rounds) {
+    CtxLock lk_(d ? d->ctx : nullptr);
     if (!d) return ZOLA_EINVAL;
     zola_ctx* c = d->ctx;
     if (!fit || !acc || F < 0 || cap < 0 || !n_locs || (F > 0 && (!adu || !offset || !gain || !variance)) ||

@@ -162,13 +162,14 @@ static int upload_tables(zola_ctx* c) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         return ZOLA_OK;
     }
-    std::vector<float> tab(4 * 1024, 0.f);
+    std::vector<float> tab(8 * 1024, 0.f);
     for (int p = 0; p < c->D; p++) {
         int w = c->wu[p] * 32 + c->wv[p];
         tab[w] = (float)c->phase[p];
         tab[1024 + w] = (float)c->amp[p];
         tab[2048 + w] = (float)(c->kz[p] - c->kz0);
         tab[3072 + w] = (float)(c->kzo[p] - c->kzo0);
+        for (int k = 0; k < 4; k++) tab[4096 + 4 * w + k] = tab[1024 * k + w];
     }
     HIP_TRY(c, hipMemcpyAsync(c->d_tab, tab.data(), tab.size() * sizeof(float),
                               hipMemcpyHostToDevice, c->stream));
@@ -180,12 +181,22 @@ namespace {
 int roi_slot(int R) { return R == 17 ? 0 : R == 21 ? 1 : R == 31 ? 2 : R == 41 ? 3 : -1; }
 }  // namespace
 
-int zola_fit_quad(const zola_ctx* c, int R, int mode) {
+int zola_fit_layout(const zola_ctx* c, int R, int mode) {
     const int i = roi_slot(R);
     const int l = i >= 0 ? c->layout[i][mode == ZOLA_FIT_LM ? 1 : 0] : ZOLA_LAYOUT_AUTO;
     if (l == ZOLA_LAYOUT_WAVE) return 0;
     if (l == ZOLA_LAYOUT_QUAD) return 1;
-    return R >= 31 ? 1 : 0;  // one wave holds ceil(R/3) or more ROI rows per lane above 21
+    if (l == ZOLA_LAYOUT_PAIR) return 2;
+    // one wave holds ceil(R/3) or more ROI rows per lane above 21; the pair
+    // layout covers the LM form up to 21
+    if (R >= 31) return 1;
+    return mode == ZOLA_FIT_LM ? 2 : 0;
+}
+
+// Whether a layout (ZOLA_LAYOUT_WAVE/QUAD/PAIR) exists for (R, mode).
+static bool layout_available(int R, int mode, int layout) {
+    if (layout == ZOLA_LAYOUT_PAIR) return mode == ZOLA_FIT_LM && R <= 21;
+    return layout == ZOLA_LAYOUT_WAVE || layout == ZOLA_LAYOUT_QUAD;
 }
 
 int zola_psf_launch(zola_ctx* c, int B, const float* params, float* out, hipStream_t st) {
@@ -218,7 +229,7 @@ int zola_fit_launch(zola_ctx* c, const zola_fit_config* cfg, int B, const float*
     A.iters = iters;
     A.status = status;
     if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, st));
-    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, zola_fit_quad(c, c->R, cfg->mode), A, st));
+    else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, zola_fit_layout(c, c->R, cfg->mode), A, st));
     return ZOLA_OK;
 }
 
@@ -341,11 +352,13 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
         c->err.clear();
         return ZOLA_OK;
     }
-    HIP_TRY(c, hipMalloc((void**)&c->d_tab, 4 * 1024 * sizeof(float)));
+    // four tables of 1024 floats, then the same interleaved as 1024 float4
+    HIP_TRY(c, hipMalloc((void**)&c->d_tab, 8 * 1024 * sizeof(float)));
     c->model.phi = c->d_tab;
     c->model.amp = c->d_tab + 1024;
     c->model.kz = c->d_tab + 2048;
     c->model.kzo = c->d_tab + 3072;
+    c->model.tab4 = reinterpret_cast<const float4*>(c->d_tab + 4096);
     c->model.kappa = (float)c->kappa;
     int rc = upload_tables(c);
     if (rc) return rc;
@@ -386,15 +399,25 @@ int zola_ctx_destroy(zola_ctx* c) {
     return ZOLA_OK;
 }
 
-const char* zola_last_error(const zola_ctx* c) { return c ? c->err.c_str() : "null context"; }
+// The message is copied for the calling thread under the context's lock, so the
+// pointer stays valid until this thread's next zola_last_error call.
+const char* zola_last_error(const zola_ctx* c) {
+    if (!c) return "null context";
+    static thread_local std::string msg;
+    CtxLock lk_(c);
+    msg = c->err;
+    return msg.c_str();
+}
 
 int zola_ctx_info(const zola_ctx* c, int* info) {
+    CtxLock lk_(c);
     if (!c || !info) return ZOLA_EINVAL;
     info[0] = c->N; info[1] = c->D; info[2] = c->s2; info[3] = c->R; info[4] = c->J;
     return ZOLA_OK;
 }
 
 int zola_set_coefficients(zola_ctx* c, const double* coeffs, int J) {
+    CtxLock lk_(c);
     if (!c || !coeffs || J != c->J) return set_err(c, ZOLA_EINVAL, "coefficient count must be %d", c ? c->J : 0);
     for (int j = 0; j < J; j++) c->coef[j] = coeffs[j];
     compute_phase(c);
@@ -402,6 +425,7 @@ int zola_set_coefficients(zola_ctx* c, const double* coeffs, int J) {
 }
 
 int zola_set_sigma(zola_ctx* c, double sigma) {
+    CtxLock lk_(c);
     if (!c || !(sigma > 0)) return set_err(c, ZOLA_EINVAL, "sigma must be > 0");
     c->o.sigma = sigma;
     int rc = compute_h1(c);
@@ -410,6 +434,7 @@ int zola_set_sigma(zola_ctx* c, double sigma) {
 }
 
 int zola_set_stream(zola_ctx* c, void* s) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     // the context's buffers (templates, staging) may still be in use by work on
     // the previous stream: finish it before work on the new stream can reuse them
@@ -421,6 +446,7 @@ int zola_set_stream(zola_ctx* c, void* s) {
 }
 
 int zola_psf_forward_dev(zola_ctx* c, int B, const float* params, float* out) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (B < 0 || (B > 0 && (!params || !out))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
@@ -428,6 +454,7 @@ int zola_psf_forward_dev(zola_ctx* c, int B, const float* params, float* out) {
 }
 
 int zola_psf_forward(zola_ctx* c, int B, const float* params, float* out) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (B < 0 || (B > 0 && (!params || !out))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
@@ -446,6 +473,7 @@ int zola_psf_forward(zola_ctx* c, int B, const float* params, float* out) {
 
 int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* w,
                           float* dparams, double* da) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
@@ -475,6 +503,7 @@ int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* 
 
 int zola_psf_backward(zola_ctx* c, int B, const float* params, const float* w, float* dparams,
                       double* da) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
@@ -503,6 +532,7 @@ int zola_psf_backward(zola_ctx* c, int B, const float* params, const float* w, f
 }
 
 int zola_get_stream(const zola_ctx* c, void** stream) {
+    CtxLock lk_(c);
     if (!c || !stream) return ZOLA_EINVAL;
     *stream = (void*)c->stream;
     return ZOLA_OK;
@@ -511,6 +541,7 @@ int zola_get_stream(const zola_ctx* c, void** stream) {
 int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pixels,
                         const double* amp, const double* bg, double* nll, float* dparams,
                         double* damp, double* dbg, double* da) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0) return set_err(c, ZOLA_EINVAL, "bad batch");
@@ -549,6 +580,7 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
 int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, const float* pixels,
                              const double* amp, const double* poly, const double* offset, double* nll,
                              float* dparams, double* damp, double* dpoly, double* dsigma, double* da) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (nb < 0 || S <= 0 || (long)nb * S > (1L << 30)) return set_err(c, ZOLA_EINVAL, "bad batch");
@@ -590,18 +622,21 @@ int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, co
 int zola_fit_batch_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                        const double* init, double* theta, double* nll, double* crlb, int* iters,
                        int* status) {
+    CtxLock lk_(c);
     return zola_fit_batch_zoil_dev(c, cfg, B, pixels, init, nullptr, theta, nll, crlb, iters, status);
 }
 
 int zola_fit_batch_zoil_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                             const double* init, const float* zoil, double* theta, double* nll, double* crlb,
                             int* iters, int* status) {
+    CtxLock lk_(c);
     return zola_fit_batch_ext_dev(c, cfg, B, pixels, init, zoil, nullptr, theta, nll, crlb, iters, status);
 }
 
 int zola_fit_batch_scmos_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                              const float* varg, const double* init, double* theta, double* nll, double* crlb,
                              int* iters, int* status) {
+    CtxLock lk_(c);
     if (c && !varg) return set_err(c, ZOLA_EINVAL, "null sCMOS map");
     return zola_fit_batch_ext_dev(c, cfg, B, pixels, init, nullptr, varg, theta, nll, crlb, iters, status);
 }
@@ -609,6 +644,7 @@ int zola_fit_batch_scmos_dev(zola_ctx* c, const zola_fit_config* cfg, int B, con
 int zola_fit_batch_ext_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels, const double* init,
                            const float* zoil, const float* varg, double* theta, double* nll, double* crlb,
                            int* iters, int* status) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
     if (B == 0) return ZOLA_OK;
@@ -622,17 +658,21 @@ int zola_fit_batch_ext_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const
 }
 
 int zola_set_fit_layout(zola_ctx* c, int R, int mode, int layout) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "fit layouts apply to the 64 grid");
     const int i = roi_slot(R);
     if (i < 0) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available", R);
     if (mode != ZOLA_FIT_REFERENCE && mode != ZOLA_FIT_LM) return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", mode);
-    if (layout < ZOLA_LAYOUT_AUTO || layout > ZOLA_LAYOUT_QUAD) return set_err(c, ZOLA_EINVAL, "unknown layout %d", layout);
+    if (layout < ZOLA_LAYOUT_AUTO || layout > ZOLA_LAYOUT_PAIR) return set_err(c, ZOLA_EINVAL, "unknown layout %d", layout);
+    if (layout != ZOLA_LAYOUT_AUTO && !layout_available(R, mode, layout))
+        return set_err(c, ZOLA_EUNSUPPORTED, "layout %d not available for side %d, mode %d", layout, R, mode);
     c->layout[i][mode == ZOLA_FIT_LM ? 1 : 0] = layout;
     return ZOLA_OK;
 }
 
 int zola_autotune_fit(zola_ctx* c, int R, int mode, int B, int* layout, double* ms) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "fit layouts apply to the 64 grid");
     if (roi_slot(R) < 0) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available", R);
@@ -665,7 +705,7 @@ int zola_autotune_fit(zola_ctx* c, int R, int mode, int B, int* layout, double* 
     int *d_it = nullptr, *d_st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = ZOLA_OK;
-    double t[2] = {0, 0};
+    double t[3] = {0, 0, 0};
     auto run = [&]() -> int {
         HIP_TRY(c, hipMalloc((void**)&d_par, par.size() * sizeof(float)));
         HIP_TRY(c, hipMalloc((void**)&d_px, (size_t)B * R2 * sizeof(float)));
@@ -701,7 +741,9 @@ int zola_autotune_fit(zola_ctx* c, int R, int mode, int B, int* layout, double* 
         A.crlb = d_cr;
         A.iters = d_it;
         A.status = d_st;
-        for (int q = 0; q < 2; q++) {
+        for (int q = 0; q < 3; q++) {
+            // kernel layout q: 0 wave, 1 quad, 2 pair (ZOLA_LAYOUT_* is q + 1)
+            if (!layout_available(R, mode, q + 1)) continue;
             HIP_TRY(c, zola_launch_fit64(c->model, R, mode, q, A, c->stream));  // warm-up
             float best = 1e30f;
             for (int rep = 0; rep < 2; rep++) {
@@ -724,19 +766,21 @@ int zola_autotune_fit(zola_ctx* c, int R, int mode, int B, int* layout, double* 
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (rc) return rc;
-    const int choice = t[1] < t[0] ? ZOLA_LAYOUT_QUAD : ZOLA_LAYOUT_WAVE;
+    int best = 0;
+    for (int q = 1; q < 3; q++)
+        if (t[q] > 0 && t[q] < t[best]) best = q;
+    const int choice = best + 1;  // ZOLA_LAYOUT_WAVE, _QUAD, _PAIR
     c->layout[roi_slot(R)][mode == ZOLA_FIT_LM ? 1 : 0] = choice;
     if (layout) *layout = choice;
-    if (ms) {
-        ms[0] = t[0];
-        ms[1] = t[1];
-    }
+    if (ms)
+        for (int q = 0; q < 3; q++) ms[q] = t[q];
     return ZOLA_OK;
 }
 
 int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int* R, const int* B,
                        const float* const* pixels, const double* const* init, double* const* theta,
                        double* const* nll, double* const* crlb, int* const* iters, int* const* status) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (!cfg || G <= 0 || G > 8 || !R || !B || !pixels || !init || !theta || !nll || !crlb || !iters || !status)
         return set_err(c, ZOLA_EINVAL, "bad mixed-fit arguments (1 <= G <= 8)");
@@ -785,7 +829,7 @@ int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int
             A.iters = iters[g];
             A.status = status[g];
             err = c->N == zola::kN2 ? zola_launch_fit128(c->model128, R[g], cfg->mode, A, c->side[g])
-                                    : zola_launch_fit64(c->model, R[g], cfg->mode, zola_fit_quad(c, R[g], cfg->mode),
+                                    : zola_launch_fit64(c->model, R[g], cfg->mode, zola_fit_layout(c, R[g], cfg->mode),
                                                         A, c->side[g]);
             if (err != hipSuccess) break;
         }
@@ -824,6 +868,7 @@ int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int
 int zola_fit_batch(zola_ctx* c, const zola_fit_config* cfg, int B, const float* pixels,
                    const double* init, double* theta, double* nll, double* crlb, int* iters,
                    int* status) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
     if (B == 0) return ZOLA_OK;
@@ -856,8 +901,103 @@ int zola_fit_batch(zola_ctx* c, const zola_fit_config* cfg, int B, const float* 
     return ZOLA_OK;
 }
 
+int zola_fit_ragged(zola_ctx* c, const zola_fit_config* cfg, int B, const long long* roi_offsets,
+                    const int* roi_sizes, const float* pixels, long long n_pixels, const double* init,
+                    double* theta, double* nll, double* crlb, int* iters, int* status) {
+    CtxLock lk_(c);
+    if (!c) return ZOLA_EINVAL;
+    if (!cfg || B < 0 || n_pixels < 0) return set_err(c, ZOLA_EINVAL, "bad ragged-fit arguments");
+    if (B == 0) return ZOLA_OK;
+    if (!roi_offsets || !roi_sizes || !pixels || !init || !theta || !nll || !crlb || !iters || !status)
+        return set_err(c, ZOLA_EINVAL, "null buffer");
+    // group the ROIs by side, in ROI order within a group
+    std::vector<int> sides;
+    std::vector<int> grp(B);
+    for (int b = 0; b < B; b++) {
+        const int R = roi_sizes[b];
+        const long long off = roi_offsets[b];
+        if (R <= 0 || off < 0 || off + (long long)R * R > n_pixels)
+            return set_err(c, ZOLA_EINVAL, "ROI %d (side %d at offset %lld) outside the %lld pixels", b, R, off,
+                           n_pixels);
+        int g = 0;
+        while (g < (int)sides.size() && sides[g] != R) g++;
+        if (g == (int)sides.size()) {
+            if (g == 8) return set_err(c, ZOLA_EINVAL, "more than 8 ROI sides in one ragged batch");
+            const bool ok = c->N == zola::kN ? zola_kernels_support_roi(R) : (R == c->R && zola_kernels_support_roi128(R));
+            if (!ok) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available on the %d grid", R, c->N);
+            sides.push_back(R);
+        }
+        grp[b] = g;
+    }
+    const int G = (int)sides.size();
+    std::vector<int> cnt(G, 0);
+    for (int b = 0; b < B; b++) cnt[grp[b]]++;
+    // device layout: per group [pixels][init][theta][nll][crlb][iters][status], 256 B aligned
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    std::vector<size_t> gpx(G), gin(G);
+    size_t px_total = 0, in_total = 0, out_total = 0;
+    for (int g = 0; g < G; g++) {
+        const size_t n = (size_t)cnt[g];
+        gpx[g] = px_total;
+        px_total += al(n * sides[g] * sides[g] * sizeof(float));
+        gin[g] = in_total;
+        in_total += al(n * 5 * sizeof(double));
+        out_total += al(n * 5 * sizeof(double)) + al(n * sizeof(double)) + al(n * 3 * sizeof(double)) +
+                     2 * al(n * sizeof(int));
+    }
+    int rc = stage(c, px_total + in_total + out_total);
+    if (rc) return rc;
+    // host packing: one contiguous image of the device pixel and init sections
+    std::vector<char> hin(px_total + in_total);
+    std::vector<int> pos(B);
+    {
+        std::vector<int> k(G, 0);
+        for (int b = 0; b < B; b++) {
+            const int g = grp[b], R = sides[g];
+            pos[b] = k[g]++;
+            std::memcpy(hin.data() + gpx[g] + (size_t)pos[b] * R * R * sizeof(float), pixels + roi_offsets[b],
+                        (size_t)R * R * sizeof(float));
+            std::memcpy(hin.data() + px_total + gin[g] + (size_t)pos[b] * 5 * sizeof(double), init + (size_t)b * 5,
+                        5 * sizeof(double));
+        }
+    }
+    char* base = (char*)c->d_stage;
+    HIP_TRY(c, hipMemcpyAsync(base, hin.data(), hin.size(), hipMemcpyHostToDevice, c->stream));
+    const float* gp[8];
+    const double* gi[8];
+    double *gt[8], *gn[8], *gc[8];
+    int *gk[8], *gs[8];
+    char* o = base + px_total + in_total;
+    for (int g = 0; g < G; g++) {
+        const size_t n = (size_t)cnt[g];
+        gp[g] = (const float*)(base + gpx[g]);
+        gi[g] = (const double*)(base + px_total + gin[g]);
+        gt[g] = (double*)o; o += al(n * 5 * sizeof(double));
+        gn[g] = (double*)o; o += al(n * sizeof(double));
+        gc[g] = (double*)o; o += al(n * 3 * sizeof(double));
+        gk[g] = (int*)o; o += al(n * sizeof(int));
+        gs[g] = (int*)o; o += al(n * sizeof(int));
+    }
+    rc = zola_fit_mixed_dev(c, cfg, G, sides.data(), cnt.data(), gp, gi, gt, gn, gc, gk, gs);
+    if (rc) return rc;
+    std::vector<char> hout(out_total);
+    HIP_TRY(c, hipMemcpyAsync(hout.data(), base + px_total + in_total, out_total, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; b++) {
+        const int g = grp[b], p = pos[b];
+        auto host = [&](const void* dev) { return hout.data() + ((const char*)dev - (base + px_total + in_total)); };
+        std::memcpy(theta + (size_t)b * 5, (const double*)host(gt[g]) + (size_t)p * 5, 5 * sizeof(double));
+        nll[b] = ((const double*)host(gn[g]))[p];
+        std::memcpy(crlb + (size_t)b * 3, (const double*)host(gc[g]) + (size_t)p * 3, 3 * sizeof(double));
+        iters[b] = ((const int*)host(gk[g]))[p];
+        status[b] = ((const int*)host(gs[g]))[p];
+    }
+    return ZOLA_OK;
+}
+
 int zola_fit_init_dev(zola_ctx* c, int B, const float* pixels, int nZ, double zmin_t,
                       double zstep_t, double zfocus, double* init) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (B < 0 || nZ <= 0 || !(zstep_t > 0)) return set_err(c, ZOLA_EINVAL, "bad init arguments");
     if (B == 0) return ZOLA_OK;

@@ -14,8 +14,9 @@ bool zola_kernels_support_roi(int R);
 hipError_t zola_launch_psf_forward64(const zola::Model64& M, int R, int B, const float* params,
                                      float* out, hipStream_t st);
 
-// quad: 0 one wave per ROI (zola_psf64.hpp), 1 four waves per ROI (zola_psf64q.hpp)
-hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, int quad, const zola::FitArgs& A,
+// layout: 0 one wave per ROI (zola_psf64.hpp), 1 four waves per ROI (zola_psf64q.hpp),
+// 2 two waves per ROI (zola_psf64p.hpp; LM form, R = 17 and 21)
+hipError_t zola_launch_fit64(const zola::Model64& M, int R, int mode, int layout, const zola::FitArgs& A,
                              hipStream_t st);
 
 // tstat: per-template {sum t, sum t^2 - (sum t)^2 / R^2} from zola_launch_template_stats64

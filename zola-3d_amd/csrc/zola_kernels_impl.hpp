@@ -267,6 +267,16 @@ hipError_t launch_fit64q_r(const Model64& M, int mode, const FitArgs& A, hipStre
     return hipGetLastError();
 }
 
+// N = 64 with two waves (one block) per ROI, LM form, see zola_psf64p.hpp.
+template <int R>
+hipError_t launch_fit64p_r(const Model64& M, const FitArgs& A, hipStream_t st) {
+    hipError_t e;
+    const int g = fit_grid(k_fit_lm64p<R>, 128, 1, A, st, e);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fit_lm64p<R>), dim3(g), dim3(128), 0, st, M, A);
+    return hipGetLastError();
+}
+
 // N = 128: four waves per PSF (one block), see zola_psf128.hpp.
 template <int R>
 __global__ __launch_bounds__(256) void k_psf_forward128(Model128 M, int B, const float* __restrict__ params,

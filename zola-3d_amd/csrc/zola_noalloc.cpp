@@ -55,6 +55,7 @@ int zola_scratch_setup(zola_ctx* c) {
 
 extern "C" int zola_psf_backward_noalloc_dev(zola_ctx* c, int B, const float* params, const float* w,
                                              float* dparams, double* da) {
+    CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
     if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");

@@ -55,6 +55,7 @@ struct Model64 {
     const float* amp;
     const float* kz;
     const float* kzo;
+    const float4* tab4;  // the same four tables interleaved per window point {phi, amp, kz, kzo}
     float kappa;       // 2 pi / (dx N): kx = kappa (n_u + 1/2)
     float h1[64];      // blur kernel over signed offset d (index d & 63)
     float hd[64];      // d/dt of the blur kernel (spectral), divided by dx

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libzola_hip.so")
 
 FIT_REFERENCE = 0
 FIT_LM = 1
-LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_QUAD = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_QUAD, LAYOUT_PAIR = 0, 1, 2, 3
 ST_CONVERGED, ST_MAXITER, ST_NAN, ST_LOWPHOTON = 1, 2, 4, 8
 
 SYMBOLS = [
@@ -35,6 +35,7 @@ SYMBOLS = [
     "zola_calib_json_format", "zola_calib_json_parse", "zola_loc_csv_header", "zola_loc_csv_rows",
     "zola_formats_last_error", "zola_bead_prepare", "zola_bead_prepare_scmos",
     "zola_psf_backward_noalloc_dev", "zola_calib_eval_full_dev", "zola_calib_fit_full",
+    "zola_fit_ragged",
 ]
 CAND_FIELDS = 7
 LOC_FIELDS = 12
@@ -129,6 +130,8 @@ def lib():
             L.zola_fit_batch_ext_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 9
         if hasattr(L, "zola_fit_mixed_dev"):
             L.zola_fit_mixed_dev.argtypes = [vp, C.POINTER(FitConfig), C.c_int] + [vp] * 9
+        if hasattr(L, "zola_fit_ragged"):
+            L.zola_fit_ragged.argtypes = [vp, C.POINTER(FitConfig), C.c_int, vp, vp, vp, C.c_longlong] + [vp] * 6
         if hasattr(L, "zola_detector_create"):
             L.zola_detector_create.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double,
                                                C.c_double, C.c_double, C.c_double, C.POINTER(vp)]
@@ -372,17 +375,44 @@ class Context:
                                          _ptr(status)), "fit_batch")
         return dict(theta=theta, nll=nll, crlb=crlb, iters=iters, status=status)
 
+    def fit_ragged(self, cfg, rois, init):
+        """Host form for ROIs of several sides: rois is a sequence of square
+        float32 arrays (sides the library covers on this grid), init (B, 5)."""
+        B = len(rois)
+        sizes = np.zeros(B, np.int32)
+        offs = np.zeros(B, np.int64)
+        parts = []
+        n = 0
+        for b, r in enumerate(rois):
+            a = np.ascontiguousarray(r, dtype=np.float32)
+            if a.ndim != 2 or a.shape[0] != a.shape[1]:
+                raise ValueError(f"rois[{b}]: expected a square 2-D array, got shape {a.shape}")
+            sizes[b] = a.shape[0]
+            offs[b] = n
+            n += a.size
+            parts.append(a.ravel())
+        px = np.concatenate(parts) if parts else np.zeros(0, np.float32)
+        ini = _host(init, "init", (B, 5), "f8")
+        theta = np.zeros((B, 5)); nll = np.zeros(B); crlb = np.zeros((B, 3))
+        iters = np.zeros(B, np.int32); status = np.zeros(B, np.int32)
+        self._check(lib().zola_fit_ragged(self._h, C.byref(cfg), B, _ptr(offs), _ptr(sizes), _ptr(px), n,
+                                          _ptr(ini), _ptr(theta), _ptr(nll), _ptr(crlb), _ptr(iters),
+                                          _ptr(status)), "fit_ragged")
+        return dict(theta=theta, nll=nll, crlb=crlb, iters=iters, status=status)
+
     def set_fit_layout(self, R, mode, layout):
-        """Work layout of the 64-grid fits of side R: LAYOUT_AUTO / WAVE / QUAD."""
+        """Work layout of the 64-grid fits of side R: LAYOUT_AUTO / WAVE / QUAD / PAIR
+        (PAIR: LM form, R <= 21)."""
         self._check(lib().zola_set_fit_layout(self._h, R, mode, layout), "set_fit_layout")
 
     def autotune_fit(self, R, mode, B=8192):
-        """Times both layouts on a synthetic batch, keeps the faster; returns
-        (layout, ms one wave per ROI, ms four waves per ROI)."""
+        """Times every layout that exists for (R, mode) on a synthetic batch, keeps
+        the fastest; returns (layout, ms one wave per ROI, ms four waves per ROI,
+        ms two waves per ROI or 0 where that layout does not exist)."""
         lay = C.c_int(0)
-        ms = np.zeros(2)
+        ms = np.zeros(3)
         self._check(lib().zola_autotune_fit(self._h, R, mode, B, C.byref(lay), _ptr(ms)), "autotune_fit")
-        return lay.value, float(ms[0]), float(ms[1])
+        return lay.value, float(ms[0]), float(ms[1]), float(ms[2])
 
     def search_psf_center(self, zfocus, axial_range):
         """Centre of the template z range (SearchPSFcenter.getPosition), um."""
