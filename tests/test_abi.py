@@ -37,7 +37,7 @@ def test_invalid_arguments_are_rejected_before_device_use():
     assert b"invalid optics" in lib.zola_last_error(h)
     lib.zola_ctx_destroy(h)
     h = C.c_void_p()
-    big = Z.optics(size_fft=256)
+    big = Z.optics(size_fft=512)
     assert lib.zola_ctx_create(C.byref(big), 21, None, 0, C.byref(h)) == -2
     assert b"pupil grid" in lib.zola_last_error(h)
     lib.zola_ctx_destroy(h)

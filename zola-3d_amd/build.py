@@ -15,7 +15,8 @@ OUT = os.path.join(HERE, "libzola_hip.so")
 # the 128-grid instantiations compile longest: they start first
 SOURCES = (sorted(os.path.basename(p) for p in glob.glob(os.path.join(CSRC, "zola_inst_n128*.hip"))) +
            ["zola_dispatch.hip", "zola_host.cpp", "zola_calib.cpp", "zola_detect.hip", "zola_frames.cpp",
-            "zola_center.cpp", "zola_formats.cpp", "zola_beads.hip", "zola_noalloc.cpp", "zola_center_dev.hip"] +
+            "zola_center.cpp", "zola_formats.cpp", "zola_beads.hip", "zola_noalloc.cpp", "zola_center_dev.hip",
+            "zola_grid256.hip"] +
            sorted(os.path.basename(p) for p in glob.glob(os.path.join(CSRC, "zola_inst_*.hip"))
                   if not os.path.basename(p).startswith("zola_inst_n128")))
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature",

@@ -119,6 +119,7 @@ struct Search {
 extern "C" int zola_search_psf_center(zola_ctx* c, double zfocus, double axial_range, double* center) {
     CtxLock lk_(c);
     if (!c || !center) return ZOLA_EINVAL;
+    ZOLA_REFUSE_GRID256(c);
     if (!(axial_range > 0)) return set_err(c, ZOLA_EINVAL, "axial range must be positive");
     Search S{c, c->N, zfocus};
     double stepZ = c->o.xystep * 5;

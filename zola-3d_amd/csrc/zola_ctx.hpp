@@ -43,6 +43,12 @@ struct zola_ctx {
     size_t qbar_bytes = 0;
     int* d_queue = nullptr;       // ROI counters of the persistent fits (one per concurrent group)
     float* scratch = nullptr;     // device address of the fixed allocation-free scratch (zola_noalloc.cpp)
+    // N = 256 (zola_grid256.hip): FP64 pupil window, Gaussian spectra, twiddles, work buffers
+    double4* d_tab256 = nullptr;
+    double* d_gk256 = nullptr;
+    double2* d_tw256 = nullptr;
+    char* d_work256 = nullptr;
+    size_t work256_bytes = 0;
     // graph of the last mixed-size fit (one kernel node per ROI-size group)
     hipGraphExec_t mixed_exec = nullptr;
     hipStream_t side[8] = {};
@@ -92,6 +98,20 @@ int zola_center_fisher_dev(zola_ctx* c, int nz, const double* zs, double zfocus,
 // Kernel layout of a 64-grid fit: 0 one wave per ROI, 1 four waves per ROI,
 // 2 two waves per ROI (LM form, R <= 21).
 int zola_fit_layout(const zola_ctx* c, int R, int mode);
+
+constexpr int kGrid256 = 256;  // pupil grid of the calibration-only contexts
+
+inline int set_err(zola_ctx* c, int code, const char* fmt, ...);
+
+// Entry points of the localization path (fits, initial values, detection, the
+// focus-centre search) refuse N = 256 contexts, which serve PSF synthesis, the
+// adjoint and calibration.
+#define ZOLA_REFUSE_GRID256(c)                                                                   \
+    do {                                                                                         \
+        if ((c)->N == kGrid256)                                                                  \
+            return set_err((c), ZOLA_EUNSUPPORTED,                                               \
+                           "N=256 contexts serve PSF synthesis, the adjoint and calibration");   \
+    } while (0)
 
 inline int set_err(zola_ctx* c, int code, const char* fmt, ...) {
     if (c) {

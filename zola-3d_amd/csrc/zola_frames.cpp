@@ -293,6 +293,7 @@ int zola_detector_create(zola_ctx* c, int W, int H, double zmin, double zmax, do
     CtxLock lk_(c);
     if (!c || !out) return ZOLA_EINVAL;
     *out = nullptr;
+    ZOLA_REFUSE_GRID256(c);
     if (W <= 0 || H <= 0 || !(zstep > 0) || !(zmax > zmin))
         return set_err(c, ZOLA_EINVAL, "bad detector geometry or z range");
     const int R = c->R;

@@ -20,6 +20,7 @@
 #include "zola_kernels.h"
 #include "zola_adjoint64.hpp"
 #include "zola_ctx.hpp"
+#include "zola_grid256.h"
 
 namespace {
 
@@ -56,6 +57,11 @@ static void compute_phase(zola_ctx* c) {
 }
 
 static int compute_h1(zola_ctx* c) {
+    // the 256 grid blurs by FFT with the 2-D Gaussian spectrum (zola256_upload_gauss)
+    if (c->N == kGrid256) {
+        c->model_ver++;
+        return ZOLA_OK;
+    }
     // G1(k) = exp(-(k' - s2/2)^2 / (2 sigF^2)), k' = (k + s2/2) % s2, sigF = s2/(2 pi sigma);
     // h1(d) = (1/s2) sum_k G1(k) cos(2 pi k d / s2)  (G1 is even, so h1 is real).
     const int s2 = c->s2;
@@ -151,6 +157,7 @@ static int upload_taps128(zola_ctx* c) {
 
 static int upload_tables(zola_ctx* c) {
     c->model_ver++;
+    if (c->N == kGrid256) return zola256_upload_tables(c);
     if (c->N == zola::kN2) {
         std::vector<float4> tab(zola::kW2 * zola::kW2, make_float4(0.f, 0.f, 0.f, 0.f));
         for (int p = 0; p < c->D; p++)
@@ -200,6 +207,13 @@ static bool layout_available(int R, int mode, int layout) {
 }
 
 int zola_psf_launch(zola_ctx* c, int B, const float* params, float* out, hipStream_t st) {
+    if (c->N == kGrid256) {
+        Eval256 e;
+        e.B = B;
+        e.params = params;
+        e.out = out;
+        return zola256_run(c, e, st);
+    }
     if (c->N == zola::kN2) HIP_TRY(c, zola_launch_psf_forward128(c->model128, c->R, B, params, out, st));
     else HIP_TRY(c, zola_launch_psf_forward64(c->model, c->R, B, params, out, st));
     return ZOLA_OK;
@@ -258,14 +272,16 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
     c->radius_px = std::sqrt(c->ringsize) * (N * o->xystep);
     c->kappa = 2 * PI / (o->xystep * N);
     *out = c;
-    if (N != zola::kN && N != zola::kN2)
-        return set_err(c, ZOLA_EUNSUPPORTED, "pupil grid N=%d: this build covers N=64 and N=128", N);
-    // the 64-grid kernels blur on the whole grid; the 128-grid ones on s2 = 32, 64 or 128
+    if (N != zola::kN && N != zola::kN2 && N != kGrid256)
+        return set_err(c, ZOLA_EUNSUPPORTED, "pupil grid N=%d: this build covers N=64, 128 and 256", N);
+    // the 64-grid kernels blur on the whole grid; the 128-grid ones on s2 = 32, 64 or 128;
+    // the 256 grid (calibration) on any s2 of 16..256
     if (N == zola::kN && c->s2 != N)
         return set_err(c, ZOLA_EUNSUPPORTED, "blur grid s2=%d must equal N=%d (R >= N/4)", c->s2, N);
-    if (N == zola::kN ? !zola_kernels_support_roi(R) : !zola_kernels_support_roi128(R))
+    if (N == kGrid256 ? (R < 8 || R > kGrid256)
+                      : (N == zola::kN ? !zola_kernels_support_roi(R) : !zola_kernels_support_roi128(R)))
         return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not instantiated for N=%d", R, N);
-    const int halfw = N == zola::kN ? zola::kW / 2 : zola::kW2 / 2;
+    const int halfw = N == zola::kN ? zola::kW / 2 : N == zola::kN2 ? zola::kW2 / 2 : N / 4;
 
     const double left = (o->nwat / o->wavelength) * (o->nwat / o->wavelength);
     const double left_oil = (o->noil / o->wavelength) * (o->noil / o->wavelength);
@@ -337,6 +353,12 @@ int zola_ctx_create(const zola_optics* o, int R, const double* coeffs, int J, zo
         if (rs) return rs;
     }
     HIP_TRY(c, hipMalloc((void**)&c->d_queue, 8 * sizeof(int)));
+    if (N == kGrid256) {
+        int rc = zola256_setup(c);
+        if (rc) return rc;
+        c->err.clear();
+        return ZOLA_OK;
+    }
     if (N == zola::kN2) {
         HIP_TRY(c, hipMalloc((void**)&c->d_tab128, zola::kW2 * zola::kW2 * sizeof(float4)));
         HIP_TRY(c, hipMalloc((void**)&c->d_taps128, (2 + 64) * zola::kN2 * sizeof(float)));
@@ -384,6 +406,7 @@ int zola_ctx_destroy(zola_ctx* c) {
     if (c->d_Zw) (void)hipFree(c->d_Zw);
     if (c->d_qbar) (void)hipFree(c->d_qbar);
     if (c->d_queue) (void)hipFree(c->d_queue);
+    zola256_free(c);
     if (c->mixed_exec) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipGraphExecDestroy(c->mixed_exec);
@@ -430,6 +453,7 @@ int zola_set_sigma(zola_ctx* c, double sigma) {
     c->o.sigma = sigma;
     int rc = compute_h1(c);
     if (rc == ZOLA_OK && c->N == zola::kN2) rc = upload_taps128(c);
+    if (rc == ZOLA_OK && c->N == kGrid256) rc = zola256_upload_gauss(c);
     return rc;
 }
 
@@ -475,9 +499,19 @@ int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* 
                           float* dparams, double* da) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
-    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
+    if (c->N != zola::kN && c->N != kGrid256)
+        return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 and 256 pupil grids");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
+    if (c->N == kGrid256) {
+        Eval256 e;
+        e.B = B;
+        e.params = params;
+        e.w = w;
+        e.dparams = dparams;
+        e.da = da;
+        return zola256_run(c, e, c->stream);
+    }
     float* qbar = nullptr;
     if (da && c->J > 0) {
         const size_t need = (size_t)B * 1024 * sizeof(float);
@@ -505,7 +539,8 @@ int zola_psf_backward(zola_ctx* c, int B, const float* params, const float* w, f
                       double* da) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
-    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
+    if (c->N != zola::kN && c->N != kGrid256)
+        return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 and 256 pupil grids");
     if (B < 0 || (B > 0 && (!params || !w || !dparams))) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
     const size_t R2 = (size_t)c->R * c->R;
@@ -543,11 +578,26 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
                         double* damp, double* dbg, double* da) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
-    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
+    if (c->N != zola::kN && c->N != kGrid256)
+        return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 and 256 pupil grids");
     if (B < 0) return set_err(c, ZOLA_EINVAL, "bad batch");
     if (B == 0) return ZOLA_OK;
     if (!params || !pixels || !amp || !bg || !nll || !dparams || !damp || !dbg)
         return set_err(c, ZOLA_EINVAL, "null buffer");
+    if (c->N == kGrid256) {
+        Eval256 e;
+        e.B = B;
+        e.params = params;
+        e.pixels = pixels;
+        e.amp = amp;
+        e.bg = bg;
+        e.nll = nll;
+        e.dparams = dparams;
+        e.damp = damp;
+        e.dbg = dbg;
+        e.da = da;
+        return zola256_run(c, e, c->stream);
+    }
     float* qbar = nullptr;
     if (da && c->J > 0) {
         const size_t need = (size_t)B * 1024 * sizeof(float);
@@ -582,12 +632,30 @@ int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, co
                              float* dparams, double* damp, double* dpoly, double* dsigma, double* da) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
-    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 pupil grid");
+    if (c->N != zola::kN && c->N != kGrid256)
+        return set_err(c, ZOLA_EUNSUPPORTED, "the adjoint covers the 64 and 256 pupil grids");
     if (nb < 0 || S <= 0 || (long)nb * S > (1L << 30)) return set_err(c, ZOLA_EINVAL, "bad batch");
     const int B = nb * S;
     if (B == 0) return ZOLA_OK;
     if (!params || !pixels || !amp || !poly || !nll || !dparams || !damp)
         return set_err(c, ZOLA_EINVAL, "null buffer");
+    if (c->N == kGrid256) {
+        Eval256 e;
+        e.B = B;
+        e.params = params;
+        e.pixels = pixels;
+        e.amp = amp;
+        e.S = S;
+        e.poly = poly;
+        e.offset = offset;
+        e.nll = nll;
+        e.dparams = dparams;
+        e.damp = damp;
+        e.dpoly = dpoly;
+        e.dsig = dsigma;
+        e.da = da;
+        return zola256_run(c, e, c->stream);
+    }
     float* qbar = nullptr;
     if (da && c->J > 0) {
         const size_t need = (size_t)B * 1024 * sizeof(float);
@@ -646,6 +714,7 @@ int zola_fit_batch_ext_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const
                            int* iters, int* status) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
+    ZOLA_REFUSE_GRID256(c);
     if (!cfg || B < 0) return set_err(c, ZOLA_EINVAL, "bad fit arguments");
     if (B == 0) return ZOLA_OK;
     if (!pixels || !init || !theta || !nll || !crlb || !iters || !status)
@@ -782,6 +851,7 @@ int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int
                        double* const* nll, double* const* crlb, int* const* iters, int* const* status) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
+    ZOLA_REFUSE_GRID256(c);
     if (!cfg || G <= 0 || G > 8 || !R || !B || !pixels || !init || !theta || !nll || !crlb || !iters || !status)
         return set_err(c, ZOLA_EINVAL, "bad mixed-fit arguments (1 <= G <= 8)");
     if (cfg->mode != ZOLA_FIT_REFERENCE && cfg->mode != ZOLA_FIT_LM)
@@ -906,6 +976,7 @@ int zola_fit_ragged(zola_ctx* c, const zola_fit_config* cfg, int B, const long l
                     double* theta, double* nll, double* crlb, int* iters, int* status) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
+    ZOLA_REFUSE_GRID256(c);
     if (!cfg || B < 0 || n_pixels < 0) return set_err(c, ZOLA_EINVAL, "bad ragged-fit arguments");
     if (B == 0) return ZOLA_OK;
     if (!roi_offsets || !roi_sizes || !pixels || !init || !theta || !nll || !crlb || !iters || !status)
@@ -999,6 +1070,7 @@ int zola_fit_init_dev(zola_ctx* c, int B, const float* pixels, int nZ, double zm
                       double zstep_t, double zfocus, double* init) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
+    ZOLA_REFUSE_GRID256(c);
     if (B < 0 || nZ <= 0 || !(zstep_t > 0)) return set_err(c, ZOLA_EINVAL, "bad init arguments");
     if (B == 0) return ZOLA_OK;
     if (!pixels || !init) return set_err(c, ZOLA_EINVAL, "null buffer");
