@@ -192,6 +192,92 @@ def calibration_bench(torch, Z, stream, cpu=True):
     return out
 
 
+def calibration256_bench(torch, Z, stream, max_iter=400):
+    """The plugin's default calibration geometry: sizeFFT 140 (pupil grid 256),
+    64 px bead patches (blur grid 128), 5 beads x 41 slices, 45 modes, the full
+    model, FP64 throughout.  One forward+adjoint evaluation timed with HIP events,
+    then L-BFGS from a zero start, bounded at max_iter iterations."""
+    Rc, J, nb, S = 64, 45, 5, 41
+    rng = np.random.default_rng(0x20A1D3 + 30)
+    truth = rng.normal(0.0, 0.15, J)
+    truth[:3] = 0.0
+    truth[Z.nm_to_slot(*Z.noll_to_nm(5))] = 0.8
+    dz = (np.arange(S) - S // 2) * 0.05
+    xyz = np.stack([rng.uniform(-0.05, 0.05, nb), rng.uniform(-0.05, 0.05, nb), rng.uniform(-0.05, 0.05, nb)], 1)
+    A = 2e5 * (1.0 + 0.15 * np.sin(np.arange(S)[None, :] / 5.0 + rng.uniform(0, 6, (nb, 1))))
+    opt = Z.optics(size_fft=140)
+    gen = Z.Context(opt, Rc, truth)
+    gen.set_stream(stream.cuda_stream)
+    par = np.zeros((nb * S, 4), np.float32)
+    for b in range(nb):
+        for s_ in range(S):
+            par[b * S + s_] = [xyz[b, 0], xyz[b, 1], 0.0, dz[s_] + xyz[b, 2]]
+    P = torch.from_numpy(par).cuda()
+    psf = torch.empty(nb * S, Rc, Rc, device="cuda")
+    gen.psf_forward_dev(P, psf)
+    amp_t = torch.from_numpy(A.reshape(-1)).float().cuda()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x20A1D3 + 30)
+    stacks = torch.poisson(amp_t[:, None, None] * psf + 10.0, generator=g).float().contiguous()
+    ctx = Z.Context(opt, Rc, np.zeros(J))
+    ctx.set_stream(stream.cuda_stream)
+    B = nb * S
+    amp = torch.from_numpy(A.reshape(-1)).cuda()
+    pol = torch.zeros(nb, 6, dtype=torch.float64, device="cuda")
+    pol[:, 0] = 10.0
+    nll = torch.empty(B, dtype=torch.float64, device="cuda")
+    dpar = torch.empty(B, 4, device="cuda")
+    dA = torch.empty_like(nll)
+    dpoly = torch.empty(B, 6, dtype=torch.float64, device="cuda")
+    dsig = torch.empty_like(nll)
+    da = torch.zeros(J, dtype=torch.float64, device="cuda")
+    args = (nb, S, P, stacks, amp, pol, nll, dpar, dA)
+    kw = dict(dpoly=dpoly, dsigma=dsig, da=da)
+    ctx.calib_eval_full_dev(*args, **kw)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.calib_eval_full_dev(*args, **kw)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eval_ms = e0.elapsed_time(e1) / reps
+    host = stacks.cpu().numpy().reshape(nb, S, Rc, Rc)
+    c0 = np.array([np.median(np.concatenate([host[b, :, 0, :].ravel(), host[b, :, -1, :].ravel()]))
+                   for b in range(nb)])
+    A0 = host.reshape(nb, S, -1).sum(2) - Rc * Rc * c0[:, None]
+    poly0 = np.zeros((nb, 6))
+    poly0[:, 0] = c0
+    mask = np.ones(J, np.int32)
+    mask[:3] = 0
+    t0 = time.perf_counter()
+    res = ctx.calib_fit_full(host, dz, A0, np.zeros((nb, 3)), poly0, np.zeros(J), sigma=1.0, fit_mask=mask,
+                             max_iter=max_iter, gtol=1e-4)
+    dt = time.perf_counter() - t0
+    # algorithmic work of one evaluation: a forward and an adjoint 256 x 256 pupil
+    # transform per PSF plus the Zernike projection (2 D J)
+    fft = 5 * 256 * 256 * np.log2(256 * 256)
+    W = B * (2 * fft + 2 * ctx.D * J)
+    out = {
+        "workload": "plugin default calibration grid: sizeFFT 140 (pupil 256), 64 px patches (blur grid 128), "
+                    "5 beads x 41 slices, 45 Zernike modes, full model, FP64; L-BFGS from a zero start, "
+                    f"at most {max_iter} iterations",
+        "iterations": res["iters"], "evaluations": res["evals"],
+        "iters_per_s": round(res["iters"] / dt, 2), "evals_per_s": round(res["evals"] / dt, 2),
+        "eval_ms": round(eval_ms, 4),
+        "max_coeff_err_rad": round(float(np.abs(res["coeffs"] - truth)[3:].max()), 5),
+        "max_coeff_err_rad_but_defocus": round(float(np.delete(np.abs(res["coeffs"] - truth), [0, 1, 2, 4]).max()), 5),
+        "sigma": round(res["sigma"], 5),
+        "nll": res["nll"],
+        "roofline": {"bound": "valu-fp64", "achieved": round(W / (eval_ms * 1e-3) / 1e12, 4), "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": round(W / (eval_ms * 1e-3) / 1e12 / 78.6, 5),
+                     "algorithmic_flops_per_eval": float(W)},
+    }
+    gen.close()
+    ctx.close()
+    return out
+
+
 def _cgroup_cpus():
     """CPUs granted by the cgroup v2 quota (cpu.max), or None without a quota."""
     try:
@@ -683,6 +769,7 @@ def main():
         }
     if not args.no_calib:
         out["calibration"] = calibration_bench(torch, Z, stream, cpu=not args.no_cpu)
+        out["calibration_n256"] = calibration256_bench(torch, Z, stream)
     if not args.no_n128:
         out["depth_n128"] = depth128_bench(torch, Z, stream, args.n128_batch)
     if not args.no_mixed:

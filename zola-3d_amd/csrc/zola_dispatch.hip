@@ -27,28 +27,32 @@ hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, 
 template <int R>
 hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t st);
 
-// da_j += sum_b sum_p Z_j(p) qbar[b][p] (FP64), one block per coefficient.
-__global__ __launch_bounds__(256) void k_project_zernike(int B, int J, const double* __restrict__ Zw,
-                                                         const float* __restrict__ qbar,
-                                                         double* __restrict__ da) {
-    const int jc = blockIdx.x;
-    if (jc >= J) return;
-    double acc = 0.0;
-    for (int p = threadIdx.x; p < 1024; p += blockDim.x) {
-        const double zv = Zw[(size_t)jc * 1024 + p];
-        if (zv == 0.0) continue;
-        double s = 0.0;
-        for (int b = 0; b < B; b++) s += (double)qbar[(size_t)b * 1024 + p];
-        acc += zv * s;
-    }
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+// da_j += sum_p Z_j(p) Q(p), Q(p) = sum_b qbar[b][p] (FP64), by one block of
+// 1024 threads: thread p forms Q(p) once (the PSF sum is shared by every
+// coefficient), then the products of eight coefficients at a time are summed
+// over the window by LDS trees in a fixed order.
+__global__ __launch_bounds__(1024) void k_project_zernike(int B, int J, const double* __restrict__ Zw,
+                                                          const float* __restrict__ qbar,
+                                                          double* __restrict__ da) {
+    constexpr int kG = 8;
+    __shared__ double red[kG][1024];
+    const int p = threadIdx.x;
+    double q = 0.0;
+    for (int b = 0; b < B; b++) q += (double)qbar[(size_t)b * 1024 + p];
+    for (int j0 = 0; j0 < J; j0 += kG) {
+#pragma unroll
+        for (int g = 0; g < kG; g++) red[g][p] = j0 + g < J ? Zw[(size_t)(j0 + g) * 1024 + p] * q : 0.0;
+        __syncthreads();
+        for (int o = 512; o > 0; o >>= 1) {
+            if (p < o) {
+#pragma unroll
+                for (int g = 0; g < kG; g++) red[g][p] += red[g][p + o];
+            }
+            __syncthreads();
+        }
+        if (p < kG && j0 + p < J) da[j0 + p] += red[p][0];
         __syncthreads();
     }
-    if (threadIdx.x == 0) da[jc] += red[0];
 }
 
 }  // namespace zola
@@ -116,7 +120,7 @@ hipError_t zola_launch_adjoint64(const Model64& M, int R, const AdjArgs& A, hipS
 hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* da,
                                        hipStream_t st) {
     if (J <= 0 || B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_project_zernike, dim3(J), dim3(256), 0, st, B, J, Zw, qbar, da);
+    hipLaunchKernelGGL(k_project_zernike, dim3(1), dim3(1024), 0, st, B, J, Zw, qbar, da);
     return hipGetLastError();
 }
 

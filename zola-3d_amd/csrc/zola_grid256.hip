@@ -397,18 +397,21 @@ __global__ __launch_bounds__(256) void k_dparams256(Geo g, const double* qbar, f
     if (threadIdx.x < 4) dparams[(size_t)b * 4 + threadIdx.x] = (float)a[threadIdx.x];
 }
 
-// 7b: da_j += sum_b sum_w Z_j(w) qbar_b(w), one block per coefficient.
-__global__ __launch_bounds__(256) void k_project256(int B, int J, const double* Zw, const double* qbar, double* da) {
+// 7b: Q(w) = sum_b qbar_b(w), one thread per window point.
+__global__ __launch_bounds__(256) void k_qsum256(int B, const double* qbar, double* Q) {
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= kWin) return;
+    double s = 0.0;
+    for (int b = 0; b < B; b++) s += qbar[(size_t)b * kWin + w];
+    Q[w] = s;
+}
+
+// 7c: da_j += sum_w Z_j(w) Q(w), one block per coefficient.
+__global__ __launch_bounds__(256) void k_project256(int J, const double* Zw, const double* Q, double* da) {
     __shared__ double red[256];
     const int jz = blockIdx.x;
     double a[1] = {0.0};
-    for (int w = threadIdx.x; w < kWin; w += 256) {
-        const double zv = Zw[(size_t)jz * kWin + w];
-        if (zv == 0.0) continue;
-        double s = 0.0;
-        for (int b = 0; b < B; b++) s += qbar[(size_t)b * kWin + w];
-        a[0] += zv * s;
-    }
+    for (int w = threadIdx.x; w < kWin; w += 256) a[0] += Zw[(size_t)jz * kWin + w] * Q[w];
     block_sum256<1>(a, red);
     if (threadIdx.x == 0) da[jz] += a[0];
 }
@@ -511,7 +514,8 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
     // per-PSF work: S1 (s2 x 128), E, T, FI (s2 x s2 each), Bc, dBc (R x R), qbar (128 x 128 doubles)
     const size_t nS1 = al256((size_t)B * s2 * kW4 * sizeof(double2)), nE = al256((size_t)B * s2 * s2 * sizeof(double2));
     const size_t nB = al256((size_t)B * R * R * sizeof(double2)), nQ = al256((size_t)B * kWin * sizeof(double));
-    const size_t need = nS1 + 3 * nE + 2 * nB + (adjoint ? nQ : 0);
+    const size_t nQs = al256((size_t)kWin * sizeof(double));
+    const size_t need = nS1 + 3 * nE + 2 * nB + (adjoint ? nQ + nQs : 0);
     if (need > c->work256_bytes) {
         if (c->d_work256) HIP_TRY(c, hipFree(c->d_work256));
         c->d_work256 = nullptr;
@@ -527,6 +531,7 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
     double2* Bc = (double2*)p; p += nB;
     double2* dBc = (double2*)p; p += nB;
     double* qbar = (double*)p;
+    double* Qs = (double*)(p + nQ);
     const Geo g{c->d_tab256, c->kappa, s2, R};
     const double2* tw = c->d_tw256;
     const double* gk = c->d_gk256;
@@ -576,7 +581,9 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
         HIP_TRY(c, hipGetLastError());
     }
     if (e.da && c->J > 0) {
-        hipLaunchKernelGGL(k_project256, dim3(c->J), dim3(256), 0, st, B, c->J, c->d_Zw, qbar, e.da);
+        hipLaunchKernelGGL(k_qsum256, dim3(kWin / 256), dim3(256), 0, st, B, qbar, Qs);
+        HIP_TRY(c, hipGetLastError());
+        hipLaunchKernelGGL(k_project256, dim3(c->J), dim3(256), 0, st, c->J, c->d_Zw, Qs, e.da);
         HIP_TRY(c, hipGetLastError());
     }
     return ZOLA_OK;
