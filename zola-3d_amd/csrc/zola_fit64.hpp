@@ -95,6 +95,38 @@ __device__ __forceinline__ double scmos_term(const float* vroi, int lane, int wr
     return vroi ? (double)vroi[SY::template pix<R>(lane, wr, t)] : 0.0;
 }
 
+// Natural logarithm of a positive, finite, normal x (the Poisson means of a fit
+// are >= 0.01), to 2 ulp: x = 2^e m with m in [sqrt(1/2), sqrt(2)), and
+// ln m = 2 atanh(s) = 2 s sum_k s^2k / (2k + 1), s = (m - 1)/(m + 1),
+// |s| < 0.1716, summed to k = 10 (truncation below 1e-17 relative); e ln 2 is
+// added in two parts (ln 2 split as in fdlibm's e_log.c) so that e ln2_hi is
+// exact.  About a quarter of the instructions of the library's log, which
+// carries extra precision the likelihood does not need.
+__device__ __forceinline__ double log_pos(double x) {
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m += m;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (f + 2.0);
+    const double s2 = s * s;
+    double t = 1.0 / 21.0;
+    t = t * s2 + 1.0 / 19.0;
+    t = t * s2 + 1.0 / 17.0;
+    t = t * s2 + 1.0 / 15.0;
+    t = t * s2 + 1.0 / 13.0;
+    t = t * s2 + 1.0 / 11.0;
+    t = t * s2 + 1.0 / 9.0;
+    t = t * s2 + 1.0 / 7.0;
+    t = t * s2 + 1.0 / 5.0;
+    t = t * s2 + 1.0 / 3.0;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double de = (double)e, ss = s + s;
+    return de * ln2_hi + (ss + (ss * s2 * t + de * ln2_lo));
+}
+
 // Poisson NLL without ln n! (Localization.computeLikelihood, 1125-1143), FP64;
 // mu = A psf + B, plus the sCMOS term var/gain^2 of the pixel when vroi is
 // given (1131-1132).
@@ -109,7 +141,7 @@ __device__ __forceinline__ double nll64(int lane, int wr, typename SY::Red& red,
     for (int t = 0; t < RPL; t++) {
         if (SY::template valid<R>(lane, wr, t)) {
             const double mu = (double)psf[t] * A + Bg + scmos_term<SY, R>(vroi, lane, wr, t);
-            acc += mu - (double)n[t] * log(mu);
+            acc += mu - (double)n[t] * log_pos(mu);
         }
     }
     return SY::sum(acc, red, lane, wr);
