@@ -1,8 +1,9 @@
 // zola_adjoint64.hpp -- reverse-mode (adjoint) pass of the PSF model on the
 // 64x64 pupil grid, one wave per PSF.
 //
-// Given w = dL/dpsf on the ROI, the wave recomputes the forward pass (E kept
-// in VGPRs), then runs every stage backwards:
+// Given w = dL/dpsf on the ROI, the wave recomputes the forward pass (the
+// stage-1 data kept in LDS, E recomputed from them in the backward pass), then
+// runs every stage backwards:
 //   psf = m/S, m = |Bl|          -> dL/dBl = sign(Bl) (w/S - sum(w m)/S^2)
 //   Bl = h (x) I (separable)     -> dL/dI  = h^T (x) dL/dBl  (same kernel, transposed sums)
 //   I = |E|^2                    -> dL/dE  = 2 dL/dI E
@@ -40,30 +41,6 @@ struct AdjArgs {
     double* dsig;          // B: dNLL/dsigma (the SIG kernels only)
 };
 
-// Stage 2 of E keeping E (row a = lane) plus the forward row blur.
-template <int R>
-__device__ __forceinline__ void stage2_E_keep64(float h1reg, const WaveScratch64& S, int lane,
-                                                float (&acc)[R], float (&Er)[2][32],
-                                                float (&Ei)[2][32]) {
-#pragma unroll
-    for (int j = 0; j < R; j++) acc[j] = 0.f;
-    static_for<0, 2>([&](auto hpc) {
-        constexpr int HP = decltype(hpc)::value;
-        float re[32], im[32];
-        stage2_half64<HP>(S, lane, re, im);
-        float img[1][32];
-        static_for<0, 32>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            Er[HP][m] = re[fidx<32>(m)];
-            Ei[HP][m] = im[fidx<32>(m)];
-            img[0][m] = Er[HP][m] * Er[HP][m] + Ei[HP][m] * Ei[HP][m];
-        });
-        const float hr[1] = {h1reg};
-        float (&a1)[1][R] = *reinterpret_cast<float (*)[1][R]>(&acc);
-        row_blur_half<R, HP, 1>(hr, img, a1);
-    });
-}
-
 // SIG (calibration form only): also dNLL/dsigma.  The blur kernel is the
 // separable h1 (x) h1 with h1 = IDFT(G1), G1 a Gaussian of width sigma px, so
 // d(Bl)/dsigma = (hs (x) h1 + h1 (x) hs) I with hs = dh1/dsigma, and
@@ -72,10 +49,14 @@ __device__ __forceinline__ void stage2_E_keep64(float h1reg, const WaveScratch64
 template <int R, int WPB, bool CALIB, bool SIG = false>
 __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A) {
     static_assert(CALIB || !SIG, "the sigma derivative is formed in the calibration form");
+    // S: stage-1 data, kept until the row adjoint (E is recomputed from it there
+    // instead of being held in registers through the backward pass); Tt: the
+    // column-blur transpose, then dL/dBl
     struct Lds {
         BlockTables64 T;
         float hs[64];
         WaveScratch64 S[WPB];
+        float Tt[WPB][R * kTS];
     };
     __shared__ Lds lds;
     BlockTables64& T = lds.T;
@@ -90,20 +71,23 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
     const int b = blockIdx.x * WPB + w;
     if (b >= A.B) return;
     WaveScratch64& S = lds.S[w];
+    float* Tt = lds.Tt[w];
     const float x = A.params[b * 4 + 0], y = A.params[b * 4 + 1];
     const float z = A.params[b * 4 + 2], zoil = A.params[b * 4 + 3];
     const float X = M.kappa * x, Y = M.kappa * y;
     const float h1reg = T.h3[lane];
 
-    // ---- forward, keeping E ----
+    // ---- forward (the stage-1 data stay in S) ----
     stage1_64(T, S, lane, 0, X, Y, z, zoil, M.kappa);
     wave_lds_fence();
-    float Er[2][32], Ei[2][32];
     float bl[RPL];
     {
         float acc[R];
-        stage2_E_keep64<R>(h1reg, S, lane, acc, Er, Ei);
-        column_blur64<R>(T, S, lane, acc, bl);
+        stage2_E64<R>(h1reg, S, lane, acc);
+        store_rows64<R>(Tt, lane, acc);
+        wave_lds_fence();
+        column_pass64<R>(T.h3, Tt, lane, bl);
+        wave_lds_fence();  // Tt is reused for dL/dBl
     }
     const int j = lane % R, g = lane / R;
     float wv[RPL];
@@ -182,7 +166,7 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
     const float inv = 1.f / ssum, c2 = wm * inv * inv;
 
     // ---- dL/dBl at the ROI pixels, stored as Bt[i][j] in LDS ----
-    float* Bt = S.re;             // R*R floats
+    float* Bt = Tt;               // R*R floats
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
         const bool ok = g < G && g * RPL + t < R;
@@ -212,6 +196,13 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
             if constexpr (SIG) us[jj] = accs;
         }
     }
+    // Pin u and us here: they are read only in the row adjoint, and without this
+    // the compiler sinks their sums there, keeping all R x R loads of Bt live.
+#pragma unroll
+    for (int jj = 0; jj < R; jj++) {
+        __asm__ __volatile__("" : "+v"(u[jj]));
+        if constexpr (SIG) __asm__ __volatile__("" : "+v"(us[jj]));
+    }
     const float hsreg = SIG ? lds.hs[lane] : 0.f;
     float gsig = 0.f;
     // row adjoint along b, dL/dE = 2 dL/dI E, then the adjoint of the
@@ -224,25 +215,46 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
     }
     static_for<0, 2>([&](auto hpc) {
         constexpr int HP = decltype(hpc)::value;
+        // E at the columns 2m + HP of row lane, recomputed from the stage-1 data:
+        // column 2m + HP at index fidx(m)
+        float Eh_r[32], Eh_i[32];
+        stage2_half64<HP>(S, lane, Eh_r, Eh_i);
         float er[32], ei[32];
-        static_for<0, 32>([&](auto mc) {
-            constexpr int m = decltype(mc)::value;
-            constexpr int bcol = 2 * m + HP;
-            float ib = 0.f, ibs = 0.f;
-            static_for<0, R>([&](auto jc) {
-                constexpr int jj = decltype(jc)::value;
-                const float hc = __int_as_float(
-                    __builtin_amdgcn_readlane(__float_as_int(h1reg), (jj - R / 2 - bcol) & 63));
-                ib += hc * u[jj];
-                if constexpr (SIG) {
-                    const float hsc = __int_as_float(
-                        __builtin_amdgcn_readlane(__float_as_int(hsreg), (jj - R / 2 - bcol) & 63));
-                    ibs += hsc * u[jj] + hc * us[jj];
-                }
+        // dL/dI(a, b) at the columns b = 2m + HP: ib[m] = sum_jj h1((jj - R/2 - b) & 63) u[jj],
+        // grouped by tap index c = (jj - R/2 - b) & 63 (one broadcast per tap and
+        // group, as the forward row blur does; jj = (c + R/2 + b) & 63 is a
+        // constant), MG columns at a time: the sigma form keeps two accumulators
+        // per column
+        constexpr int MG = SIG ? 8 : 16;
+        static_for<0, 32 / MG>([&](auto gc) {
+            constexpr int m0 = decltype(gc)::value * MG;
+            float ib[MG], ibs[SIG ? MG : 1];
+#pragma unroll
+            for (int t = 0; t < MG; t++) {
+                ib[t] = 0.f;
+                if constexpr (SIG) ibs[t] = 0.f;
+            }
+            static_for<0, 64>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                const float hc = bcast(h1reg, c);
+                const float hsc = SIG ? bcast(hsreg, c) : 0.f;
+                static_for<0, MG>([&](auto tc) {
+                    constexpr int t = decltype(tc)::value;
+                    constexpr int jj = (c + R / 2 + 2 * (m0 + t) + HP) & 63;
+                    if constexpr (jj < R) {
+                        ib[t] += hc * u[jj];
+                        if constexpr (SIG) ibs[t] += hsc * u[jj] + hc * us[jj];
+                    }
+                });
+                __builtin_amdgcn_sched_barrier(0);
             });
-            if constexpr (SIG) gsig += ibs * (Er[HP][m] * Er[HP][m] + Ei[HP][m] * Ei[HP][m]);
-            er[m] = 2.f * ib * Er[HP][m];
-            ei[m] = 2.f * ib * Ei[HP][m];
+            static_for<0, MG>([&](auto tc) {
+                constexpr int t = decltype(tc)::value, m = m0 + t;
+                const float Erm = Eh_r[fidx<32>(m)], Eim = Eh_i[fidx<32>(m)];
+                if constexpr (SIG) gsig += ibs[t] * (Erm * Erm + Eim * Eim);
+                er[m] = 2.f * ib[t] * Erm;
+                ei[m] = 2.f * ib[t] * Eim;
+            });
         });
         fft_dif<32, -1>(er, ei);
         // output n (mod 32) sits at fidx(n); add exp(-2 pi i n HP / 64) * Y[n]

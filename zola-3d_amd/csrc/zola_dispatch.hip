@@ -37,8 +37,16 @@ __global__ __launch_bounds__(1024) void k_project_zernike(int B, int J, const do
     constexpr int kG = 8;
     __shared__ double red[kG][1024];
     const int p = threadIdx.x;
-    double q = 0.0;
-    for (int b = 0; b < B; b++) q += (double)qbar[(size_t)b * 1024 + p];
+    // eight partial sums keep eight loads in flight (a single running sum waits
+    // out each load's latency in turn); combined in a fixed order
+    double qk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int b = 0;
+    for (; b + 8 <= B; b += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) qk[k] += (double)qbar[(size_t)(b + k) * 1024 + p];
+    }
+    for (int k = 0; b + k < B; k++) qk[k] += (double)qbar[(size_t)(b + k) * 1024 + p];
+    const double q = ((qk[0] + qk[1]) + (qk[2] + qk[3])) + ((qk[4] + qk[5]) + (qk[6] + qk[7]));
     for (int j0 = 0; j0 < J; j0 += kG) {
 #pragma unroll
         for (int g = 0; g < kG; g++) red[g][p] = j0 + g < J ? Zw[(size_t)(j0 + g) * 1024 + p] * q : 0.0;

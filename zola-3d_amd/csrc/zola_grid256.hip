@@ -401,9 +401,15 @@ __global__ __launch_bounds__(256) void k_dparams256(Geo g, const double* qbar, f
 __global__ __launch_bounds__(256) void k_qsum256(int B, const double* qbar, double* Q) {
     const int w = blockIdx.x * 256 + threadIdx.x;
     if (w >= kWin) return;
-    double s = 0.0;
-    for (int b = 0; b < B; b++) s += qbar[(size_t)b * kWin + w];
-    Q[w] = s;
+    // eight partial sums keep eight loads in flight; combined in a fixed order
+    double qk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int b = 0;
+    for (; b + 8 <= B; b += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) qk[k] += qbar[(size_t)(b + k) * kWin + w];
+    }
+    for (int k = 0; b + k < B; k++) qk[k] += qbar[(size_t)(b + k) * kWin + w];
+    Q[w] = ((qk[0] + qk[1]) + (qk[2] + qk[3])) + ((qk[4] + qk[5]) + (qk[6] + qk[7]));
 }
 
 // 7c: da_j += sum_w Z_j(w) Q(w), one block per coefficient.
