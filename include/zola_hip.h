@@ -107,8 +107,9 @@ int  zola_psf_backward_dev(zola_ctx* ctx, int B, const float* params, const floa
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving or calibration loops, harnesses that forbid
  * hipMalloc).  Same arguments and results as zola_psf_backward_dev; the per-PSF
- * scratch (dL/dphase on the pupil window, 1024 floats per PSF) lives in a fixed
- * device array of ZOLA_NOALLOC_MAX_PSF x 1024 floats (8 MiB) that every GPU the
+ * scratch (dL/dphase on the pupil window, 1024 floats per PSF, and its 1024
+ * FP64 sums over the PSFs) lives in a fixed device array of
+ * ZOLA_NOALLOC_MAX_PSF x 1024 floats + 8 KiB (8 MiB + 8 KiB) that every GPU the
  * library runs on holds until the process ends, instead of the context's
  * grow-only buffer.  B above ZOLA_NOALLOC_MAX_PSF returns ZOLA_EINVAL.  All calls
  * on one GPU share that array: calls that overlap on one GPU (two streams or two

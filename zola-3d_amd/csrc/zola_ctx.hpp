@@ -101,6 +101,12 @@ int zola_fit_layout(const zola_ctx* c, int R, int mode);
 
 constexpr int kGrid256 = 256;  // pupil grid of the calibration-only contexts
 
+// The 64-grid adjoint writes B x 1024 floats of dL/dphase; the 1024 FP64 sums
+// over the PSFs that the Zernike projection forms follow them in the same
+// buffer (B x 4096 bytes from its start, so 8-byte aligned).
+constexpr size_t kQBytes = 1024 * sizeof(double);
+inline double* qsum_slot(float* qbar, int B) { return (double*)(qbar + (size_t)B * 1024); }
+
 inline int set_err(zola_ctx* c, int code, const char* fmt, ...);
 
 // Entry points of the localization path (fits, initial values, detection, the

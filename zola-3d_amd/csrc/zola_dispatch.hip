@@ -3,6 +3,7 @@
 
 #include "zola_kernels.h"
 #include "zola_adjoint64.hpp"
+#include "zola_project.hpp"
 #include "zola_psf128.hpp"
 
 namespace zola {
@@ -26,42 +27,6 @@ hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, 
                          double zmin_t, double zstep_t, double* init, hipStream_t st);
 template <int R>
 hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t st);
-
-// da_j += sum_p Z_j(p) Q(p), Q(p) = sum_b qbar[b][p] (FP64), by one block of
-// 1024 threads: thread p forms Q(p) once (the PSF sum is shared by every
-// coefficient), then the products of eight coefficients at a time are summed
-// over the window by LDS trees in a fixed order.
-__global__ __launch_bounds__(1024) void k_project_zernike(int B, int J, const double* __restrict__ Zw,
-                                                          const float* __restrict__ qbar,
-                                                          double* __restrict__ da) {
-    constexpr int kG = 8;
-    __shared__ double red[kG][1024];
-    const int p = threadIdx.x;
-    // eight partial sums keep eight loads in flight (a single running sum waits
-    // out each load's latency in turn); combined in a fixed order
-    double qk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int b = 0;
-    for (; b + 8 <= B; b += 8) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) qk[k] += (double)qbar[(size_t)(b + k) * 1024 + p];
-    }
-    for (int k = 0; b + k < B; k++) qk[k] += (double)qbar[(size_t)(b + k) * 1024 + p];
-    const double q = ((qk[0] + qk[1]) + (qk[2] + qk[3])) + ((qk[4] + qk[5]) + (qk[6] + qk[7]));
-    for (int j0 = 0; j0 < J; j0 += kG) {
-#pragma unroll
-        for (int g = 0; g < kG; g++) red[g][p] = j0 + g < J ? Zw[(size_t)(j0 + g) * 1024 + p] * q : 0.0;
-        __syncthreads();
-        for (int o = 512; o > 0; o >>= 1) {
-            if (p < o) {
-#pragma unroll
-                for (int g = 0; g < kG; g++) red[g][p] += red[g][p + o];
-            }
-            __syncthreads();
-        }
-        if (p < kG && j0 + p < J) da[j0 + p] += red[p][0];
-        __syncthreads();
-    }
-}
 
 }  // namespace zola
 
@@ -125,11 +90,9 @@ hipError_t zola_launch_adjoint64(const Model64& M, int R, const AdjArgs& A, hipS
 #undef CALL
 }
 
-hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* da,
+hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* Q, double* da,
                                        hipStream_t st) {
-    if (J <= 0 || B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_project_zernike, dim3(1), dim3(1024), 0, st, B, J, Zw, qbar, da);
-    return hipGetLastError();
+    return launch_window_projection<float, 256>(B, J, kW * kW, Zw, qbar, Q, da, st);
 }
 
 // 128 grid: the plugin's default patch (32, blur grid 64), smaller even patches

@@ -35,6 +35,7 @@
 #include "zola_ctx.hpp"
 #include "zola_fft_f64.hpp"
 #include "zola_grid256.h"
+#include "zola_project.hpp"
 
 namespace zola {
 
@@ -397,31 +398,6 @@ __global__ __launch_bounds__(256) void k_dparams256(Geo g, const double* qbar, f
     if (threadIdx.x < 4) dparams[(size_t)b * 4 + threadIdx.x] = (float)a[threadIdx.x];
 }
 
-// 7b: Q(w) = sum_b qbar_b(w), one thread per window point.
-__global__ __launch_bounds__(256) void k_qsum256(int B, const double* qbar, double* Q) {
-    const int w = blockIdx.x * 256 + threadIdx.x;
-    if (w >= kWin) return;
-    // eight partial sums keep eight loads in flight; combined in a fixed order
-    double qk[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int b = 0;
-    for (; b + 8 <= B; b += 8) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) qk[k] += qbar[(size_t)(b + k) * kWin + w];
-    }
-    for (int k = 0; b + k < B; k++) qk[k] += qbar[(size_t)(b + k) * kWin + w];
-    Q[w] = ((qk[0] + qk[1]) + (qk[2] + qk[3])) + ((qk[4] + qk[5]) + (qk[6] + qk[7]));
-}
-
-// 7c: da_j += sum_w Z_j(w) Q(w), one block per coefficient.
-__global__ __launch_bounds__(256) void k_project256(int J, const double* Zw, const double* Q, double* da) {
-    __shared__ double red[256];
-    const int jz = blockIdx.x;
-    double a[1] = {0.0};
-    for (int w = threadIdx.x; w < kWin; w += 256) a[0] += Zw[(size_t)jz * kWin + w] * Q[w];
-    block_sum256<1>(a, red);
-    if (threadIdx.x == 0) da[jz] += a[0];
-}
-
 template <int L, bool COLMAJOR, class IO>
 hipError_t lines(const IO& io, int nlines, double sgn, const double2* tw, hipStream_t st) {
     hipLaunchKernelGGL((k_fft_lines<L, COLMAJOR, IO>), dim3(fft_lines_grid<L>(nlines)), dim3(256), 0, st, io, nlines,
@@ -587,10 +563,7 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
         HIP_TRY(c, hipGetLastError());
     }
     if (e.da && c->J > 0) {
-        hipLaunchKernelGGL(k_qsum256, dim3(kWin / 256), dim3(256), 0, st, B, qbar, Qs);
-        HIP_TRY(c, hipGetLastError());
-        hipLaunchKernelGGL(k_project256, dim3(c->J), dim3(256), 0, st, c->J, c->d_Zw, Qs, e.da);
-        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, (launch_window_projection<double, 1024>(B, c->J, kWin, c->d_Zw, qbar, Qs, e.da, st)));
     }
     return ZOLA_OK;
 }

@@ -514,7 +514,7 @@ int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* 
     }
     float* qbar = nullptr;
     if (da && c->J > 0) {
-        const size_t need = (size_t)B * 1024 * sizeof(float);
+        const size_t need = (size_t)B * 1024 * sizeof(float) + kQBytes;
         if (need > c->qbar_bytes) {
             if (c->d_qbar) HIP_TRY(c, hipFree(c->d_qbar));
             c->d_qbar = nullptr;
@@ -531,7 +531,7 @@ int zola_psf_backward_dev(zola_ctx* c, int B, const float* params, const float* 
     A.dparams = dparams;
     A.qbar = qbar;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
-    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream));
     return ZOLA_OK;
 }
 
@@ -600,7 +600,7 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
     }
     float* qbar = nullptr;
     if (da && c->J > 0) {
-        const size_t need = (size_t)B * 1024 * sizeof(float);
+        const size_t need = (size_t)B * 1024 * sizeof(float) + kQBytes;
         if (need > c->qbar_bytes) {
             if (c->d_qbar) HIP_TRY(c, hipFree(c->d_qbar));
             c->d_qbar = nullptr;
@@ -623,7 +623,7 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
     A.dbg = dbg;
     A.S = 1;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
-    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream));
     return ZOLA_OK;
 }
 
@@ -658,7 +658,7 @@ int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, co
     }
     float* qbar = nullptr;
     if (da && c->J > 0) {
-        const size_t need = (size_t)B * 1024 * sizeof(float);
+        const size_t need = (size_t)B * 1024 * sizeof(float) + kQBytes;
         if (need > c->qbar_bytes) {
             if (c->d_qbar) HIP_TRY(c, hipFree(c->d_qbar));
             c->d_qbar = nullptr;
@@ -683,7 +683,7 @@ int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, co
     A.dpoly = dpoly;
     A.dsig = dsigma;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
-    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream));
     return ZOLA_OK;
 }
 

@@ -29,7 +29,9 @@ hipError_t zola_launch_template_stats64(int R, int nZ, const float* tmpl, double
 hipError_t zola_launch_adjoint64(const zola::Model64& M, int R, const zola::AdjArgs& A,
                                  hipStream_t st);
 
-hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar,
+// da[0..J) += sum_p Z_j(p) Q(p) over the 32x32 window, Q(p) = sum_b qbar[b][p]
+// formed into Q (1024 doubles of device memory).
+hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* Q,
                                        double* da, hipStream_t st);
 
 // 128x128 pupil grid

@@ -5,16 +5,18 @@
 // basis.  Growing that buffer calls hipMalloc, which callers under HIP graph
 // capture, in tight serving or calibration loops, or in harnesses that forbid
 // allocation cannot do.  zola_psf_backward_noalloc_dev uses instead the fixed
-// device array `scratch` below: ZOLA_NOALLOC_MAX_PSF x 1024 floats (8 MiB),
-// held by every GPU the library runs on until the process ends.
+// device array `scratch` below: ZOLA_NOALLOC_MAX_PSF x 1024 floats (8 MiB) for
+// dL/dphase plus 1024 doubles (8 KiB) for its sums over the PSFs, held by every
+// GPU the library runs on until the process ends.
 //
 // In host code `scratch` names a host-side stand-in, not device memory, so its
 // device address is looked up once per device (hipGetSymbolAddress, with that
 // device current) when a context is created there, and only that address is
 // passed to kernels.  Every call on one GPU shares the array; the adjoint
-// kernel overwrites all 1024 entries of each PSF it handles before the
-// projection reads them, so nothing needs clearing between calls, but calls
-// that overlap on one GPU (two streams, two contexts) race.
+// kernel overwrites all 1024 entries of each PSF it handles, and the sum kernel
+// all 1024 sums, before the projection reads them, so nothing needs clearing
+// between calls, but calls that overlap on one GPU (two streams, two contexts)
+// race.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -27,7 +29,7 @@
 namespace {
 
 constexpr int kMaxDevices = 64;
-constexpr size_t kScratchFloats = (size_t)ZOLA_NOALLOC_MAX_PSF * 1024;
+constexpr size_t kScratchFloats = (size_t)ZOLA_NOALLOC_MAX_PSF * 1024 + kQBytes / sizeof(float);
 
 static __device__ __align__(256) float scratch[kScratchFloats];
 
@@ -75,6 +77,6 @@ extern "C" int zola_psf_backward_noalloc_dev(zola_ctx* c, int B, const float* pa
     A.dparams = dparams;
     A.qbar = qbar;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
-    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, da, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream));
     return ZOLA_OK;
 }
