@@ -136,9 +136,9 @@ int  zola_psf_backward_noalloc_dev(zola_ctx* ctx, int B, const float* params, co
  * process/gpu/Modelmany_double_.java:271-387):
  * ZOLA_LAYOUT_WAVE one wave per ROI; ZOLA_LAYOUT_QUAD four waves (one block)
  * per ROI, ceil(R/4) ROI rows per lane; ZOLA_LAYOUT_PAIR two waves (one block)
- * per ROI, LM form at R <= 21 only; ZOLA_LAYOUT_AUTO (the default) four waves
- * above R = 21, below that two waves for the LM form and one for the
- * reference form. */
+ * per ROI, LM form at R <= 21 only; ZOLA_LAYOUT_AUTO (the default) two waves
+ * for the LM form at R <= 21, one wave otherwise (the fastest layouts measured
+ * on MI355X). */
 #define ZOLA_LAYOUT_AUTO 0
 #define ZOLA_LAYOUT_WAVE 1
 #define ZOLA_LAYOUT_QUAD 2

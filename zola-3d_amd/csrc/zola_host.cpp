@@ -194,10 +194,11 @@ int zola_fit_layout(const zola_ctx* c, int R, int mode) {
     if (l == ZOLA_LAYOUT_WAVE) return 0;
     if (l == ZOLA_LAYOUT_QUAD) return 1;
     if (l == ZOLA_LAYOUT_PAIR) return 2;
-    // one wave holds ceil(R/3) or more ROI rows per lane above 21; the pair
-    // layout covers the LM form up to 21
-    if (R >= 31) return 1;
-    return mode == ZOLA_FIT_LM ? 2 : 0;
+    // the fastest layouts on 32,768 (LM) and 8,192 (reference form) ROIs:
+    // two waves for the LM form up to 21, one wave otherwise (four waves
+    // took 1.4x (LM) and 2.4x (reference form) the time at 31 and 1.04x and
+    // 1.7x at 41)
+    return (mode == ZOLA_FIT_LM && R <= 21) ? 2 : 0;
 }
 
 // Whether a layout (ZOLA_LAYOUT_WAVE/QUAD/PAIR) exists for (R, mode).

@@ -22,7 +22,8 @@
 //    there.  After a block barrier (the stage-1 buffers are free) it forms the
 //    row-blur partial sums over those columns at the R crop columns, one
 //    channel at a time (h1 (x) I, hd (x) I, h1 (x) Iz), each stored transposed
-//    in LDS over the stage-1 buffers; the column pass adds the two partials.
+//    in LDS over the stage-1 buffers; after a barrier the two waves add the
+//    two partials once (each wave half of the entries).
 //  * Column pass: lane = (ROI column j, row group g); wave wr owns ROI rows
 //    [wr RH, wr RH + RH) (RH = ceil(R/2)), RPL rows per lane.
 #pragma once
@@ -123,22 +124,42 @@ __device__ __forceinline__ void stage2_pair64(const WaveScratch64& SE, const Wav
     });
 }
 
+// acc[jj] += sum_c k[c] img[m] over the columns 2m + HP, jj = (c + R/2 + 2m +
+// HP) mod 64 < R.  The 64 taps are read from LDS four at a time at one
+// address for the whole wave (a broadcast: no VALU work to spread them).
+template <int R, int HP>
+__device__ __forceinline__ void row_blur_half_lds(const float* __restrict__ k, const float (&img)[32],
+                                                  float (&acc)[R]) {
+    static_for<0, 16>([&](auto c4c) {
+        constexpr int c4 = decltype(c4c)::value;
+        const float4 t4 = reinterpret_cast<const float4*>(k)[c4];
+        const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
+        static_for<0, 4>([&](auto uc) {
+            constexpr int c = 4 * c4 + decltype(uc)::value;
+            static_for<0, 32>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                constexpr int jj = (c + R / 2 + 2 * m + HP) & 63;
+                if constexpr (jj < R) acc[jj] += tv[decltype(uc)::value] * img[m];
+            });
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
 // Row-blur partial sums over the columns 2m + HP, one channel at a time (only
 // R accumulators live), stored transposed at Tw[(c R + j) kIp + lane]:
 // c = 0: h1 (x) I, 1: hd (x) I, 2: h1 (x) Iz, at the R crop columns j.
 template <int R, int HP, int kIp>
-__device__ __forceinline__ void row_blur_pair64(float h1reg, float hdreg, const float (&im)[32],
-                                                const float (&iz)[32], float* Tw, int lane) {
+__device__ __forceinline__ void row_blur_pair64(const BlurTaps64& T, const float (&im)[32], const float (&iz)[32],
+                                                float* Tw, int lane) {
     static_for<0, 3>([&](auto cc) {
         constexpr int c = decltype(cc)::value;
-        float acc[1][R];
+        float acc[R];
 #pragma unroll
-        for (int j = 0; j < R; j++) acc[0][j] = 0.f;
-        const float kr[1] = {c == 1 ? hdreg : h1reg};
-        const float (&img)[1][32] = *reinterpret_cast<const float (*)[1][32]>(c == 2 ? &iz : &im);
-        row_blur_half<R, HP, 1>(kr, img, acc);
+        for (int j = 0; j < R; j++) acc[j] = 0.f;
+        row_blur_half_lds<R, HP>(c == 1 ? T.hd3 : T.h3, c == 2 ? iz : im, acc);
 #pragma unroll
-        for (int j = 0; j < R; j++) Tw[(c * R + j) * kIp + lane] = acc[0][j];
+        for (int j = 0; j < R; j++) Tw[(c * R + j) * kIp + lane] = acc[j];
     });
 }
 
@@ -166,13 +187,22 @@ __device__ __forceinline__ void eval64p(const Model64& M, const BlurTaps64& T, f
         if (wr == 0) stage2_pair64<0>(SE, SZ, lane, im, iz);
         else stage2_pair64<1>(SE, SZ, lane, im, iz);
         __syncthreads();  // both waves are done reading the stage-1 buffers
-        const float h1reg = T.h3[lane], hdreg = T.hd3[lane];
         float* Tw = scr + wr * 3 * R * kIp;
-        if (wr == 0) row_blur_pair64<R, 0, kIp>(h1reg, hdreg, im, iz, Tw, lane);
-        else row_blur_pair64<R, 1, kIp>(h1reg, hdreg, im, iz, Tw, lane);
+        if (wr == 0) row_blur_pair64<R, 0, kIp>(T, im, iz, Tw, lane);
+        else row_blur_pair64<R, 1, kIp>(T, im, iz, Tw, lane);
     }
     __syncthreads();
-    // column pass over the sum of the two partials; lanes past G groups read
+    // the two waves' partial sums added once (each wave half of them), into
+    // wave 0's array, which the column pass then reads alone
+    {
+        float* P0 = scr + wr * kIp + lane;
+        const float* P1 = P0 + 3 * R * kIp;
+#pragma unroll
+        for (int e = 0; e < 3 * R; e += 2)
+            if (e + wr < 3 * R) P0[e * kIp] += P1[e * kIp];
+    }
+    __syncthreads();
+    // column pass over the summed partials; lanes past G groups read
     // column 0 and their outputs are never used.  Output row i = r0 + t takes
     // h1((i - R/2 - a) mod 64) = h1((a - t - c0) mod 64) (h1 even) and
     // hd((i - R/2 - a) mod 64) = -hd((a - t - c0) mod 64) (hd odd) with
@@ -184,7 +214,6 @@ __device__ __forceinline__ void eval64p(const Model64& M, const BlurTaps64& T, f
     const float* K1 = T.h3 + kb;
     const float* KD = T.hd3 + kb;
     const float* A0 = scr + jr * kIp;
-    const float* A1 = A0 + 3 * R * kIp;
 #pragma unroll
     for (int t = 0; t < RPL; t++) vB[t] = vX[t] = vY[t] = vZ[t] = 0.f;
     // sliding windows w1[u] = K1[a + u], wd[u] = KD[a + u] (u < RPL): one new
@@ -207,9 +236,9 @@ __device__ __forceinline__ void eval64p(const Model64& M, const BlurTaps64& T, f
             }
             w1[RPL - 1] = K1[a + RPL - 1];
             wd[RPL - 1] = KD[a + RPL - 1];
-            const float t0 = A0[a] + A1[a];
-            const float t1 = A0[R * kIp + a] + A1[R * kIp + a];
-            const float tz = A0[2 * R * kIp + a] + A1[2 * R * kIp + a];
+            const float t0 = A0[a];
+            const float t1 = A0[R * kIp + a];
+            const float tz = A0[2 * R * kIp + a];
 #pragma unroll
             for (int t = 0; t < RPL; t++) {
                 const float k1 = w1[RPL - 1 - t], kd = wd[RPL - 1 - t];
