@@ -194,6 +194,7 @@ int zola_center_fisher_dev(zola_ctx* c, int nz, const double* zs, double zfocus,
     const int P = 7 * std::min(nz, kMaxChunk);
     // dense fftshifted pupil, Gaussian of the blur grid (s2 = N here), twiddles
     std::vector<PupilCell> cell(n, PupilCell{0, 0, 0, 0});
+    ensure_phase(c);
     for (int p = 0; p < c->D; p++) {
         const int a = (c->di[p] + N / 2) % N, b = (c->dj[p] + N / 2) % N;
         cell[(size_t)a * N + b] = PupilCell{c->amp[p], c->phase[p], c->kz[p], c->kzo[p]};

@@ -18,7 +18,11 @@ struct zola_ctx {
     int N = 0, R = 0, s2 = 0, D = 0, J = 0;
     double center = 0, ringsize = 0, radius_px = 0, kappa = 0, kz0 = 0, kzo0 = 0;
     std::vector<int> di, dj, wu, wv;
-    std::vector<double> kz, kzo, amp, Z, coef, phase;
+    std::vector<double> kz, kzo, amp, Z, coef;
+    // phase on the disk points (FP64); after a coefficient update that went
+    // straight to the device tables it is recomputed on first host use
+    mutable std::vector<double> phase;
+    mutable bool phase_stale = false;
     zola::Model64 model{};
     float* d_tab = nullptr;  // 4 x 1024 floats: phi, amp, kz', kzo'
     zola::Model128 model128{};
@@ -100,6 +104,19 @@ int zola_center_fisher_dev(zola_ctx* c, int nz, const double* zs, double zfocus,
 int zola_fit_layout(const zola_ctx* c, int R, int mode);
 
 constexpr int kGrid256 = 256;  // pupil grid of the calibration-only contexts
+
+// phase(p) = sum_j Z_j(p) a_j on the disk points, summed in j order
+// (ZernikePhase.java:200-224); the device update of the tables forms the same
+// products and sums in the same order.
+inline void compute_phase(const zola_ctx* c) {
+    c->phase.assign(c->D, 0.0);
+    for (int j = 0; j < c->J; j++)
+        for (int p = 0; p < c->D; p++) c->phase[p] += c->Z[(size_t)j * c->D + p] * c->coef[j];
+    c->phase_stale = false;
+}
+inline void ensure_phase(const zola_ctx* c) {
+    if (c->phase_stale) compute_phase(c);
+}
 
 // The 64-grid adjoint writes B x 1024 floats of dL/dphase; the 1024 FP64 sums
 // over the PSFs that the Zernike projection forms follow them in the same

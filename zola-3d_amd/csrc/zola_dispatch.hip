@@ -28,9 +28,40 @@ hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, 
 template <int R>
 hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t st);
 
+// phase(w) = sum_j Z_j(w) a_j, products and sums in j order, not fused:
+// the host's FP64 phase bit for bit, rounded to float as the host upload does
+// (zero outside the disk, where the basis is zero).
+__global__ __launch_bounds__(256) void k_phase_tab64(int J, CoefVals cv, const double* __restrict__ Zw,
+                                                     float* __restrict__ tab) {
+#pragma clang fp contract(off)
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= 1024) return;
+    double ph = 0.0;
+    // eight basis loads in flight, then the products and sums in j order
+    for (int j0 = 0; j0 < J; j0 += 8) {
+        double zv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) zv[k] = j0 + k < J ? Zw[(size_t)(j0 + k) * 1024 + w] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (j0 + k < J) ph = ph + zv[k] * cv.a[j0 + k];
+    }
+    const float f = (float)ph;
+    tab[w] = f;
+    tab[4096 + 4 * w] = f;
+}
+
 }  // namespace zola
 
 using namespace zola;
+
+hipError_t zola_launch_phase_tab64(int J, const double* a, const double* Zw, float* tab, hipStream_t st) {
+    if (J <= 0 || J > kMaxCoefDev) return hipErrorInvalidValue;
+    CoefVals cv{};
+    for (int j = 0; j < J; j++) cv.a[j] = a[j];
+    hipLaunchKernelGGL(k_phase_tab64, dim3(4), dim3(256), 0, st, J, cv, Zw, tab);
+    return hipGetLastError();
+}
 
 #define ZOLA_R_DISPATCH(R, CALL) \
     switch (R) {                 \

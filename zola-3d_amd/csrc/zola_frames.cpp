@@ -78,6 +78,7 @@ void zola_psf_f64_host_r(const zola_ctx* c, int R, double x, double y, double zo
     const int N = c->N, s2 = ref_pow2(std::min(2 * R, N));
     const double cc = c->center, xs = c->o.xystep;
     std::vector<cd> E((size_t)N * N, cd(0, 0));
+    ensure_phase(c);
     for (int p = 0; p < c->D; p++) {
         const int i = c->di[p], ii = c->dj[p];
         const double kx = (i - cc) * (2 * PI) / (xs * N), ky = (ii - cc) * (2 * PI) / (xs * N);

@@ -37,5 +37,10 @@ struct Eval256 {
 int zola256_setup(zola_ctx* c);           // device tables of a new N = 256 context
 int zola256_upload_tables(zola_ctx* c);   // pupil window {phase, amp, kz, kzo} after a coefficient change
 int zola256_upload_gauss(zola_ctx* c);    // Gaussian spectrum and its sigma derivative after a sigma change
+// The same two updates formed on the device (stream-ordered, no host pass and
+// no synchronisation): the phase from the coefficients and the basis already
+// there, the Gaussian from sigma.
+int zola256_set_phase(zola_ctx* c);
+int zola256_set_gauss(zola_ctx* c);
 void zola256_free(zola_ctx* c);
 int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st);

@@ -35,6 +35,7 @@
 #include "zola_ctx.hpp"
 #include "zola_fft_f64.hpp"
 #include "zola_grid256.h"
+#include "zola_kernels.h"
 #include "zola_project.hpp"
 
 namespace zola {
@@ -447,6 +448,7 @@ int zola256_setup(zola_ctx* c) {
 }
 
 int zola256_upload_tables(zola_ctx* c) {
+    ensure_phase(c);
     std::vector<double4> tab(kWin, make_double4(0.0, 0.0, 0.0, 0.0));
     for (int p = 0; p < c->D; p++)
         tab[c->wu[p] * kW4 + c->wv[p]] = make_double4(c->phase[p], c->amp[p], c->kz[p], c->kzo[p]);
@@ -474,6 +476,67 @@ int zola256_upload_gauss(zola_ctx* c) {
         }
     HIP_TRY(c, hipMemcpyAsync(c->d_gk256, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ZOLA_OK;
+}
+
+namespace zola {
+namespace {
+
+// tab[w].x = sum_j Z_j(w) a_j in j order, products and sums not fused (the
+// host's phase bit for bit; zero outside the disk).
+__global__ __launch_bounds__(256) void k_phase256(int J, CoefVals cv, const double* __restrict__ Zw,
+                                                  double4* __restrict__ tab) {
+#pragma clang fp contract(off)
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= kWin) return;
+    double ph = 0.0;
+    for (int j0 = 0; j0 < J; j0 += 8) {
+        double zv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) zv[k] = j0 + k < J ? Zw[(size_t)(j0 + k) * kWin + w] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (j0 + k < J) ph = ph + zv[k] * cv.a[j0 + k];
+    }
+    tab[w].x = ph;
+}
+
+// zola256_upload_gauss's tables on the device: the same expressions, with the
+// device exp (within an ulp of the host's; the spectrum is rounded to float as
+// PSFPhaseMany.java:387-404 does).
+__global__ __launch_bounds__(256) void k_gauss256(int s2, double sigma, double* __restrict__ g) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= s2 * s2) return;
+    const int k1 = k / s2, k2 = k % s2;
+    const double sigF = (1.0 / sigma) * s2 / (3.14159265358979323846 * 2.0);
+    const double sigpow = sigF * sigF;
+    const double cc = (double)(s2 / 2);
+    const double j = (double)((k1 + s2 / 2) % s2), jj = (double)((k2 + s2 / 2) % s2);
+    const double r2 = (j - cc) * (j - cc) + (jj - cc) * (jj - cc);   // integers: exact
+    const double e = exp(-.5 * r2 / sigpow);
+    g[k] = (double)(float)e;
+    g[(size_t)s2 * s2 + k] = -e * r2 / (sigpow * sigma);
+}
+
+}  // namespace
+}  // namespace zola
+
+int zola256_set_phase(zola_ctx* c) {
+    if (c->J <= 0 || c->J > kMaxCoefDev || !c->d_Zw) return zola256_upload_tables(c);
+    CoefVals cv{};
+    for (int j = 0; j < c->J; j++) cv.a[j] = c->coef[j];
+    hipLaunchKernelGGL(k_phase256, dim3(kWin / 256), dim3(256), 0, c->stream, c->J, cv, (const double*)c->d_Zw,
+                       c->d_tab256);
+    HIP_TRY(c, hipGetLastError());
+    return ZOLA_OK;
+}
+
+int zola256_set_gauss(zola_ctx* c) {
+    const int s2 = c->s2;
+    hipLaunchKernelGGL(k_gauss256, dim3((s2 * s2 + 255) / 256), dim3(256), 0, c->stream, s2, c->o.sigma,
+                       c->d_gk256);
+    HIP_TRY(c, hipGetLastError());
     return ZOLA_OK;
 }
 

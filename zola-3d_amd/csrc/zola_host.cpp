@@ -50,12 +50,6 @@ double zernike_value(int n, int m, double theta, double rho) {
 
 }  // namespace
 
-static void compute_phase(zola_ctx* c) {
-    c->phase.assign(c->D, 0.0);
-    for (int j = 0; j < c->J; j++)
-        for (int p = 0; p < c->D; p++) c->phase[p] += c->Z[(size_t)j * c->D + p] * c->coef[j];
-}
-
 static int compute_h1(zola_ctx* c) {
     // the 256 grid blurs by FFT with the 2-D Gaussian spectrum (zola256_upload_gauss)
     if (c->N == kGrid256) {
@@ -157,6 +151,7 @@ static int upload_taps128(zola_ctx* c) {
 
 static int upload_tables(zola_ctx* c) {
     c->model_ver++;
+    ensure_phase(c);
     if (c->N == kGrid256) return zola256_upload_tables(c);
     if (c->N == zola::kN2) {
         std::vector<float4> tab(zola::kW2 * zola::kW2, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -444,6 +439,16 @@ int zola_set_coefficients(zola_ctx* c, const double* coeffs, int J) {
     CtxLock lk_(c);
     if (!c || !coeffs || J != c->J) return set_err(c, ZOLA_EINVAL, "coefficient count must be %d", c ? c->J : 0);
     for (int j = 0; j < J; j++) c->coef[j] = coeffs[j];
+    // calibration loops change the coefficients every evaluation: the 64 and
+    // 256 grids form the phase tables on the device from the Zernike basis
+    // already there (no host pass over the disk, no copy, no synchronisation)
+    if (J > 0 && J <= kMaxCoefDev && c->d_Zw && (c->N == zola::kN || c->N == kGrid256)) {
+        c->model_ver++;
+        c->phase_stale = true;
+        if (c->N == kGrid256) return zola256_set_phase(c);
+        HIP_TRY(c, zola_launch_phase_tab64(J, c->coef.data(), c->d_Zw, c->d_tab, c->stream));
+        return ZOLA_OK;
+    }
     compute_phase(c);
     return upload_tables(c);
 }
@@ -454,7 +459,7 @@ int zola_set_sigma(zola_ctx* c, double sigma) {
     c->o.sigma = sigma;
     int rc = compute_h1(c);
     if (rc == ZOLA_OK && c->N == zola::kN2) rc = upload_taps128(c);
-    if (rc == ZOLA_OK && c->N == kGrid256) rc = zola256_upload_gauss(c);
+    if (rc == ZOLA_OK && c->N == kGrid256) rc = zola256_set_gauss(c);
     return rc;
 }
 

@@ -29,6 +29,17 @@ hipError_t zola_launch_template_stats64(int R, int nZ, const float* tmpl, double
 hipError_t zola_launch_adjoint64(const zola::Model64& M, int R, const zola::AdjArgs& A,
                                  hipStream_t st);
 
+// Zernike coefficients by value for the device update of the phase tables.
+constexpr int kMaxCoefDev = 128;
+struct CoefVals {
+    double a[kMaxCoefDev];
+};
+
+// Phase tables of the 64 grid from the coefficients a[0..J) and the basis Zw
+// (J x 1024 FP64 on the 32x32 window): tab[w] and the interleaved copy
+// tab[4096 + 4 w] (the table layout of Model64).
+hipError_t zola_launch_phase_tab64(int J, const double* a, const double* Zw, float* tab, hipStream_t st);
+
 // da[0..J) += sum_p Z_j(p) Q(p) over the 32x32 window, Q(p) = sum_b qbar[b][p]
 // formed into Q (1024 doubles of device memory).
 hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* Q,
