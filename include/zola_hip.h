@@ -138,11 +138,14 @@ int  zola_psf_backward_noalloc_dev(zola_ctx* ctx, int B, const float* params, co
  * per ROI, ceil(R/4) ROI rows per lane; ZOLA_LAYOUT_PAIR two waves (one block)
  * per ROI, LM form at R <= 21 only; ZOLA_LAYOUT_AUTO (the default) two waves
  * for the LM form at R <= 21, one wave otherwise (the fastest layouts measured
- * on MI355X). */
+ * on MI355X).  The 128-grid fits run four waves per ROI (ZOLA_LAYOUT_QUAD);
+ * their LM form also has ZOLA_LAYOUT_OCT, eight waves (one block of 512
+ * threads) per ROI, its default (ZOLA_LAYOUT_AUTO). */
 #define ZOLA_LAYOUT_AUTO 0
 #define ZOLA_LAYOUT_WAVE 1
 #define ZOLA_LAYOUT_QUAD 2
 #define ZOLA_LAYOUT_PAIR 3
+#define ZOLA_LAYOUT_OCT 4
 
 typedef struct zola_fit_config {
     int    mode;             /* ZOLA_FIT_*                                    */
@@ -174,10 +177,11 @@ int  zola_fit_batch_zoil_dev(zola_ctx* ctx, const zola_fit_config* cfg, int B,
                              const float* pixels, const double* init, const float* zoil,
                              double* theta, double* nll, double* crlb, int* iters, int* status);
 
-/* Sets the work layout (ZOLA_LAYOUT_*) of the 64-grid fits of ROI side R in
- * fit mode `mode` for this context.  Results of the layouts agree to FP32
- * rounding of the blur sums (a different summation order).  A layout that does
- * not exist for (R, mode) returns ZOLA_EUNSUPPORTED. */
+/* Sets the work layout (ZOLA_LAYOUT_*) of the fits of ROI side R in fit mode
+ * `mode` for this context (64 grid: every side; 128 grid: the context's side).
+ * Results of the layouts agree to FP32 rounding of the blur sums (a different
+ * summation order).  A layout that does not exist for (R, mode) returns
+ * ZOLA_EUNSUPPORTED. */
 int  zola_set_fit_layout(zola_ctx* ctx, int R, int mode, int layout);
 
 /* Autotuner of the fit layout: fits a synthetic batch of B ROIs of side R

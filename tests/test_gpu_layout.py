@@ -103,3 +103,19 @@ def test_pair_layout_is_lm_only_and_small_sides_only():
     with pytest.raises(Z.ZolaError):
         ctx.set_fit_layout(21, Z.FIT_REFERENCE, Z.LAYOUT_PAIR)
     ctx.close()
+
+
+def test_eight_wave_layout_is_the_128_grid_lm_form_only():
+    ctx = Z.Context(Z.optics(), 21, Z.astigmatic_coeffs(28))
+    with pytest.raises(Z.ZolaError):
+        ctx.set_fit_layout(21, Z.FIT_LM, Z.LAYOUT_OCT)
+    ctx.close()
+    opt = dict(size_fft=128, xystep=0.11, wavelength=0.68, noil=1.518, nwat=1.518, na=1.45, sigma=1.0)
+    ctx = Z.Context(Z.optics(**opt), 41, Z.tetrapod_coeffs(45))
+    ctx.set_fit_layout(41, Z.FIT_LM, Z.LAYOUT_OCT)
+    ctx.set_fit_layout(41, Z.FIT_LM, Z.LAYOUT_QUAD)
+    with pytest.raises(Z.ZolaError):
+        ctx.set_fit_layout(41, Z.FIT_REFERENCE, Z.LAYOUT_OCT)
+    with pytest.raises(Z.ZolaError):
+        ctx.set_fit_layout(41, Z.FIT_LM, Z.LAYOUT_PAIR)
+    ctx.close()

@@ -71,9 +71,11 @@ def test_psf128_matches_fp64(setup):
     assert err <= 1.5e-6
 
 
-def test_lm128_matches_fp64_optimum(setup):
+@pytest.mark.parametrize("layout", [Z.LAYOUT_QUAD, Z.LAYOUT_OCT])
+def test_lm128_matches_fp64_optimum(setup, layout):
     ctx, m, coef = setup
-    print("case: N=128 LM vs FP64 optimum, B=16", flush=True)
+    print(f"case: N=128 LM vs FP64 optimum, B=16, layout {layout}", flush=True)
+    ctx.set_fit_layout(R, Z.FIT_LM, layout)
     B = 16
     rois, truth = _rois(m, B, seed=128)
     init = truth.copy()
@@ -82,7 +84,10 @@ def test_lm128_matches_fp64_optimum(setup):
     init[:, 2:4] += rng.uniform(-0.05, 0.05, (B, 2))
     init[:, 4] += rng.uniform(-0.15, 0.15, B)
     cfg = ctx.fit_config(mode=Z.FIT_LM, max_iter=30, zmin=ZR[0] - 0.3, zmax=ZR[1] + 0.3, tol=1e-4)
-    res = ctx.fit(cfg, rois, init)
+    try:
+        res = ctx.fit(cfg, rois, init)
+    finally:
+        ctx.set_fit_layout(R, Z.FIT_LM, Z.LAYOUT_AUTO)
     refs = _pool(lambda mm, b: mm.lm_fit(rois[b].astype(np.float64), init[b], zmin=ZR[0] - 0.3,
                                          zmax=ZR[1] + 0.3, maxit=200, tol=1e-10), list(range(B)), coef)
     th_ref = np.array([r["theta"] for r in refs])

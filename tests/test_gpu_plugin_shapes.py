@@ -76,10 +76,12 @@ def test_plugin_shape_psf(torch_cuda, R):
     assert err < 1.5e-6
 
 
+@pytest.mark.parametrize("layout", [Z.LAYOUT_QUAD, Z.LAYOUT_OCT])
 @pytest.mark.parametrize("R", SIZES)
-def test_plugin_shape_lm_fit(torch_cuda, R):
-    print(f"case: N = 128, R = {R}, LM fit", flush=True)
+def test_plugin_shape_lm_fit(torch_cuda, R, layout):
+    print(f"case: N = 128, R = {R}, LM fit, layout {layout}", flush=True)
     a, ctx, m = _models(R)
+    ctx.set_fit_layout(R, Z.FIT_LM, layout)
     B = 16
     rois, truth = _rois(m, B, seed=500 + R)
     init = truth.copy()

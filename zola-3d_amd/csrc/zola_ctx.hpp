@@ -61,6 +61,8 @@ struct zola_ctx {
     // work layout of the 64-grid fits per ROI side (17, 21, 31, 41) and mode
     // (reference form, LM): ZOLA_LAYOUT_*
     int layout[4][2] = {};
+    // LM-form layout of the 128-grid fits (ZOLA_LAYOUT_AUTO, QUAD or OCT)
+    int layout128 = 0;
     // Entry points on one context run one at a time (CtxLock): host threads may
     // share a handle, their calls are serialised.
     mutable std::recursive_mutex mu;
@@ -102,6 +104,8 @@ int zola_center_fisher_dev(zola_ctx* c, int nz, const double* zs, double zfocus,
 // Kernel layout of a 64-grid fit: 0 one wave per ROI, 1 four waves per ROI,
 // 2 two waves per ROI (LM form, R <= 21).
 int zola_fit_layout(const zola_ctx* c, int R, int mode);
+// Kernel layout of a 128-grid fit: 1 four waves per ROI, 3 eight waves (LM form).
+int zola_fit_layout128(const zola_ctx* c, int mode);
 
 constexpr int kGrid256 = 256;  // pupil grid of the calibration-only contexts
 

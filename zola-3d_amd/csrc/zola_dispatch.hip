@@ -23,6 +23,8 @@ hipError_t launch_psf_forward128_r(const Model128& M, int B, const float* params
 template <int R>
 hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStream_t st);
 template <int R>
+hipError_t launch_fit128o_r(const Model128& M, const FitArgs& A, hipStream_t st);
+template <int R>
 hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, const double* tstat,
                          double zmin_t, double zstep_t, double* init, hipStream_t st);
 template <int R>
@@ -146,7 +148,13 @@ hipError_t zola_launch_psf_forward128(const Model128& M, int R, int B, const flo
 #undef CALL
 }
 
-hipError_t zola_launch_fit128(const Model128& M, int R, int mode, const FitArgs& A, hipStream_t st) {
+hipError_t zola_launch_fit128(const Model128& M, int R, int mode, int layout, const FitArgs& A, hipStream_t st) {
+    if (layout == 3) {
+        if (mode != 1) return hipErrorInvalidValue;
+#define CALL(RR) launch_fit128o_r<RR>(M, A, st)
+        ZOLA_R128_DISPATCH(R, CALL)
+#undef CALL
+    }
 #define CALL(RR) launch_fit128_r<RR>(M, mode, A, st)
     ZOLA_R128_DISPATCH(R, CALL)
 #undef CALL

@@ -23,6 +23,7 @@
 #include "zola_psf128.hpp"
 #include "zola_psf64q.hpp"
 #include "zola_psf64p.hpp"
+#include "zola_psf128o.hpp"
 
 namespace zola {
 
@@ -102,6 +103,10 @@ __device__ __forceinline__ double scmos_term(const float* vroi, int lane, int wr
 // added in two parts (ln 2 split as in fdlibm's e_log.c) so that e ln2_hi is
 // exact.  About a quarter of the instructions of the library's log, which
 // carries extra precision the likelihood does not need.
+// 1/21, 1/19, ..., 1/3: the atanh series of log_pos.
+__constant__ const double kLogSeries[10] = {1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0,
+                                            1.0 / 11.0, 1.0 / 9.0,  1.0 / 7.0,  1.0 / 5.0,  1.0 / 3.0};
+
 __device__ __forceinline__ double log_pos(double x) {
     int e;
     double m = frexp(x, &e);
@@ -112,16 +117,15 @@ __device__ __forceinline__ double log_pos(double x) {
     const double f = m - 1.0;
     const double s = f / (f + 2.0);
     const double s2 = s * s;
-    double t = 1.0 / 21.0;
-    t = t * s2 + 1.0 / 19.0;
-    t = t * s2 + 1.0 / 17.0;
-    t = t * s2 + 1.0 / 15.0;
-    t = t * s2 + 1.0 / 13.0;
-    t = t * s2 + 1.0 / 11.0;
-    t = t * s2 + 1.0 / 9.0;
-    t = t * s2 + 1.0 / 7.0;
-    t = t * s2 + 1.0 / 5.0;
-    t = t * s2 + 1.0 / 3.0;
+    // The series coefficients are read through a pointer the compiler cannot
+    // see through: scalar loads at each use.  As literals they are built in
+    // VGPR pairs once, hoisted out of the fit loops and held (or spilled)
+    // through every evaluation.
+    const double* C = kLogSeries;
+    __asm__ __volatile__("" : "+s"(C));
+    double t = C[0];
+#pragma unroll
+    for (int k = 1; k < 10; k++) t = t * s2 + C[k];
     const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
     const double de = (double)e, ss = s + s;
     return de * ln2_hi + (ss + (ss * s2 * t + de * ln2_lo));
@@ -165,6 +169,29 @@ __device__ __forceinline__ bool chol5(const double (&A)[25], double (&Lc)[25]) {
 #pragma unroll
             for (int k = 0; k < j; k++) t -= Lc[i * 5 + k] * Lc[j * 5 + k];
             Lc[i * 5 + j] = t * inv;
+        }
+    }
+    return ok;
+}
+
+// chol5 in place: the lower triangle of A is replaced by its Cholesky factor
+// (the same operations in the same order as chol5; half the registers).
+__device__ __forceinline__ bool chol5_ip(double (&A)[25]) {
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        double s = A[j * 5 + j];
+#pragma unroll
+        for (int k = 0; k < j; k++) s -= A[j * 5 + k] * A[j * 5 + k];
+        ok = ok && (s > 0.0);
+        A[j * 5 + j] = sqrt(fmax(s, 1e-300));
+        const double inv = 1.0 / A[j * 5 + j];
+#pragma unroll
+        for (int i = j + 1; i < 5; i++) {
+            double t = A[i * 5 + j];
+#pragma unroll
+            for (int k = 0; k < j; k++) t -= A[i * 5 + k] * A[j * 5 + k];
+            A[i * 5 + j] = t * inv;
         }
     }
     return ok;
@@ -349,6 +376,96 @@ struct Synth128 : QuadWaves {
                                    float z, float zoil, float (&vB)[Roi128<R>::RQ], float (&vX)[Roi128<R>::RQ],
                                    float (&vY)[Roi128<R>::RQ], float (&vZ)[Roi128<R>::RQ]) {
         eval128<R, true>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
+    }
+};
+
+// Eight waves per ROI (zola_psf128o.hpp): lane = ROI column j, wave wr owns
+// rows wr*R8 + t (R8 = ceil(R/8)).  Reductions are summed over the eight waves
+// through LDS in wave order, so every wave sees identical values and takes
+// identical branches.
+// Lane l's x at every lane (both halves of an FP64 value through readlane).
+__device__ __forceinline__ double bcast_d(double x, int l) {
+    const long long u = __builtin_bit_cast(long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
+}
+
+struct OctWaves {
+    static constexpr int kWaves = 8;
+    template <int R>
+    static constexpr int RPL = Roi128o<R>::R8;
+    template <int R>
+    __device__ __forceinline__ static bool valid(int lane, int wr, int t) { return lane < R && wr * RPL<R> + t < R; }
+    template <int R>
+    __device__ __forceinline__ static int pix(int lane, int wr, int t) { return (wr * RPL<R> + t) * R + lane; }
+    struct Red {
+        double d[8][24];
+        float f[8][4];
+    };
+    // The eight waves' partials of value k are added in wave order by lane k
+    // (eight LDS reads per lane, not 8 K), then broadcast: every wave forms the
+    // same totals bit for bit.
+    template <int K>
+    __device__ __forceinline__ static void sumv(double (&v)[K], Red& r, int lane, int wr) {
+        static_assert(K <= 24, "reduction width");
+        wave_sum_rows<K>(v);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (lane == 0) r.d[wr][k] = v[k];
+        __syncthreads();
+        double tot = 0.0;
+        if (lane < K) {
+            tot = r.d[0][lane];
+#pragma unroll
+            for (int q = 1; q < 8; q++) tot += r.d[q][lane];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = bcast_d(tot, k);
+        __syncthreads();
+    }
+    template <int K>
+    __device__ __forceinline__ static void sumv(float (&v)[K], Red& r, int lane, int wr) {
+        static_assert(K <= 4, "reduction width");
+        wave_sum_rows<K>(v);
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (lane == 0) r.f[wr][k] = v[k];
+        __syncthreads();
+        float tot = 0.f;
+        if (lane < K) {
+            tot = r.f[0][lane];
+#pragma unroll
+            for (int q = 1; q < 8; q++) tot += r.f[q][lane];
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tot), k));
+        __syncthreads();
+    }
+    __device__ __forceinline__ static double sum(double v, Red& r, int lane, int wr) {
+        double a[1] = {v};
+        sumv<1>(a, r, lane, wr);
+        return a[0];
+    }
+    __device__ __forceinline__ static float sum(float v, Red& r, int lane, int wr) {
+        float a[1] = {v};
+        sumv<1>(a, r, lane, wr);
+        return a[0];
+    }
+};
+
+// The LM form on the 128 grid by eight waves per ROI.
+struct Synth128o : OctWaves {
+    using Model = Model128;
+    using Shared = NoShared;
+    static constexpr int kLmFloats = 2 * (int)(sizeof(Buf128) / sizeof(float));
+    __device__ __forceinline__ static void load(Shared&, const Model&) {}
+    template <int R>
+    __device__ __forceinline__ static void lm_eval(const Model& M, const Shared&, float* scr, int lane, int wr, float x,
+                                                   float y, float z, float zoil, float (&vB)[Roi128o<R>::R8],
+                                                   float (&vX)[Roi128o<R>::R8], float (&vY)[Roi128o<R>::R8],
+                                                   float (&vZ)[Roi128o<R>::R8]) {
+        eval128o<R>(M, *reinterpret_cast<Buf128 (*)[2]>(scr), lane, wr, x, y, z, zoil, vB, vX, vY, vZ);
     }
 };
 
@@ -660,15 +777,15 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
             // decision from LDS; they keep no Fisher matrix, secant term or
             // gradient of their own (only the first wave solves).
             if (!first && wr == 0) {
-                double Am[25], Lc[25], d[5];
+                double Lc[25], d[5];
     #pragma unroll
                 for (int p = 0; p < 5; p++) {
     #pragma unroll
-                    for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q) + hget(L.S, p, q);
-                    Am[p * 5 + p] += L.lambda * hget(L.H, p, p);
+                    for (int q = 0; q <= p; q++) Lc[p * 5 + q] = hget(L.H, p, q) + hget(L.S, p, q);
+                    Lc[p * 5 + p] += L.lambda * hget(L.H, p, p);
                     d[p] = -L.gr[p];
                 }
-                if (!chol5(Am, Lc)) {
+                if (!chol5_ip(Lc)) {
                     L.lambda *= L.nu;
                     L.nu *= 2;
                     eval_now = false;
@@ -694,11 +811,11 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
     #pragma unroll
                                 for (int p = 0; p < 5; p++) {
     #pragma unroll
-                                    for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q) + hget(L.S, p, q);
-                                    Am[p * 5 + p] += lc * hget(L.H, p, p);
+                                    for (int q = 0; q <= p; q++) Lc[p * 5 + q] = hget(L.H, p, q) + hget(L.S, p, q);
+                                    Lc[p * 5 + p] += lc * hget(L.H, p, p);
                                     dc[p] = -L.gr[p];
                                 }
-                                if (chol5(Am, Lc)) {
+                                if (chol5_ip(Lc)) {
                                     chol5_solve(Lc, dc);
                                     have = true;
                                 } else {
@@ -870,12 +987,12 @@ __device__ __forceinline__ void fit_lm_body(const typename SY::Model& M, const F
         // CRLB of x, y, z from the Fisher matrix at the optimum (first wave)
         double cr[3] = {0.0, 0.0, 0.0};
         if (wr == 0) {
-            double Am[25], Lc[25];
+            double Lc[25];
     #pragma unroll
             for (int p = 0; p < 5; p++)
     #pragma unroll
-                for (int q = 0; q < 5; q++) Am[p * 5 + q] = hget(L.H, p, q);
-            const bool pd = chol5(Am, Lc);
+                for (int q = 0; q <= p; q++) Lc[p * 5 + q] = hget(L.H, p, q);
+            const bool pd = chol5_ip(Lc);
     #pragma unroll
             for (int c = 2; c < 5; c++) {
                 double e[5];
@@ -909,6 +1026,14 @@ template <int R, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fit_lm128(Model128 M, FitArgs A) {
     __shared__ LmLds<Synth128, WPB> lds;
     fit_lm_body<Synth128, R, WPB>(M, A, lds);
+}
+
+// One ROI per block and CU (the LDS holds one ROI's stage-1 buffers), two
+// waves per SIMD: at most 256 registers per lane.
+template <int R>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fit_lm128o(Model128 M, FitArgs A) {
+    __shared__ LmLds<Synth128o, 8> lds;
+    fit_lm_body<Synth128o, R, 8>(M, A, lds);
 }
 
 // Two waves per SIMD (four blocks per CU): at most 256 registers per lane.

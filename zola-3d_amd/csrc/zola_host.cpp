@@ -196,9 +196,19 @@ int zola_fit_layout(const zola_ctx* c, int R, int mode) {
     return (mode == ZOLA_FIT_LM && R <= 21) ? 2 : 0;
 }
 
+// Kernel layout of a 128-grid fit (zola_launch_fit128): 1 four waves per ROI,
+// 3 eight waves per ROI (LM form).
+int zola_fit_layout128(const zola_ctx* c, int mode) {
+    if (mode != ZOLA_FIT_LM) return 1;
+    if (c->layout128 == ZOLA_LAYOUT_QUAD) return 1;
+    // default: eight waves (65,536 41^2 ROIs: 66.3 ms against 77.1 ms for four)
+    return 3;
+}
+
 // Whether a layout (ZOLA_LAYOUT_WAVE/QUAD/PAIR) exists for (R, mode).
 static bool layout_available(int R, int mode, int layout) {
     if (layout == ZOLA_LAYOUT_PAIR) return mode == ZOLA_FIT_LM && R <= 21;
+    if (layout == ZOLA_LAYOUT_OCT) return false;
     return layout == ZOLA_LAYOUT_WAVE || layout == ZOLA_LAYOUT_QUAD;
 }
 
@@ -238,7 +248,8 @@ int zola_fit_launch(zola_ctx* c, const zola_fit_config* cfg, int B, const float*
     A.crlb = crlb;
     A.iters = iters;
     A.status = status;
-    if (c->N == zola::kN2) HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, A, st));
+    if (c->N == zola::kN2)
+        HIP_TRY(c, zola_launch_fit128(c->model128, c->R, cfg->mode, zola_fit_layout128(c, cfg->mode), A, st));
     else HIP_TRY(c, zola_launch_fit64(c->model, c->R, cfg->mode, zola_fit_layout(c, c->R, cfg->mode), A, st));
     return ZOLA_OK;
 }
@@ -735,11 +746,21 @@ int zola_fit_batch_ext_dev(zola_ctx* c, const zola_fit_config* cfg, int B, const
 int zola_set_fit_layout(zola_ctx* c, int R, int mode, int layout) {
     CtxLock lk_(c);
     if (!c) return ZOLA_EINVAL;
-    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "fit layouts apply to the 64 grid");
+    if (mode != ZOLA_FIT_REFERENCE && mode != ZOLA_FIT_LM) return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", mode);
+    if (layout < ZOLA_LAYOUT_AUTO || layout > ZOLA_LAYOUT_OCT) return set_err(c, ZOLA_EINVAL, "unknown layout %d", layout);
+    if (c->N == zola::kN2) {
+        // 128 grid: four waves per ROI for both forms, eight for the LM form
+        if (R != c->R) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available", R);
+        const bool ok = layout == ZOLA_LAYOUT_AUTO || layout == ZOLA_LAYOUT_QUAD ||
+                        (layout == ZOLA_LAYOUT_OCT && mode == ZOLA_FIT_LM);
+        if (!ok) return set_err(c, ZOLA_EUNSUPPORTED, "layout %d not available on the 128 grid, mode %d", layout, mode);
+        if (mode == ZOLA_FIT_LM) c->layout128 = layout;
+        return ZOLA_OK;
+    }
+    if (c->N != zola::kN) return set_err(c, ZOLA_EUNSUPPORTED, "fit layouts apply to the 64 and 128 grids");
     const int i = roi_slot(R);
     if (i < 0) return set_err(c, ZOLA_EUNSUPPORTED, "ROI side %d not available", R);
-    if (mode != ZOLA_FIT_REFERENCE && mode != ZOLA_FIT_LM) return set_err(c, ZOLA_EINVAL, "unknown fit mode %d", mode);
-    if (layout < ZOLA_LAYOUT_AUTO || layout > ZOLA_LAYOUT_PAIR) return set_err(c, ZOLA_EINVAL, "unknown layout %d", layout);
+    if (layout == ZOLA_LAYOUT_OCT) return set_err(c, ZOLA_EUNSUPPORTED, "eight waves per ROI serve the 128 grid");
     if (layout != ZOLA_LAYOUT_AUTO && !layout_available(R, mode, layout))
         return set_err(c, ZOLA_EUNSUPPORTED, "layout %d not available for side %d, mode %d", layout, R, mode);
     c->layout[i][mode == ZOLA_FIT_LM ? 1 : 0] = layout;
@@ -904,7 +925,7 @@ int zola_fit_mixed_dev(zola_ctx* c, const zola_fit_config* cfg, int G, const int
             A.crlb = crlb[g];
             A.iters = iters[g];
             A.status = status[g];
-            err = c->N == zola::kN2 ? zola_launch_fit128(c->model128, R[g], cfg->mode, A, c->side[g])
+            err = c->N == zola::kN2 ? zola_launch_fit128(c->model128, R[g], cfg->mode, zola_fit_layout128(c, cfg->mode), A, c->side[g])
                                     : zola_launch_fit64(c->model, R[g], cfg->mode, zola_fit_layout(c, R[g], cfg->mode),
                                                         A, c->side[g]);
             if (err != hipSuccess) break;

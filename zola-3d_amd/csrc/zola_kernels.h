@@ -49,7 +49,8 @@ hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const flo
 bool zola_kernels_support_roi128(int R);
 hipError_t zola_launch_psf_forward128(const zola::Model128& M, int R, int B, const float* params,
                                       float* out, hipStream_t st);
-hipError_t zola_launch_fit128(const zola::Model128& M, int R, int mode, const zola::FitArgs& A,
+// layout: 1 four waves per ROI (both forms), 3 eight waves per ROI (LM form only).
+hipError_t zola_launch_fit128(const zola::Model128& M, int R, int mode, int layout, const zola::FitArgs& A,
                               hipStream_t st);
 
 // Zeroes n ints (a kernel, so that it can sit in a captured graph).

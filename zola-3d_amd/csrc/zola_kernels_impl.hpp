@@ -330,6 +330,16 @@ hipError_t launch_fit128_r(const Model128& M, int mode, const FitArgs& A, hipStr
     return hipGetLastError();
 }
 
+// LM form, eight waves (one block) per ROI.
+template <int R>
+hipError_t launch_fit128o_r(const Model128& M, const FitArgs& A, hipStream_t st) {
+    hipError_t e;
+    const int g = fit_grid(k_fit_lm128o<R>, 512, 1, A, st, e);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fit_lm128o<R>), dim3(g), dim3(512), 0, st, M, A);
+    return hipGetLastError();
+}
+
 template <int R>
 hipError_t launch_tstats_r(int nZ, const float* tmpl, double* tstat, hipStream_t st) {
     constexpr int W = 4;

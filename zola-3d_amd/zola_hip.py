@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libzola_hip.so")
 
 FIT_REFERENCE = 0
 FIT_LM = 1
-LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_QUAD, LAYOUT_PAIR = 0, 1, 2, 3
+LAYOUT_AUTO, LAYOUT_WAVE, LAYOUT_QUAD, LAYOUT_PAIR, LAYOUT_OCT = 0, 1, 2, 3, 4
 ST_CONVERGED, ST_MAXITER, ST_NAN, ST_LOWPHOTON = 1, 2, 4, 8
 
 SYMBOLS = [
