@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--no-beads", action="store_true")
     p.add_argument("--no-host", action="store_true")
     p.add_argument("--n128-batch", type=int, default=1_000_000)
+    p.add_argument("--n128-layout", choices=["auto", "quad", "oct"], default="auto",
+                   help="LM kernel layout of the 128-grid leg: four or eight waves per ROI")
     p.add_argument("--mixed-total", type=int, default=200000)
     p.add_argument("--frames", type=int, default=2000)
     p.add_argument("--cpu-sample", type=int, default=0, help="ROIs for the CPU baseline (0: auto)")
@@ -321,7 +323,7 @@ def _events(torch, n=2):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
 
-def depth128_bench(torch, Z, stream, B):
+def depth128_bench(torch, Z, stream, B, layout="auto"):
     """Large-range depth-dependent run on the 128x128 pupil: 45-mode tetrapod PSF
     (Noll 5, 12, 23 at 2-3 rad, others N(0, 0.1)), n_sample 1.33 / n_imm 1.518,
     focal depth U[0, 4] um per ROI, z U[-3, 3] um, x, y U(+-1 px), photons
@@ -330,6 +332,8 @@ def depth128_bench(torch, Z, stream, B):
     Rn = 41
     ctx = Z.Context(Z.optics(size_fft=128, nwat=1.33), Rn, Z.tetrapod_coeffs(45))
     ctx.set_stream(stream.cuda_stream)
+    lay = {"auto": Z.LAYOUT_AUTO, "quad": Z.LAYOUT_QUAD, "oct": Z.LAYOUT_OCT}[layout]
+    ctx.set_fit_layout(Rn, Z.FIT_LM, lay)
     g = torch.Generator(device="cuda")
     g.manual_seed(0x128)
     u = lambda: torch.rand(B, device="cuda", generator=g, dtype=torch.float64)
@@ -370,7 +374,15 @@ def depth128_bench(torch, Z, stream, B):
         "converged_frac": round(float(np.mean((sn & Z.ST_CONVERGED) != 0)), 5),
         "median_abs_err_vs_truth_nm": [round(float(v), 3) for v in np.median(err, axis=0)],
         "median_crlb_nm": [round(float(v), 3) for v in np.median(cr.cpu().numpy() * 1000, axis=0)],
+        "lm_layout": {"auto": "default (eight waves per ROI)", "quad": "four waves per ROI",
+                      "oct": "eight waves per ROI"}[layout],
     }
+    # algorithmic FLOPs: 5 n log2 n per 128x128 pupil field (n = 128^2), as for the headline's 64^2 fields
+    f128 = 5.0 * 128 * 128 * np.log2(128 * 128)
+    ach = float((sn >> 8).astype(np.float64).sum()) * f128 / (ms / 1e3) / 1e12
+    out["roofline"] = {"bound": "valu", "kernel": "k_fit_lm128o<41>" if layout != "quad" else "k_fit_lm128<41,4>",
+                       "achieved": round(ach, 4), "peak": 157.3, "unit": "TFLOP/s", "frac": round(ach / 157.3, 5),
+                       "algorithmic_flops_per_field": f128}
     Bp = max(1, min(B // 8, 8192))
     cfgp = ctx.fit_config(mode=Z.FIT_REFERENCE, max_iter=20, zmin=-3.3, zmax=3.3, photon_threshold=0.0)
     args = (px[:Bp].contiguous(), init[:Bp].contiguous(), zoil[:Bp].contiguous(), th[:Bp], nl[:Bp], cr[:Bp],
@@ -771,7 +783,7 @@ def main():
         out["calibration"] = calibration_bench(torch, Z, stream, cpu=not args.no_cpu)
         out["calibration_n256"] = calibration256_bench(torch, Z, stream)
     if not args.no_n128:
-        out["depth_n128"] = depth128_bench(torch, Z, stream, args.n128_batch)
+        out["depth_n128"] = depth128_bench(torch, Z, stream, args.n128_batch, args.n128_layout)
     if not args.no_mixed:
         out["mixed_sizes"] = mixed_bench(torch, Z, stream, args.mixed_total)
     if not args.no_frames:
