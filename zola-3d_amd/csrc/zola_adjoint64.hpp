@@ -1,9 +1,9 @@
 // zola_adjoint64.hpp -- reverse-mode (adjoint) pass of the PSF model on the
-// 64x64 pupil grid, one wave per PSF.
+// 64x64 pupil grid, four waves (one block) per PSF.
 //
-// Given w = dL/dpsf on the ROI, the wave recomputes the forward pass (the
-// stage-1 data kept in LDS, E recomputed from them in the backward pass), then
-// runs every stage backwards:
+// Given w = dL/dpsf on the ROI (or, in the calibration form, the likelihood of
+// ZernikePhaseRetrieval.Model.computeLikelihood, 1859-1867, formed from the
+// data), the block runs the forward pass, then every stage backwards:
 //   psf = m/S, m = |Bl|          -> dL/dBl = sign(Bl) (w/S - sum(w m)/S^2)
 //   Bl = h (x) I (separable)     -> dL/dI  = h^T (x) dL/dBl  (same kernel, transposed sums)
 //   I = |E|^2                    -> dL/dE  = 2 dL/dI E
@@ -11,9 +11,27 @@
 //   P = A exp(i q)               -> dL/dq  = A (-sin q dL/dPr + cos q dL/dPi)
 // and reduces dL/dq against dq/dx, dq/dy, dq/dz, dq/dzoil.  dL/dq itself is
 // written per PSF so that dL/da_j = sum_p Z_j(p) dL/dq(p) can be formed for
-// the whole batch by one projection kernel (Zernike basis in FP64).
+// the whole batch by the projection kernels (zola_project.hpp, FP64).
+//
+// A calibration evaluation has a few hundred PSFs, so one wave per PSF would
+// leave most of the chip idle and take one wave's latency; the work of a PSF
+// is split over four waves:
+//  * forward: stage 1 on wave 0; stage 2, wave wr the output columns
+//    b = 4k + wr of every grid row (lane), kept in registers for the backward
+//    pass; row blur, wave wr a quarter of the crop columns; column blur and
+//    likelihood, wave wr a quarter of the ROI rows (lane = ROI column);
+//  * column adjoint, wave wr a quarter of the crop columns for every grid row;
+//    row adjoint, wave wr the columns 4k + wr again, so dL/dE = 2 dL/dI E uses
+//    the E held since the forward pass;
+//  * adjoint of stage 2: each wave's 16 columns give a 16-point DFT and a
+//    twiddle per window frequency; the four partial sums are added in LDS in
+//    wave order;
+//  * adjoint of stage 1: every wave runs the 32-point transforms of the window
+//    columns, then handles 8 of the 32 window frequencies (the sincos and
+//    gradient work); the per-wave gradient sums are added in wave order.
 #pragma once
 #include "zola_fit64.hpp"
+#include "zola_psf64q.hpp"
 
 namespace zola {
 
@@ -41,22 +59,92 @@ struct AdjArgs {
     double* dsig;          // B: dNLL/dsigma (the SIG kernels only)
 };
 
+constexpr int kPq = 33;   // row stride of the stage-2 adjoint partials
+
+template <int R>
+struct Adj64q {
+    static constexpr int RQ = Roi64q<R>::RQ;   // ROI rows / crop columns per wave
+    static constexpr int N3 = R - 3 * RQ;      // crop columns of wave 3
+};
+
+// Block sums of K doubles over the four waves, added in wave order.
+template <int K>
+__device__ __forceinline__ void block_sum4(double (&v)[K], double (*red)[16], int lane, int wr) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) red[wr][k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    __syncthreads();
+}
+
+// Row adjoint of the columns b = 4k + Q of grid row a (lane): ib[k] =
+// sum_jj h1((jj - R/2 - b) & 63) u[jj], grouped by tap index c (one broadcast
+// per tap); SIG: ibs[k] = sum_jj hs((jj - R/2 - b) & 63) u[jj] + h1(..) us[jj].
+template <int R, int Q, bool SIG>
+__device__ __forceinline__ void row_adjoint_q(float h1reg, float hsreg, const float (&u)[R],
+                                              const float (&us)[SIG ? R : 1], float (&ib)[16],
+                                              float (&ibs)[SIG ? 16 : 1]) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        ib[k] = 0.f;
+        if constexpr (SIG) ibs[k] = 0.f;
+    }
+    static_for<0, 64>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        const float hc = bcast(h1reg, c);
+        const float hsc = SIG ? bcast(hsreg, c) : 0.f;
+        static_for<0, 16>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int jj = (c + R / 2 + 4 * k + Q) & 63;
+            if constexpr (jj < R) {
+                ib[k] += hc * u[jj];
+                if constexpr (SIG) ibs[k] += hsc * u[jj] + hc * us[jj];
+            }
+        });
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+// Adjoint of stage 2 for the columns 4k + Q of row a: part[jslot] =
+// sum_k dE(a, 4k + Q) exp(-2 pi i n (4k + Q) / 64), n = slot_n(jslot).
+template <int Q>
+__device__ __forceinline__ void stage2_adj_q(float (&dr)[16], float (&di)[16], float* Pr, float* Pi, int a) {
+    fft_dif<16, -1>(dr, di);
+    static_for<0, 32>([&](auto jc) {
+        constexpr int jslot = decltype(jc)::value;
+        constexpr int n = slot_n(jslot);
+        constexpr int k = fidx<16>((n + 16) % 16);
+        float r = dr[k], i = di[k];
+        twiddle_mul<64, ((-n * Q) % 64 + 64) % 64, +1>(r, i);
+        Pr[a * kPq + jslot] = r;
+        Pi[a * kPq + jslot] = i;
+    });
+}
+
 // SIG (calibration form only): also dNLL/dsigma.  The blur kernel is the
 // separable h1 (x) h1 with h1 = IDFT(G1), G1 a Gaussian of width sigma px, so
 // d(Bl)/dsigma = (hs (x) h1 + h1 (x) hs) I with hs = dh1/dsigma, and
 //   dNLL/dsigma = sum_{a,b} I(a,b) sum_j [us(a,j) h1(j - b) + u(a,j) hs(j - b)]
 // with u, us the column adjoints of dL/dBl through h1 and hs.
-template <int R, int WPB, bool CALIB, bool SIG = false>
-__global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A) {
+template <int R, bool CALIB, bool SIG = false>
+__global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
     static_assert(CALIB || !SIG, "the sigma derivative is formed in the calibration form");
-    // S: stage-1 data, kept until the row adjoint (E is recomputed from it there
-    // instead of being held in registers through the backward pass); Tt: the
-    // column-blur transpose, then dL/dBl
+    constexpr int RQ = Adj64q<R>::RQ, N3 = Adj64q<R>::N3, kP = kPq;
     struct Lds {
         BlockTables64 T;
         float hs[64];
-        WaveScratch64 S[WPB];
-        float Tt[WPB][R * kTS];
+        WaveScratch64 S;          // stage 1; then I; then the row-blurred rows; then dL/d(stage-1 output)
+        float Bt[R * R];          // dL/dBl on the ROI
+        float U[64 * R];          // column adjoint u(a, jj)
+        float Us[SIG ? 64 * R : 1];
+        float Pr[4][64 * kP];     // stage-2 adjoint partials per wave
+        float Pi[4][64 * kP];
+        double red[4][16];
     };
     __shared__ Lds lds;
     BlockTables64& T = lds.T;
@@ -64,44 +152,75 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
     if constexpr (SIG) {
         for (int t = threadIdx.x; t < 64; t += blockDim.x) lds.hs[t] = M.hs[t];
     }
-    __syncthreads();
-    constexpr int G = RoiLayout<R>::G, RPL = RoiLayout<R>::RPL;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int b = blockIdx.x * WPB + w;
+    const int b = blockIdx.x;
     if (b >= A.B) return;
-    WaveScratch64& S = lds.S[w];
-    float* Tt = lds.Tt[w];
+    const int wr = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    __syncthreads();
     const float x = A.params[b * 4 + 0], y = A.params[b * 4 + 1];
     const float z = A.params[b * 4 + 2], zoil = A.params[b * 4 + 3];
     const float X = M.kappa * x, Y = M.kappa * y;
     const float h1reg = T.h3[lane];
 
-    // ---- forward (the stage-1 data stay in S) ----
-    stage1_64(T, S, lane, 0, X, Y, z, zoil, M.kappa);
-    wave_lds_fence();
-    float bl[RPL];
-    {
-        float acc[R];
-        stage2_E64<R>(h1reg, S, lane, acc);
-        store_rows64<R>(Tt, lane, acc);
-        wave_lds_fence();
-        column_pass64<R>(T.h3, Tt, lane, bl);
-        wave_lds_fence();  // Tt is reused for dL/dBl
+    // ---- forward: stage 1 (wave 0)
+    if (wr == 0) stage1_64(T, lds.S, lane, 0, X, Y, z, zoil, M.kappa);
+    __syncthreads();
+    // ---- stage 2: E(a, 4k + wr) of row a = lane, kept for the backward pass
+    float Er[16], Ei[16];
+    switch (wr) {
+        case 0: stage2_q64<0>(lds.S, lane, Er, Ei); break;
+        case 1: stage2_q64<1>(lds.S, lane, Er, Ei); break;
+        case 2: stage2_q64<2>(lds.S, lane, Er, Ei); break;
+        default: stage2_q64<3>(lds.S, lane, Er, Ei); break;
     }
-    const int j = lane % R, g = lane / R;
-    float wv[RPL];
-    float ssum = 0.f, wm = 0.f;
+    __syncthreads();  // all stage-1 reads are done
+    float* const Sf = reinterpret_cast<float*>(&lds.S);   // the stage-1 buffer as one float array
+    float* Ib = Sf;  // real [64][kIq]
 #pragma unroll
-    for (int t = 0; t < RPL; t++) {
-        const bool ok = g < G && g * RPL + t < R;
-        if (ok) ssum += fabsf(bl[t]);
+    for (int k = 0; k < 16; k++) {
+        const int f = fidx<16>(k);
+        Ib[lane * kIq + 4 * k + wr] = Er[f] * Er[f] + Ei[f] * Ei[f];
     }
-    ssum = wave_sum(ssum);
+    __syncthreads();
+    // ---- row blur at the crop columns [wr RQ, ...) of row a = lane
+    float ra[1][RQ], rl[1][N3];
+    switch (wr) {
+        case 0: row_blur64q<R, 0 * RQ, RQ, 1>(Ib, Ib, lane, h1reg, 0.f, ra); break;
+        case 1: row_blur64q<R, 1 * RQ, RQ, 1>(Ib, Ib, lane, h1reg, 0.f, ra); break;
+        case 2: row_blur64q<R, 2 * RQ, RQ, 1>(Ib, Ib, lane, h1reg, 0.f, ra); break;
+        default: row_blur64q<R, 3 * RQ, N3, 1>(Ib, Ib, lane, h1reg, 0.f, rl); break;
+    }
+    __syncthreads();  // all intensity reads are done
+    float* Rb = Sf;  // [column][row]
+    if (wr < 3) {
+#pragma unroll
+        for (int jj = 0; jj < RQ; jj++) Rb[(wr * RQ + jj) * kIq + lane] = ra[0][jj];
+    } else {
+#pragma unroll
+        for (int jj = 0; jj < N3; jj++) Rb[(3 * RQ + jj) * kIq + lane] = rl[0][jj];
+    }
+    __syncthreads();
+    // ---- column blur: lane j, ROI rows [wr RQ, wr RQ + RQ)
+    const int j = lane < R ? lane : 0;
+    float bl[RQ], d0[RQ], d1[RQ], d2[RQ];
+    switch (wr) {
+        case 0: column_blur64q<R, 0 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
+        case 1: column_blur64q<R, 1 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
+        case 2: column_blur64q<R, 2 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
+        default: column_blur64q<R, 3 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
+    }
+    auto ok = [&](int t) { return lane < R && wr * RQ + t < R; };
+    float wv[RQ];
+    double sacc[1] = {0.0};
+#pragma unroll
+    for (int t = 0; t < RQ; t++)
+        if (ok(t)) sacc[0] += (double)fabsf(bl[t]);
+    // the PSF sum in FP32 as the one-wave form forms it, here added per wave
+    // and over the waves in FP64 then rounded (the normalisation differs from
+    // the one-wave kernel in the last bit only)
+    block_sum4<1>(sacc, lds.red, lane, wr);
+    const float ssum = (float)sacc[0];
     if constexpr (CALIB) {
-        // ZernikePhaseRetrieval.Model.computeLikelihood (1859-1867): mu = psf A +
-        // background (+ sCMOS map), replaced by 1e-8 where it is <= 0 (there the
-        // likelihood term is constant, so its gradient is 0); w = dNLL/dpsf = A (1 - n/mu)
         const double av = A.amp[b];
         const int bead = (A.poly || A.offset) ? b / A.S : 0;
         double pc[6] = {A.poly ? 0.0 : A.bg[b], 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -110,13 +229,13 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
             for (int c = 0; c < 6; c++) pc[c] = A.poly[bead * 6 + c];
         }
         const double jd = (double)j;
-        double nl = 0.0, da = 0.0, dm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // [0] nll, [1] dNLL/damp, [2..7] sum r x monomial
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int t = 0; t < RPL; t++) {
-            const bool ok = g < G && g * RPL + t < R;
+        for (int t = 0; t < RQ; t++) {
             wv[t] = 0.f;
-            if (ok) {
-                const int i = g * RPL + t;
+            if (ok(t)) {
+                const int i = wr * RQ + t;
                 const double id = (double)i;
                 const double ps = (double)(fabsf(bl[t]) / ssum);
                 const double n = A.pixels[(size_t)b * R * R + i * R + j];
@@ -129,54 +248,47 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
                 double mu = ps * av + bgp;
                 const bool clamped = mu <= 0;
                 if (clamped) mu = 1e-8;
-                nl += mu - n * log(mu);
+                acc[0] += mu - n * log(mu);
                 const double r = clamped ? 0.0 : 1.0 - n / mu;
-                da += ps * r;
+                acc[1] += ps * r;
 #pragma unroll
-                for (int c = 0; c < 6; c++) dm[c] += r * mono[c];
+                for (int c = 0; c < 6; c++) acc[2 + c] += r * mono[c];
                 wv[t] = (float)(av * r);
             }
         }
-        nl = wave_sum(nl);
-        da = wave_sum(da);
-#pragma unroll
-        for (int c = 0; c < 6; c++) dm[c] = wave_sum(dm[c]);
-        if (lane == 0) {
-            A.nll[b] = nl;
-            A.damp[b] = da;
-            if (A.dbg) A.dbg[b] = dm[0];
+        block_sum4<8>(acc, lds.red, lane, wr);
+        if (threadIdx.x == 0) {
+            A.nll[b] = acc[0];
+            A.damp[b] = acc[1];
+            if (A.dbg) A.dbg[b] = acc[2];
             if (A.dpoly) {
 #pragma unroll
-                for (int c = 0; c < 6; c++) A.dpoly[(size_t)b * 6 + c] = dm[c];
+                for (int c = 0; c < 6; c++) A.dpoly[(size_t)b * 6 + c] = acc[2 + c];
             }
         }
     } else {
 #pragma unroll
-        for (int t = 0; t < RPL; t++) {
-            const bool ok = g < G && g * RPL + t < R;
-            wv[t] = ok ? A.w[(size_t)b * R * R + (g * RPL + t) * R + j] : 0.f;
-        }
+        for (int t = 0; t < RQ; t++) wv[t] = ok(t) ? A.w[(size_t)b * R * R + (wr * RQ + t) * R + j] : 0.f;
     }
+    double wacc[1] = {0.0};
 #pragma unroll
-    for (int t = 0; t < RPL; t++) {
-        const bool ok = g < G && g * RPL + t < R;
-        if (ok) wm += wv[t] * fabsf(bl[t]);
-    }
-    wm = wave_sum(wm);
+    for (int t = 0; t < RQ; t++)
+        if (ok(t)) wacc[0] += (double)(wv[t] * fabsf(bl[t]));
+    block_sum4<1>(wacc, lds.red, lane, wr);
+    const float wm = (float)wacc[0];
     const float inv = 1.f / ssum, c2 = wm * inv * inv;
 
-    // ---- dL/dBl at the ROI pixels, stored as Bt[i][j] in LDS ----
-    float* Bt = Tt;               // R*R floats
+    // ---- dL/dBl at the ROI pixels, Bt[i][j]
 #pragma unroll
-    for (int t = 0; t < RPL; t++) {
-        const bool ok = g < G && g * RPL + t < R;
+    for (int t = 0; t < RQ; t++) {
         const float sg = bl[t] < 0.f ? -1.f : 1.f;
-        if (ok) Bt[(g * RPL + t) * R + j] = sg * (wv[t] * inv - c2);
+        if (ok(t)) lds.Bt[(wr * RQ + t) * R + j] = sg * (wv[t] * inv - c2);
     }
-    wave_lds_fence();
-    // column adjoint for this lane's grid row a: u(j) = sum_i h1[(i - R/2 - a) & 63] Bt[i][j]
-    float u[R], us[SIG ? R : 1];
+    __syncthreads();
+    // ---- column adjoint for grid row a = lane, crop columns [wr RQ, ...):
+    // u(a, jj) = sum_i h1((i - R/2 - a) & 63) Bt[i][jj]
     {
+        const int jn = wr < 3 ? RQ : N3;
         float c[R], cs[SIG ? R : 1];
 #pragma unroll
         for (int i = 0; i < R; i++) {
@@ -184,152 +296,114 @@ __global__ __launch_bounds__(64 * WPB) void k_psf_adjoint64(Model64 M, AdjArgs A
             if constexpr (SIG) cs[i] = lds.hs[(i - R / 2 - lane) & 63];
         }
 #pragma unroll
-        for (int jj = 0; jj < R; jj++) {
-            float acc = 0.f, accs = 0.f;
+        for (int q = 0; q < RQ; q++) {
+            if (q < jn) {
+                const int jj = wr * RQ + q;
+                float acc = 0.f, accs = 0.f;
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                const float bt = Bt[i * R + jj];
-                acc += c[i] * bt;
-                if constexpr (SIG) accs += cs[i] * bt;
+                for (int i = 0; i < R; i++) {
+                    const float bt = lds.Bt[i * R + jj];
+                    acc += c[i] * bt;
+                    if constexpr (SIG) accs += cs[i] * bt;
+                }
+                lds.U[lane * R + jj] = acc;
+                if constexpr (SIG) lds.Us[lane * R + jj] = accs;
             }
-            u[jj] = acc;
-            if constexpr (SIG) us[jj] = accs;
         }
     }
-    // Pin u and us here: they are read only in the row adjoint, and without this
-    // the compiler sinks their sums there, keeping all R x R loads of Bt live.
-#pragma unroll
-    for (int jj = 0; jj < R; jj++) {
-        __asm__ __volatile__("" : "+v"(u[jj]));
-        if constexpr (SIG) __asm__ __volatile__("" : "+v"(us[jj]));
-    }
-    const float hsreg = SIG ? lds.hs[lane] : 0.f;
+    __syncthreads();
+    // ---- row adjoint and dL/dE = 2 dL/dI E at the columns 4k + wr of row a = lane
     float gsig = 0.f;
-    // row adjoint along b, dL/dE = 2 dL/dI E, then the adjoint of the
-    // stage-2 IDFT: S1bar(a, n) = sum_b dL/dE(a,b) exp(-2 pi i n b / 64)
-    float sr[32], si[32];
+    {
+        float u[R], us[SIG ? R : 1];
 #pragma unroll
-    for (int k = 0; k < 32; k++) {
-        sr[k] = 0.f;
-        si[k] = 0.f;
+        for (int jj = 0; jj < R; jj++) {
+            u[jj] = lds.U[lane * R + jj];
+            if constexpr (SIG) us[jj] = lds.Us[lane * R + jj];
+        }
+        const float hsreg = SIG ? lds.hs[lane] : 0.f;
+        float ib[16], ibs[SIG ? 16 : 1];
+        switch (wr) {
+            case 0: row_adjoint_q<R, 0, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
+            case 1: row_adjoint_q<R, 1, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
+            case 2: row_adjoint_q<R, 2, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
+            default: row_adjoint_q<R, 3, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
+        }
+        float dr[16], di[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int f = fidx<16>(k);
+            if constexpr (SIG) gsig += ibs[k] * (Er[f] * Er[f] + Ei[f] * Ei[f]);
+            dr[k] = 2.f * ib[k] * Er[f];
+            di[k] = 2.f * ib[k] * Ei[f];
+        }
+        switch (wr) {
+            case 0: stage2_adj_q<0>(dr, di, lds.Pr[0], lds.Pi[0], lane); break;
+            case 1: stage2_adj_q<1>(dr, di, lds.Pr[1], lds.Pi[1], lane); break;
+            case 2: stage2_adj_q<2>(dr, di, lds.Pr[2], lds.Pi[2], lane); break;
+            default: stage2_adj_q<3>(dr, di, lds.Pr[3], lds.Pi[3], lane); break;
+        }
     }
-    static_for<0, 2>([&](auto hpc) {
-        constexpr int HP = decltype(hpc)::value;
-        // E at the columns 2m + HP of row lane, recomputed from the stage-1 data:
-        // column 2m + HP at index fidx(m)
-        float Eh_r[32], Eh_i[32];
-        stage2_half64<HP>(S, lane, Eh_r, Eh_i);
-        float er[32], ei[32];
-        // dL/dI(a, b) at the columns b = 2m + HP: ib[m] = sum_jj h1((jj - R/2 - b) & 63) u[jj],
-        // grouped by tap index c = (jj - R/2 - b) & 63 (one broadcast per tap and
-        // group, as the forward row blur does; jj = (c + R/2 + b) & 63 is a
-        // constant), MG columns at a time: the sigma form keeps two accumulators
-        // per column
-        constexpr int MG = SIG ? 8 : 16;
-        static_for<0, 32 / MG>([&](auto gc) {
-            constexpr int m0 = decltype(gc)::value * MG;
-            float ib[MG], ibs[SIG ? MG : 1];
-#pragma unroll
-            for (int t = 0; t < MG; t++) {
-                ib[t] = 0.f;
-                if constexpr (SIG) ibs[t] = 0.f;
-            }
-            static_for<0, 64>([&](auto cc) {
-                constexpr int c = decltype(cc)::value;
-                const float hc = bcast(h1reg, c);
-                const float hsc = SIG ? bcast(hsreg, c) : 0.f;
-                static_for<0, MG>([&](auto tc) {
-                    constexpr int t = decltype(tc)::value;
-                    constexpr int jj = (c + R / 2 + 2 * (m0 + t) + HP) & 63;
-                    if constexpr (jj < R) {
-                        ib[t] += hc * u[jj];
-                        if constexpr (SIG) ibs[t] += hsc * u[jj] + hc * us[jj];
-                    }
-                });
-                __builtin_amdgcn_sched_barrier(0);
-            });
-            static_for<0, MG>([&](auto tc) {
-                constexpr int t = decltype(tc)::value, m = m0 + t;
-                const float Erm = Eh_r[fidx<32>(m)], Eim = Eh_i[fidx<32>(m)];
-                if constexpr (SIG) gsig += ibs[t] * (Erm * Erm + Eim * Eim);
-                er[m] = 2.f * ib[t] * Erm;
-                ei[m] = 2.f * ib[t] * Eim;
-            });
-        });
-        fft_dif<32, -1>(er, ei);
-        // output n (mod 32) sits at fidx(n); add exp(-2 pi i n HP / 64) * Y[n]
-        static_for<0, 32>([&](auto jc) {
-            constexpr int jslot = decltype(jc)::value;   // n = slot_n(jslot)
-            constexpr int k = fidx<32>((slot_n(jslot) + 32) % 32);
-            float r = er[k], i = ei[k];
-            if constexpr (HP == 1) twiddle_mul<64, (64 - ((slot_n(jslot) + 64) % 64)) % 64, +1>(r, i);
-            sr[jslot] += r;
-            si[jslot] += i;
-        });
-    });
-    wave_lds_fence();
-    // S1bar(a = lane, v) with v = slot_n(jslot) + 16, layout [a][v] stride kS1
-#pragma unroll
-    for (int jslot = 0; jslot < 32; jslot++) {
-        const int v = slot_n(jslot) + 16;
-        S.re[lane * kS1 + v] = sr[jslot];
-        S.im[lane * kS1 + v] = si[jslot];
+    __syncthreads();
+    // ---- dL/d(stage-1 output) S1bar(a, v) = sum of the four partials, in wave order
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+        const int a = e >> 5, js = e & 31;
+        const int v = slot_n(js) + 16;
+        const int o = a * kP + js;
+        lds.S.re[a * kS1 + v] = ((lds.Pr[0][o] + lds.Pr[1][o]) + lds.Pr[2][o]) + lds.Pr[3][o];
+        lds.S.im[a * kS1 + v] = ((lds.Pi[0][o] + lds.Pi[1][o]) + lds.Pi[2][o]) + lds.Pi[3][o];
     }
-    wave_lds_fence();
-    // adjoint of stage 1: lane (v, h) sums over rows a = 2m + h
+    __syncthreads();
+    // ---- adjoint of stage 1: lane (v, hh) sums over rows a = 2m + hh; wave wr
+    // then takes the window frequencies of slots [8 wr, 8 wr + 8)
     const int v = lane & 31, hh = lane >> 5;
     float pr[32], pi[32];
 #pragma unroll
     for (int m = 0; m < 32; m++) {
-        pr[m] = S.re[(2 * m + hh) * kS1 + v];
-        pi[m] = S.im[(2 * m + hh) * kS1 + v];
+        pr[m] = lds.S.re[(2 * m + hh) * kS1 + v];
+        pi[m] = lds.S.im[(2 * m + hh) * kS1 + v];
     }
     fft_dif<32, -1>(pr, pi);
     const float nv = (float)(v - 16);
-    const float Xh = X + (float)hh * (float)(2.0 * kPi / 64.0);
     const float cY = Y * (nv + 0.5f) + 0.5f * X;
-    float gx = 0.f, gy = 0.f, gz = 0.f, go = 0.f;
-#pragma unroll
-    for (int jslot = 0; jslot < 32; jslot++) {
-        const int nu = slot_n(jslot);
-        const int k = fidx<32>((nu + 32) % 32);
-        // exp(-2 pi i nu hh / 64): fold into the phase as in the forward pass
-        const float tw = -(float)hh * (float)(2.0 * kPi / 64.0) * (float)nu;
-        float st, ct;
-        sincos_reduced(tw, st, ct);
-        float r = pr[k] * ct - pi[k] * st;
-        float i = pr[k] * st + pi[k] * ct;
-        // sum of the two row halves (lane v and lane v + 32)
-        r += __shfl_xor(r, 32, 64);
-        i += __shfl_xor(i, 32, 64);
-        const int wi = (nu + 16) * kW + v;
-        const float q = T.phi[wi] + X * (float)nu + cY + T.kz[wi] * z - T.kzo[wi] * zoil;
-        float s, c;
-        sincos_reduced(q, s, c);
-        const float amp = T.amp[wi];
-        const float qb = amp * (-s * r + c * i);
-        if (hh == 0) {
-            gx += M.kappa * ((float)nu + 0.5f) * qb;
-            gy += M.kappa * (nv + 0.5f) * qb;
-            gz += T.kz[wi] * qb;
-            go -= T.kzo[wi] * qb;
-            if (A.qbar) A.qbar[(size_t)b * 1024 + wi] = qb;
+    double g4[4] = {0.0, 0.0, 0.0, 0.0};
+    static_for<0, 32>([&](auto jc) {
+        constexpr int jslot = decltype(jc)::value;
+        if (jslot / 8 == wr) {
+            constexpr int nu = slot_n(jslot);
+            constexpr int k = fidx<32>((nu + 32) % 32);
+            // exp(-2 pi i nu hh / 64)
+            const float tw = -(float)hh * (float)(2.0 * kPi / 64.0) * (float)nu;
+            float st, ct;
+            sincos_reduced(tw, st, ct);
+            float r = pr[k] * ct - pi[k] * st;
+            float i = pr[k] * st + pi[k] * ct;
+            // sum of the two row parities (lane v and lane v + 32)
+            r += __shfl_xor(r, 32, 64);
+            i += __shfl_xor(i, 32, 64);
+            const int wi = (nu + 16) * kW + v;
+            const float q = T.phi[wi] + X * (float)nu + cY + T.kz[wi] * z - T.kzo[wi] * zoil;
+            float s, c;
+            sincos_reduced(q, s, c);
+            const float qb = T.amp[wi] * (-s * r + c * i);
+            if (hh == 0) {
+                g4[0] += (double)(M.kappa * ((float)nu + 0.5f) * qb);
+                g4[1] += (double)(M.kappa * (nv + 0.5f) * qb);
+                g4[2] += (double)(T.kz[wi] * qb);
+                g4[3] -= (double)(T.kzo[wi] * qb);
+                if (A.qbar) A.qbar[(size_t)b * 1024 + wi] = qb;
+            }
         }
-    }
-    (void)Xh;
-    gx = wave_sum(gx);
-    gy = wave_sum(gy);
-    gz = wave_sum(gz);
-    go = wave_sum(go);
-    if (lane == 0) {
-        A.dparams[b * 4 + 0] = gx;
-        A.dparams[b * 4 + 1] = gy;
-        A.dparams[b * 4 + 2] = gz;
-        A.dparams[b * 4 + 3] = go;
-    }
-    if constexpr (SIG) {
-        gsig = wave_sum(gsig);
-        if (lane == 0) A.dsig[b] = (double)gsig;
+    });
+    block_sum4<4>(g4, lds.red, lane, wr);
+    double gs[1] = {(double)gsig};
+    if constexpr (SIG) block_sum4<1>(gs, lds.red, lane, wr);
+    if (threadIdx.x == 0) {
+        A.dparams[b * 4 + 0] = (float)g4[0];
+        A.dparams[b * 4 + 1] = (float)g4[1];
+        A.dparams[b * 4 + 2] = (float)g4[2];
+        A.dparams[b * 4 + 3] = (float)g4[3];
+        if constexpr (SIG) A.dsig[b] = gs[0];
     }
 }
 

@@ -39,13 +39,13 @@ __global__ __launch_bounds__(256) void k_phase_tab64(int J, CoefVals cv, const d
     const int w = blockIdx.x * 256 + threadIdx.x;
     if (w >= 1024) return;
     double ph = 0.0;
-    // eight basis loads in flight, then the products and sums in j order
-    for (int j0 = 0; j0 < J; j0 += 8) {
-        double zv[8];
+    // 32 basis loads in flight, then the products and sums in j order
+    for (int j0 = 0; j0 < J; j0 += 32) {
+        double zv[32];
 #pragma unroll
-        for (int k = 0; k < 8; k++) zv[k] = j0 + k < J ? Zw[(size_t)(j0 + k) * 1024 + w] : 0.0;
+        for (int k = 0; k < 32; k++) zv[k] = j0 + k < J ? Zw[(size_t)(j0 + k) * 1024 + w] : 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < 32; k++)
             if (j0 + k < J) ph = ph + zv[k] * cv.a[j0 + k];
     }
     const float f = (float)ph;

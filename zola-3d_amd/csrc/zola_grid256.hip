@@ -490,12 +490,12 @@ __global__ __launch_bounds__(256) void k_phase256(int J, CoefVals cv, const doub
     const int w = blockIdx.x * 256 + threadIdx.x;
     if (w >= kWin) return;
     double ph = 0.0;
-    for (int j0 = 0; j0 < J; j0 += 8) {
-        double zv[8];
+    for (int j0 = 0; j0 < J; j0 += 32) {
+        double zv[32];
 #pragma unroll
-        for (int k = 0; k < 8; k++) zv[k] = j0 + k < J ? Zw[(size_t)(j0 + k) * kWin + w] : 0.0;
+        for (int k = 0; k < 32; k++) zv[k] = j0 + k < J ? Zw[(size_t)(j0 + k) * kWin + w] : 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < 32; k++)
             if (j0 + k < J) ph = ph + zv[k] * cv.a[j0 + k];
     }
     tab[w].x = ph;

@@ -359,11 +359,11 @@ hipError_t launch_init_r(int B, const float* pixels, int nZ, const float* tmpl, 
 
 template <int R>
 hipError_t launch_adjoint_r(const Model64& M, const AdjArgs& A, hipStream_t st) {
-    constexpr int W = 4;
-    dim3 grid((A.B + W - 1) / W), block(64 * W);
-    if (A.pixels && A.dsig) hipLaunchKernelGGL((k_psf_adjoint64<R, W, true, true>), grid, block, 0, st, M, A);
-    else if (A.pixels) hipLaunchKernelGGL((k_psf_adjoint64<R, W, true>), grid, block, 0, st, M, A);
-    else hipLaunchKernelGGL((k_psf_adjoint64<R, W, false>), grid, block, 0, st, M, A);
+    // one PSF per block of four waves
+    dim3 grid(A.B), block(256);
+    if (A.pixels && A.dsig) hipLaunchKernelGGL((k_psf_adjoint64<R, true, true>), grid, block, 0, st, M, A);
+    else if (A.pixels) hipLaunchKernelGGL((k_psf_adjoint64<R, true>), grid, block, 0, st, M, A);
+    else hipLaunchKernelGGL((k_psf_adjoint64<R, false>), grid, block, 0, st, M, A);
     return hipGetLastError();
 }
 
