@@ -107,7 +107,7 @@ __device__ __forceinline__ double scmos_term(const float* vroi, int lane, int wr
 __constant__ const double kLogSeries[10] = {1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0, 1.0 / 15.0, 1.0 / 13.0,
                                             1.0 / 11.0, 1.0 / 9.0,  1.0 / 7.0,  1.0 / 5.0,  1.0 / 3.0};
 
-__device__ __forceinline__ double log_pos(double x) {
+__device__ __forceinline__ double log_pos(double x, const double (&C)[10]) {
     int e;
     double m = frexp(x, &e);
     if (m < 0.70710678118654752440) {
@@ -117,12 +117,6 @@ __device__ __forceinline__ double log_pos(double x) {
     const double f = m - 1.0;
     const double s = f / (f + 2.0);
     const double s2 = s * s;
-    // The series coefficients are read through a pointer the compiler cannot
-    // see through: scalar loads at each use.  As literals they are built in
-    // VGPR pairs once, hoisted out of the fit loops and held (or spilled)
-    // through every evaluation.
-    const double* C = kLogSeries;
-    __asm__ __volatile__("" : "+s"(C));
     double t = C[0];
 #pragma unroll
     for (int k = 1; k < 10; k++) t = t * s2 + C[k];
@@ -140,12 +134,21 @@ __device__ __forceinline__ double nll64(int lane, int wr, typename SY::Red& red,
                                         const float (&n)[SY::template RPL<R>], double A, double Bg,
                                         const float* vroi) {
     constexpr int RPL = SY::template RPL<R>;
+    // The series coefficients of log_pos are read once per call through a
+    // pointer the compiler cannot see through (scalar loads into SGPRs).  As
+    // literals they are built in VGPR pairs, hoisted out of the fit loops and
+    // held (or spilled) through every evaluation.
+    const double* Cp = kLogSeries;
+    __asm__ __volatile__("" : "+s"(Cp));
+    double C[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) C[k] = Cp[k];
     double acc = 0.0;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
         if (SY::template valid<R>(lane, wr, t)) {
             const double mu = (double)psf[t] * A + Bg + scmos_term<SY, R>(vroi, lane, wr, t);
-            acc += mu - (double)n[t] * log_pos(mu);
+            acc += mu - (double)n[t] * log_pos(mu, C);
         }
     }
     return SY::sum(acc, red, lane, wr);
