@@ -236,11 +236,12 @@ int zola_calib_fit(zola_ctx* ctx, int nb, int S, const float* pixels, const doub
             }
         // the previous evaluation ended with a stream synchronisation, so the
         // pinned input span is free to be rewritten above
-        if (hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, st) ||
-            hipMemsetAsync(d.da, 0, jz1 * sizeof(double), st))
-            return ZOLA_EHIP;
+        if (hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, st)) return ZOLA_EHIP;
+        // dL/da is written, not accumulated: no clearing pass before the evaluation
+        ctx->da_assign = true;
         r = zola_calib_eval_dev(ctx, B, d.params, d.pixels, d.amp, d.bg, d.nll, d.dparams, d.damp,
                                 d.dbg, d.da);
+        ctx->da_assign = false;
         if (r) return r;
         if (hipMemcpyAsync(hout, dout, out_bytes, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st))
             return ZOLA_EHIP;
@@ -427,15 +428,16 @@ int zola_calib_fit_full(zola_ctx* ctx, zola_calib_beads* bd, double* sigma, doub
                 hamp[i] = bd->A[i] + fa[b] + fae[i];
             }
         std::memcpy(hpoly, poly.data(), 6 * (size_t)nb * sizeof(double));
-        if (hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, st) ||
-            hipMemsetAsync(dout + o_da, 0, jz1 * sizeof(double), st))
-            return ZOLA_EHIP;
+        if (hipMemcpyAsync(din, hin, in_bytes, hipMemcpyHostToDevice, st)) return ZOLA_EHIP;
+        // dL/da is written, not accumulated: no clearing pass before the evaluation
+        ctx->da_assign = true;
         r = zola_calib_eval_full_dev(ctx, nb, S, (const float*)(din + o_par), d_px, (const double*)(din + o_amp),
                                      (const double*)(din + o_poly), d_off, (double*)(dout + o_nll),
                                      (float*)(dout + o_dp), (double*)(dout + o_dA),
                                      f_bg ? (double*)(dout + o_dpoly) : nullptr,
                                      f_sig ? (double*)(dout + o_dsig) : nullptr,
                                      nc > 0 ? (double*)(dout + o_da) : nullptr);
+        ctx->da_assign = false;
         if (r) return r;
         if (hipMemcpyAsync(hout, dout, out_bytes, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st))
             return ZOLA_EHIP;

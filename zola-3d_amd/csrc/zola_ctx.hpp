@@ -63,6 +63,9 @@ struct zola_ctx {
     int layout[4][2] = {};
     // LM-form layout of the 128-grid fits (ZOLA_LAYOUT_AUTO, QUAD or OCT)
     int layout128 = 0;
+    // set by the calibration loops around their evaluations: the full-model
+    // evaluation writes dL/da instead of adding to it (no clearing pass)
+    bool da_assign = false;
     // Entry points on one context run one at a time (CtxLock): host threads may
     // share a handle, their calls are serialised.
     mutable std::recursive_mutex mu;

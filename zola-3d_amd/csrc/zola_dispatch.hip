@@ -124,8 +124,8 @@ hipError_t zola_launch_adjoint64(const Model64& M, int R, const AdjArgs& A, hipS
 }
 
 hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* Q, double* da,
-                                       hipStream_t st) {
-    return launch_window_projection<float, 256>(B, J, kW * kW, Zw, qbar, Q, da, st);
+                                       hipStream_t st, bool assign) {
+    return launch_window_projection<float, 256>(B, J, kW * kW, Zw, qbar, Q, da, st, assign);
 }
 
 // 128 grid: the plugin's default patch (32, blur grid 64), smaller even patches

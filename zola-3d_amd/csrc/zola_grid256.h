@@ -32,6 +32,7 @@ struct Eval256 {
     double* dpoly = nullptr;
     double* dsig = nullptr;
     double* da = nullptr;
+    bool da_assign = false;   // write dL/da instead of adding to it
 };
 
 int zola256_setup(zola_ctx* c);           // device tables of a new N = 256 context

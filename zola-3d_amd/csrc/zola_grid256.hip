@@ -626,7 +626,7 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
         HIP_TRY(c, hipGetLastError());
     }
     if (e.da && c->J > 0) {
-        HIP_TRY(c, (launch_window_projection<double, 1024>(B, c->J, kWin, c->d_Zw, qbar, Qs, e.da, st)));
+        HIP_TRY(c, (launch_window_projection<double, 1024>(B, c->J, kWin, c->d_Zw, qbar, Qs, e.da, st, e.da_assign)));
     }
     return ZOLA_OK;
 }

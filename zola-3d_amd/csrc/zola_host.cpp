@@ -613,6 +613,7 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
         e.damp = damp;
         e.dbg = dbg;
         e.da = da;
+        e.da_assign = c->da_assign;
         return zola256_run(c, e, c->stream);
     }
     float* qbar = nullptr;
@@ -640,7 +641,7 @@ int zola_calib_eval_dev(zola_ctx* c, int B, const float* params, const float* pi
     A.dbg = dbg;
     A.S = 1;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
-    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream, c->da_assign));
     return ZOLA_OK;
 }
 
@@ -671,6 +672,7 @@ int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, co
         e.dpoly = dpoly;
         e.dsig = dsigma;
         e.da = da;
+        e.da_assign = c->da_assign;
         return zola256_run(c, e, c->stream);
     }
     float* qbar = nullptr;
@@ -700,7 +702,7 @@ int zola_calib_eval_full_dev(zola_ctx* c, int nb, int S, const float* params, co
     A.dpoly = dpoly;
     A.dsig = dsigma;
     HIP_TRY(c, zola_launch_adjoint64(c->model, c->R, A, c->stream));
-    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream));
+    if (qbar) HIP_TRY(c, zola_launch_project_zernike(B, c->J, c->d_Zw, qbar, qsum_slot(qbar, B), da, c->stream, c->da_assign));
     return ZOLA_OK;
 }
 

@@ -43,7 +43,7 @@ hipError_t zola_launch_phase_tab64(int J, const double* a, const double* Zw, flo
 // da[0..J) += sum_p Z_j(p) Q(p) over the 32x32 window, Q(p) = sum_b qbar[b][p]
 // formed into Q (1024 doubles of device memory).
 hipError_t zola_launch_project_zernike(int B, int J, const double* Zw, const float* qbar, double* Q,
-                                       double* da, hipStream_t st);
+                                       double* da, hipStream_t st, bool assign = false);
 
 // 128x128 pupil grid
 bool zola_kernels_support_roi128(int R);

@@ -49,7 +49,8 @@ __global__ __launch_bounds__(kQsumPoints * kQsumGroups) void k_window_qsum(int B
 
 template <int NT>
 __global__ __launch_bounds__(NT) void k_window_project(int W, const double* __restrict__ Zw,
-                                                       const double* __restrict__ Q, double* __restrict__ da) {
+                                                       const double* __restrict__ Q, double* __restrict__ da,
+                                                       int assign) {
     constexpr int kU = 8;
     __shared__ double red[NT];
     const double* z = Zw + (size_t)blockIdx.x * W;
@@ -71,19 +72,20 @@ __global__ __launch_bounds__(NT) void k_window_project(int W, const double* __re
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) da[blockIdx.x] += red[0];
+    if (threadIdx.x == 0) da[blockIdx.x] = assign ? red[0] : da[blockIdx.x] + red[0];
 }
 
-// Q into `Q` (W doubles), then da[0..J) += Z Q.  W a multiple of kQsumPoints.
+// Q into `Q` (W doubles), then da[0..J) += Z Q (assign: da = Z Q, so that a
+// caller need not clear da first).  W a multiple of kQsumPoints.
 template <class T, int NT>
 inline hipError_t launch_window_projection(int B, int J, int W, const double* Zw, const T* qbar, double* Q,
-                                           double* da, hipStream_t st) {
+                                           double* da, hipStream_t st, bool assign = false) {
     if (B <= 0 || J <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_window_qsum<T>, dim3((W + kQsumPoints - 1) / kQsumPoints),
                        dim3(kQsumPoints * kQsumGroups), 0, st, B, W, qbar, Q);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_window_project<NT>, dim3(J), dim3(NT), 0, st, W, Zw, (const double*)Q, da);
+    hipLaunchKernelGGL(k_window_project<NT>, dim3(J), dim3(NT), 0, st, W, Zw, (const double*)Q, da, assign ? 1 : 0);
     return hipGetLastError();
 }
 
