@@ -61,15 +61,22 @@ struct AdjArgs {
 
 constexpr int kPq = 33;   // row stride of the stage-2 adjoint partials
 
-template <int R>
-struct Adj64q {
-    static constexpr int RQ = Roi64q<R>::RQ;   // ROI rows / crop columns per wave
-    static constexpr int N3 = R - 3 * RQ;      // crop columns of wave 3
+// Work split of a PSF over NW waves (4 or 8): stage-2 columns b = NW k + w
+// (CPW = 64 / NW of them), crop columns [J0, J0 + NJ) and ROI rows
+// [w RQ, w RQ + RQ) of wave w.
+template <int R, int NW>
+struct AdjSplit {
+    static constexpr int CPW = 64 / NW;
+    static constexpr int RQ = (R + NW - 1) / NW;
+    template <int W>
+    static constexpr int J0 = W * RQ < R ? W * RQ : R;
+    template <int W>
+    static constexpr int NJ = (W * RQ + RQ <= R ? RQ : R - J0<W>);
 };
 
-// Block sums of K doubles over the four waves, added in wave order.
-template <int K>
-__device__ __forceinline__ void block_sum4(double (&v)[K], double (*red)[16], int lane, int wr) {
+// Block sums of K doubles over the NW waves, added in wave order.
+template <int NW, int K>
+__device__ __forceinline__ void block_sum_w(double (&v)[K], double (*red)[16], int lane, int wr) {
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
     if (lane == 0) {
@@ -78,19 +85,50 @@ __device__ __forceinline__ void block_sum4(double (&v)[K], double (*red)[16], in
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; k++) v[k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    for (int k = 0; k < K; k++) {
+        double t = red[0][k];
+#pragma unroll
+        for (int q = 1; q < NW; q++) t += red[q][k];
+        v[k] = t;
+    }
     __syncthreads();
 }
 
-// Row adjoint of the columns b = 4k + Q of grid row a (lane): ib[k] =
+// Stage 2, output columns b = NW k + Q of row p: E(p, NW k + Q) at
+// fidx<CPW>(k).  The 32 signed frequencies n fold to n mod CPW after the
+// twiddle exp(2 pi i n Q / 64), then a CPW-point IDFT.
+template <int NW, int Q>
+__device__ __forceinline__ void stage2_split64(const WaveScratch64& S, int p, float (&re)[64 / NW],
+                                               float (&im)[64 / NW]) {
+    constexpr int CPW = 64 / NW;
+    __asm__ __volatile__("" ::: "memory");
+    static_for<0, CPW>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        float ar = 0.f, ai = 0.f;
+        static_for<0, 32 / CPW>([&](auto qc) {
+            constexpr int v = s + decltype(qc)::value * CPW;   // window column, n = v - 16
+            constexpr int n = v - 16;
+            float r = S.re[p * kS1 + v], i = S.im[p * kS1 + v];
+            twiddle_mul<64, ((n * Q) % 64 + 64) % 64, +1>(r, i);
+            ar += r;
+            ai += i;
+        });
+        re[s] = ar;
+        im[s] = ai;
+    });
+    fft_dif<CPW, +1>(re, im);
+}
+
+// Row adjoint of the columns b = NW k + Q of grid row a (lane): ib[k] =
 // sum_jj h1((jj - R/2 - b) & 63) u[jj], grouped by tap index c (one broadcast
 // per tap); SIG: ibs[k] = sum_jj hs((jj - R/2 - b) & 63) u[jj] + h1(..) us[jj].
-template <int R, int Q, bool SIG>
-__device__ __forceinline__ void row_adjoint_q(float h1reg, float hsreg, const float (&u)[R],
-                                              const float (&us)[SIG ? R : 1], float (&ib)[16],
-                                              float (&ibs)[SIG ? 16 : 1]) {
+template <int R, int NW, int Q, bool SIG>
+__device__ __forceinline__ void row_adjoint_split(float h1reg, float hsreg, const float (&u)[R],
+                                                  const float (&us)[SIG ? R : 1], float (&ib)[64 / NW],
+                                                  float (&ibs)[SIG ? 64 / NW : 1]) {
+    constexpr int CPW = 64 / NW;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
+    for (int k = 0; k < CPW; k++) {
         ib[k] = 0.f;
         if constexpr (SIG) ibs[k] = 0.f;
     }
@@ -98,9 +136,9 @@ __device__ __forceinline__ void row_adjoint_q(float h1reg, float hsreg, const fl
         constexpr int c = decltype(cc)::value;
         const float hc = bcast(h1reg, c);
         const float hsc = SIG ? bcast(hsreg, c) : 0.f;
-        static_for<0, 16>([&](auto kc) {
+        static_for<0, CPW>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            constexpr int jj = (c + R / 2 + 4 * k + Q) & 63;
+            constexpr int jj = (c + R / 2 + NW * k + Q) & 63;
             if constexpr (jj < R) {
                 ib[k] += hc * u[jj];
                 if constexpr (SIG) ibs[k] += hsc * u[jj] + hc * us[jj];
@@ -110,19 +148,35 @@ __device__ __forceinline__ void row_adjoint_q(float h1reg, float hsreg, const fl
     });
 }
 
-// Adjoint of stage 2 for the columns 4k + Q of row a: part[jslot] =
-// sum_k dE(a, 4k + Q) exp(-2 pi i n (4k + Q) / 64), n = slot_n(jslot).
-template <int Q>
-__device__ __forceinline__ void stage2_adj_q(float (&dr)[16], float (&di)[16], float* Pr, float* Pi, int a) {
-    fft_dif<16, -1>(dr, di);
+// Adjoint of stage 2 for the columns NW k + Q of row a: part[jslot] =
+// sum_k dE(a, NW k + Q) exp(-2 pi i n (NW k + Q) / 64), n = slot_n(jslot).
+// ADD: added to the partials already there (a second wave's share).
+template <int NW, int Q, bool ADD>
+__device__ __forceinline__ void stage2_adj_split(float (&dr)[64 / NW], float (&di)[64 / NW], float* Pr, float* Pi,
+                                                 int a) {
+    constexpr int CPW = 64 / NW;
+    fft_dif<CPW, -1>(dr, di);
     static_for<0, 32>([&](auto jc) {
         constexpr int jslot = decltype(jc)::value;
         constexpr int n = slot_n(jslot);
-        constexpr int k = fidx<16>((n + 16) % 16);
+        constexpr int k = fidx<CPW>(((n % CPW) + CPW) % CPW);
         float r = dr[k], i = di[k];
         twiddle_mul<64, ((-n * Q) % 64 + 64) % 64, +1>(r, i);
-        Pr[a * kPq + jslot] = r;
-        Pi[a * kPq + jslot] = i;
+        if constexpr (ADD) {
+            Pr[a * kPq + jslot] += r;
+            Pi[a * kPq + jslot] += i;
+        } else {
+            Pr[a * kPq + jslot] = r;
+            Pi[a * kPq + jslot] = i;
+        }
+    });
+}
+
+// Calls f(integral_constant<int, W>) for the wave index wr (wave-uniform).
+template <int NW, class F>
+__device__ __forceinline__ void per_wave(int wr, F&& f) {
+    static_for<0, NW>([&](auto wc) {
+        if (wr == decltype(wc)::value) f(wc);
     });
 }
 
@@ -131,10 +185,13 @@ __device__ __forceinline__ void stage2_adj_q(float (&dr)[16], float (&di)[16], f
 // d(Bl)/dsigma = (hs (x) h1 + h1 (x) hs) I with hs = dh1/dsigma, and
 //   dNLL/dsigma = sum_{a,b} I(a,b) sum_j [us(a,j) h1(j - b) + u(a,j) hs(j - b)]
 // with u, us the column adjoints of dL/dBl through h1 and hs.
-template <int R, bool CALIB, bool SIG = false>
-__global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
+template <int R, int NW, bool CALIB, bool SIG = false>
+__global__ __launch_bounds__(64 * NW) void k_psf_adjoint64(Model64 M, AdjArgs A) {
     static_assert(CALIB || !SIG, "the sigma derivative is formed in the calibration form");
-    constexpr int RQ = Adj64q<R>::RQ, N3 = Adj64q<R>::N3, kP = kPq;
+    static_assert(NW == 4 || NW == 8, "four or eight waves per PSF");
+    using SP = AdjSplit<R, NW>;
+    constexpr int CPW = SP::CPW, RQ = SP::RQ;
+    constexpr int kSlots = 4;   // stage-2 partial slots (eight waves: two per slot)
     struct Lds {
         BlockTables64 T;
         float hs[64];
@@ -142,9 +199,9 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
         float Bt[R * R];          // dL/dBl on the ROI
         float U[64 * R];          // column adjoint u(a, jj)
         float Us[SIG ? 64 * R : 1];
-        float Pr[4][64 * kP];     // stage-2 adjoint partials per wave
-        float Pi[4][64 * kP];
-        double red[4][16];
+        float Pr[kSlots][64 * kPq];   // stage-2 adjoint partials
+        float Pi[kSlots][64 * kPq];
+        double red[NW][16];
     };
     __shared__ Lds lds;
     BlockTables64& T = lds.T;
@@ -165,60 +222,55 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
     // ---- forward: stage 1 (wave 0)
     if (wr == 0) stage1_64(T, lds.S, lane, 0, X, Y, z, zoil, M.kappa);
     __syncthreads();
-    // ---- stage 2: E(a, 4k + wr) of row a = lane, kept for the backward pass
-    float Er[16], Ei[16];
-    switch (wr) {
-        case 0: stage2_q64<0>(lds.S, lane, Er, Ei); break;
-        case 1: stage2_q64<1>(lds.S, lane, Er, Ei); break;
-        case 2: stage2_q64<2>(lds.S, lane, Er, Ei); break;
-        default: stage2_q64<3>(lds.S, lane, Er, Ei); break;
-    }
+    // ---- stage 2: E(a, NW k + wr) of row a = lane, kept for the backward pass
+    float Er[CPW], Ei[CPW];
+    per_wave<NW>(wr, [&](auto wc) { stage2_split64<NW, decltype(wc)::value>(lds.S, lane, Er, Ei); });
     __syncthreads();  // all stage-1 reads are done
     float* const Sf = reinterpret_cast<float*>(&lds.S);   // the stage-1 buffer as one float array
     float* Ib = Sf;  // real [64][kIq]
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int f = fidx<16>(k);
-        Ib[lane * kIq + 4 * k + wr] = Er[f] * Er[f] + Ei[f] * Ei[f];
+    for (int k = 0; k < CPW; k++) {
+        const int f = fidx<CPW>(k);
+        Ib[lane * kIq + NW * k + wr] = Er[f] * Er[f] + Ei[f] * Ei[f];
     }
     __syncthreads();
-    // ---- row blur at the crop columns [wr RQ, ...) of row a = lane
-    float ra[1][RQ], rl[1][N3];
-    switch (wr) {
-        case 0: row_blur64q<R, 0 * RQ, RQ, 1>(Ib, Ib, lane, h1reg, 0.f, ra); break;
-        case 1: row_blur64q<R, 1 * RQ, RQ, 1>(Ib, Ib, lane, h1reg, 0.f, ra); break;
-        case 2: row_blur64q<R, 2 * RQ, RQ, 1>(Ib, Ib, lane, h1reg, 0.f, ra); break;
-        default: row_blur64q<R, 3 * RQ, N3, 1>(Ib, Ib, lane, h1reg, 0.f, rl); break;
-    }
+    // ---- row blur at the crop columns [J0, J0 + NJ) of row a = lane
+    float ra[1][RQ];
+    per_wave<NW>(wr, [&](auto wc) {
+        constexpr int W = decltype(wc)::value;
+        constexpr int NJ = SP::template NJ<W>;
+        if constexpr (NJ > 0) {
+            float rw[1][NJ];
+            row_blur64q<R, SP::template J0<W>, NJ, 1>(Ib, Ib, lane, h1reg, 0.f, rw);
+#pragma unroll
+            for (int jj = 0; jj < NJ; jj++) ra[0][jj] = rw[0][jj];
+        }
+    });
     __syncthreads();  // all intensity reads are done
     float* Rb = Sf;  // [column][row]
-    if (wr < 3) {
+    per_wave<NW>(wr, [&](auto wc) {
+        constexpr int W = decltype(wc)::value;
 #pragma unroll
-        for (int jj = 0; jj < RQ; jj++) Rb[(wr * RQ + jj) * kIq + lane] = ra[0][jj];
-    } else {
-#pragma unroll
-        for (int jj = 0; jj < N3; jj++) Rb[(3 * RQ + jj) * kIq + lane] = rl[0][jj];
-    }
+        for (int jj = 0; jj < SP::template NJ<W>; jj++) Rb[(SP::template J0<W> + jj) * kIq + lane] = ra[0][jj];
+    });
     __syncthreads();
     // ---- column blur: lane j, ROI rows [wr RQ, wr RQ + RQ)
     const int j = lane < R ? lane : 0;
     float bl[RQ], d0[RQ], d1[RQ], d2[RQ];
-    switch (wr) {
-        case 0: column_blur64q<R, 0 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
-        case 1: column_blur64q<R, 1 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
-        case 2: column_blur64q<R, 2 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
-        default: column_blur64q<R, 3 * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2); break;
-    }
+#pragma unroll
+    for (int t = 0; t < RQ; t++) bl[t] = 0.f;
+    per_wave<NW>(wr, [&](auto wc) {
+        constexpr int W = decltype(wc)::value;
+        if constexpr (W * RQ < R) column_blur64q<R, W * RQ, RQ, 1>(Rb, j, h1reg, 0.f, bl, d0, d1, d2);
+    });
     auto ok = [&](int t) { return lane < R && wr * RQ + t < R; };
     float wv[RQ];
     double sacc[1] = {0.0};
 #pragma unroll
     for (int t = 0; t < RQ; t++)
         if (ok(t)) sacc[0] += (double)fabsf(bl[t]);
-    // the PSF sum in FP32 as the one-wave form forms it, here added per wave
-    // and over the waves in FP64 then rounded (the normalisation differs from
-    // the one-wave kernel in the last bit only)
-    block_sum4<1>(sacc, lds.red, lane, wr);
+    // the PSF sum added per wave and over the waves in FP64, then rounded
+    block_sum_w<NW, 1>(sacc, lds.red, lane, wr);
     const float ssum = (float)sacc[0];
     if constexpr (CALIB) {
         const double av = A.amp[b];
@@ -256,7 +308,7 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
                 wv[t] = (float)(av * r);
             }
         }
-        block_sum4<8>(acc, lds.red, lane, wr);
+        block_sum_w<NW, 8>(acc, lds.red, lane, wr);
         if (threadIdx.x == 0) {
             A.nll[b] = acc[0];
             A.damp[b] = acc[1];
@@ -274,7 +326,7 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
 #pragma unroll
     for (int t = 0; t < RQ; t++)
         if (ok(t)) wacc[0] += (double)(wv[t] * fabsf(bl[t]));
-    block_sum4<1>(wacc, lds.red, lane, wr);
+    block_sum_w<NW, 1>(wacc, lds.red, lane, wr);
     const float wm = (float)wacc[0];
     const float inv = 1.f / ssum, c2 = wm * inv * inv;
 
@@ -285,20 +337,20 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
         if (ok(t)) lds.Bt[(wr * RQ + t) * R + j] = sg * (wv[t] * inv - c2);
     }
     __syncthreads();
-    // ---- column adjoint for grid row a = lane, crop columns [wr RQ, ...):
+    // ---- column adjoint for grid row a = lane, this wave's crop columns:
     // u(a, jj) = sum_i h1((i - R/2 - a) & 63) Bt[i][jj]
     {
-        const int jn = wr < 3 ? RQ : N3;
         float c[R], cs[SIG ? R : 1];
 #pragma unroll
         for (int i = 0; i < R; i++) {
             c[i] = T.h3[64 + ((i - R / 2 - lane) & 63)];
             if constexpr (SIG) cs[i] = lds.hs[(i - R / 2 - lane) & 63];
         }
+        const int j0 = wr * RQ;
 #pragma unroll
         for (int q = 0; q < RQ; q++) {
-            if (q < jn) {
-                const int jj = wr * RQ + q;
+            const int jj = j0 + q;
+            if (jj < R) {
                 float acc = 0.f, accs = 0.f;
 #pragma unroll
                 for (int i = 0; i < R; i++) {
@@ -312,7 +364,7 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
         }
     }
     __syncthreads();
-    // ---- row adjoint and dL/dE = 2 dL/dI E at the columns 4k + wr of row a = lane
+    // ---- row adjoint and dL/dE = 2 dL/dI E at the columns NW k + wr of row a = lane
     float gsig = 0.f;
     {
         float u[R], us[SIG ? R : 1];
@@ -322,40 +374,48 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
             if constexpr (SIG) us[jj] = lds.Us[lane * R + jj];
         }
         const float hsreg = SIG ? lds.hs[lane] : 0.f;
-        float ib[16], ibs[SIG ? 16 : 1];
-        switch (wr) {
-            case 0: row_adjoint_q<R, 0, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
-            case 1: row_adjoint_q<R, 1, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
-            case 2: row_adjoint_q<R, 2, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
-            default: row_adjoint_q<R, 3, SIG>(h1reg, hsreg, u, us, ib, ibs); break;
-        }
-        float dr[16], di[16];
+        float ib[CPW], ibs[SIG ? CPW : 1];
+        per_wave<NW>(wr, [&](auto wc) {
+            row_adjoint_split<R, NW, decltype(wc)::value, SIG>(h1reg, hsreg, u, us, ib, ibs);
+        });
+        float dr[CPW], di[CPW];
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const int f = fidx<16>(k);
+        for (int k = 0; k < CPW; k++) {
+            const int f = fidx<CPW>(k);
             if constexpr (SIG) gsig += ibs[k] * (Er[f] * Er[f] + Ei[f] * Ei[f]);
             dr[k] = 2.f * ib[k] * Er[f];
             di[k] = 2.f * ib[k] * Ei[f];
         }
-        switch (wr) {
-            case 0: stage2_adj_q<0>(dr, di, lds.Pr[0], lds.Pi[0], lane); break;
-            case 1: stage2_adj_q<1>(dr, di, lds.Pr[1], lds.Pi[1], lane); break;
-            case 2: stage2_adj_q<2>(dr, di, lds.Pr[2], lds.Pi[2], lane); break;
-            default: stage2_adj_q<3>(dr, di, lds.Pr[3], lds.Pi[3], lane); break;
+        // waves 0..3 write the slots, waves 4..7 then add theirs (slot wr - 4)
+        if (wr < kSlots) {
+            per_wave<NW>(wr, [&](auto wc) {
+                constexpr int W = decltype(wc)::value;
+                if constexpr (W < kSlots) stage2_adj_split<NW, W, false>(dr, di, lds.Pr[W], lds.Pi[W], lane);
+            });
+        }
+        if constexpr (NW > kSlots) {
+            __syncthreads();
+            if (wr >= kSlots) {
+                per_wave<NW>(wr, [&](auto wc) {
+                    constexpr int W = decltype(wc)::value;
+                    if constexpr (W >= kSlots)
+                        stage2_adj_split<NW, W, true>(dr, di, lds.Pr[W - kSlots], lds.Pi[W - kSlots], lane);
+                });
+            }
         }
     }
     __syncthreads();
-    // ---- dL/d(stage-1 output) S1bar(a, v) = sum of the four partials, in wave order
-    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    // ---- dL/d(stage-1 output) S1bar(a, v) = sum of the slots, in slot order
+    for (int e = threadIdx.x; e < 64 * 32; e += 64 * NW) {
         const int a = e >> 5, js = e & 31;
         const int v = slot_n(js) + 16;
-        const int o = a * kP + js;
+        const int o = a * kPq + js;
         lds.S.re[a * kS1 + v] = ((lds.Pr[0][o] + lds.Pr[1][o]) + lds.Pr[2][o]) + lds.Pr[3][o];
         lds.S.im[a * kS1 + v] = ((lds.Pi[0][o] + lds.Pi[1][o]) + lds.Pi[2][o]) + lds.Pi[3][o];
     }
     __syncthreads();
     // ---- adjoint of stage 1: lane (v, hh) sums over rows a = 2m + hh; wave wr
-    // then takes the window frequencies of slots [8 wr, 8 wr + 8)
+    // then takes the window frequencies of slots [wr 32/NW, (wr + 1) 32/NW)
     const int v = lane & 31, hh = lane >> 5;
     float pr[32], pi[32];
 #pragma unroll
@@ -369,7 +429,7 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
     double g4[4] = {0.0, 0.0, 0.0, 0.0};
     static_for<0, 32>([&](auto jc) {
         constexpr int jslot = decltype(jc)::value;
-        if (jslot / 8 == wr) {
+        if (jslot / (32 / NW) == wr) {
             constexpr int nu = slot_n(jslot);
             constexpr int k = fidx<32>((nu + 32) % 32);
             // exp(-2 pi i nu hh / 64)
@@ -395,9 +455,9 @@ __global__ __launch_bounds__(256) void k_psf_adjoint64(Model64 M, AdjArgs A) {
             }
         }
     });
-    block_sum4<4>(g4, lds.red, lane, wr);
+    block_sum_w<NW, 4>(g4, lds.red, lane, wr);
     double gs[1] = {(double)gsig};
-    if constexpr (SIG) block_sum4<1>(gs, lds.red, lane, wr);
+    if constexpr (SIG) block_sum_w<NW, 1>(gs, lds.red, lane, wr);
     if (threadIdx.x == 0) {
         A.dparams[b * 4 + 0] = (float)g4[0];
         A.dparams[b * 4 + 1] = (float)g4[1];
