@@ -72,15 +72,21 @@ static int compute_h1(zola_ctx* c) {
         ct[m] = std::cos(2 * PI * (double)m / s2);
         st[m] = std::sin(2 * PI * (double)m / s2);
     }
+    // dG1/dsigma = -G1 kk^2 / (sigF^2 sigma)  (sigF = s2/(2 pi sigma)), formed
+    // once per k (the same operations in the same order as inside the sums);
+    // s2 is a power of two, so (k d) mod s2 is a mask
+    std::vector<double> Gs(s2);
+    for (int k = 0; k < s2; k++) {
+        const double kk = (double)((k + s2 / 2) % s2) - (double)(s2 / 2);
+        Gs[k] = G[k] * kk * kk / (sigpow * c->o.sigma);
+    }
     std::vector<double> h1(s2), hd(s2), hs(s2);
     for (int d = 0; d < s2; d++) {
         double acc = 0, accs = 0;
         for (int k = 0; k < s2; k++) {
-            // dG1/dsigma = -G1 kk^2 / (sigF^2 sigma)  (sigF = s2/(2 pi sigma))
-            double kk = (double)((k + s2 / 2) % s2) - (double)(s2 / 2);
-            const double cs = ct[(k * d) % s2];
+            const double cs = ct[(k * d) & (s2 - 1)];
             acc += G[k] * cs;
-            accs -= G[k] * kk * kk / (sigpow * c->o.sigma) * cs;
+            accs -= Gs[k] * cs;
         }
         h1[d] = acc / s2;
         hs[d] = accs / s2;
@@ -89,13 +95,15 @@ static int compute_h1(zola_ctx* c) {
     // emitter shift.  Shifting the emitter by x moves the (band-limited,
     // |k| <= 31) intensity by x/dx samples, so d(blur)/dx = (1/dx) hd (*) I with
     // hd = IDFT(i 2 pi k_s / s2 . G1); the Nyquist bin carries no intensity.
+    std::vector<double> Gd(s2);
+    for (int k = 0; k < s2; k++) {
+        int ks = k < s2 / 2 ? k : k - s2;
+        if (k == s2 / 2) ks = 0;
+        Gd[k] = G[k] * (2 * PI * ks / s2);
+    }
     for (int d = 0; d < s2; d++) {
         double acc = 0;
-        for (int k = 0; k < s2; k++) {
-            int ks = k < s2 / 2 ? k : k - s2;
-            if (k == s2 / 2) ks = 0;
-            acc -= G[k] * (2 * PI * ks / s2) * st[(k * d) % s2];
-        }
+        for (int k = 0; k < s2; k++) acc -= Gd[k] * st[(k * d) & (s2 - 1)];
         hd[d] = acc / s2 / c->o.xystep;
     }
     c->model_ver++;
