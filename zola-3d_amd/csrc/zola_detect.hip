@@ -376,11 +376,15 @@ __global__ void k_gather_rois(int W, int H, int R, int B, const int* __restrict_
 template <class IO, bool COL>
 static hipError_t launch_lines(int L, IO io, int nlines, double sgn, const double2* tw, hipStream_t st) {
     switch (L) {
+// column passes of 512 and longer lines: 4096 elements per block (8 columns,
+// 128-byte runs per row instead of 64)
 #define ZOLA_L(LL)                                                                                   \
-    case LL:                                                                                         \
-        hipLaunchKernelGGL((k_fft_lines<LL, COL, IO>), dim3(fft_lines_grid<LL>(nlines)), dim3(256), 0, \
-                           st, io, nlines, sgn, tw);                                                 \
-        break;
+    case LL: {                                                                                       \
+        constexpr int E = (COL && LL >= 512) ? 4096 : kFftElems;                                     \
+        hipLaunchKernelGGL((k_fft_lines<LL, COL, IO, E>), dim3(fft_lines_grid<LL, E>(nlines)), dim3(256), \
+                           0, st, io, nlines, sgn, tw);                                              \
+        break;                                                                                       \
+    }
         ZOLA_L(16) ZOLA_L(32) ZOLA_L(64) ZOLA_L(128) ZOLA_L(256) ZOLA_L(512) ZOLA_L(1024) ZOLA_L(2048)
 #undef ZOLA_L
         default:

@@ -1,7 +1,9 @@
 // zola_fft_f64.hpp -- batched FP64 line FFTs in LDS for the frame detector.
 //
-// A workgroup of 256 threads transforms C = 2048 / L lines of length L
-// (L a power of two, 16..2048) held in LDS as split re/im arrays (32 KiB).
+// A workgroup of 256 threads transforms C = E / L lines of length L (L a power
+// of two, 16..E; E = 2048 elements, 32 KiB of LDS as split re/im arrays, or
+// 4096 for the long column passes, so that a block reads 128-byte runs of
+// adjacent columns).
 // The transform is the Stockham autosort formulation (Govindaraju et al.,
 // "High Performance Discrete Fourier Transforms on Graphics Processors",
 // SC 2008): pass p with radix R and span Ns reads R elements L/R apart,
@@ -74,12 +76,12 @@ __device__ __forceinline__ void dft_r(double (&r)[R], double (&i)[R], double sgn
     else dft8(r, i, sgn);
 }
 
-// One Stockham pass over all C lines of the workgroup, in place (all reads
-// complete before the barrier, all writes after it).
-template <int L, int R, int NS>
+// One Stockham pass over all C = E / L lines of the workgroup, in place (all
+// reads complete before the barrier, all writes after it).
+template <int L, int R, int NS, int E>
 __device__ __forceinline__ void stockham_pass(double* sre, double* sim, const double2* __restrict__ tw,
                                               double sgn) {
-    constexpr int C = kFftElems / L;
+    constexpr int C = E / L;
     constexpr int NBT = (C * L / R) / 256;  // butterflies per thread
     static_assert(NBT >= 1, "too few butterflies per workgroup");
     double vr[NBT][R], vi[NBT][R];
@@ -122,25 +124,25 @@ __device__ __forceinline__ void stockham_pass(double* sre, double* sim, const do
 }
 
 // Radix plan: 8 while the remaining factor is >= 8, then 4 or 2.
-template <int L, int NS>
+template <int L, int NS, int E>
 __device__ __forceinline__ void stockham_all(double* sre, double* sim, const double2* tw, double sgn) {
     if constexpr (NS < L) {
         constexpr int rem = L / NS;
         constexpr int R = rem >= 8 ? 8 : rem;
-        stockham_pass<L, R, NS>(sre, sim, tw, sgn);
-        stockham_all<L, NS * R>(sre, sim, tw, sgn);
+        stockham_pass<L, R, NS, E>(sre, sim, tw, sgn);
+        stockham_all<L, NS * R, E>(sre, sim, tw, sgn);
     }
 }
 
-template <int L, bool COLMAJOR, class IO>
+template <int L, bool COLMAJOR, class IO, int E = kFftElems>
 __global__ __launch_bounds__(256) void k_fft_lines(IO io, int nlines, double sgn,
                                                    const double2* __restrict__ tw) {
-    static_assert(L >= 16 && L <= kFftElems && (L & (L - 1)) == 0, "L must be a power of 2 in [16, 2048]");
-    constexpr int C = kFftElems / L;
-    __shared__ double sre[kFftElems], sim[kFftElems];
+    static_assert(L >= 16 && L <= E && (L & (L - 1)) == 0, "L must be a power of 2 in [16, E]");
+    constexpr int C = E / L;
+    __shared__ double sre[E], sim[E];
     const int line0 = blockIdx.x * C;
 #pragma unroll 4
-    for (int m = 0; m < kFftElems / 256; m++) {
+    for (int m = 0; m < E / 256; m++) {
         const int e = threadIdx.x + 256 * m;
         const int c = COLMAJOR ? e % C : e / L;
         const int k = COLMAJOR ? e / C : e % L;
@@ -150,10 +152,10 @@ __global__ __launch_bounds__(256) void k_fft_lines(IO io, int nlines, double sgn
         sim[c * L + k] = v.y;
     }
     __syncthreads();
-    stockham_all<L, 1>(sre, sim, tw, sgn);
+    stockham_all<L, 1, E>(sre, sim, tw, sgn);
     const double sc = 1.0 / sqrt((double)L);
 #pragma unroll 4
-    for (int m = 0; m < kFftElems / 256; m++) {
+    for (int m = 0; m < E / 256; m++) {
         const int e = threadIdx.x + 256 * m;
         const int c = COLMAJOR ? e % C : e / L;
         const int k = COLMAJOR ? e / C : e % L;
@@ -161,10 +163,10 @@ __global__ __launch_bounds__(256) void k_fft_lines(IO io, int nlines, double sgn
     }
 }
 
-// Grid for k_fft_lines<L>.
-template <int L>
+// Grid for k_fft_lines<L, ., ., E>.
+template <int L, int E = kFftElems>
 inline int fft_lines_grid(int nlines) {
-    constexpr int C = kFftElems / L;
+    constexpr int C = E / L;
     return (nlines + C - 1) / C;
 }
 
