@@ -248,23 +248,34 @@ struct IoAdjStage1 {
     }
 };
 
-// Deterministic block sum of K doubles (256 threads); every thread gets the sums.
-template <int K>
-__device__ __forceinline__ void block_sum256(double (&v)[K], double* red) {
+// Deterministic block sum of K doubles over the NT / 64 waves of a block
+// (butterfly within each wave, then the wave totals in wave order); every
+// thread gets the sums.  red: K x (NT / 64) doubles.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum_nt(double (&v)[K], double* red) {
+    constexpr int NWV = NT / 64;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < K; k++) red[k * 256 + threadIdx.x] = v[k];
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < K; k++) red[k * 256 + threadIdx.x] += red[k * 256 + threadIdx.x + o];
-        }
-        __syncthreads();
+        for (int k = 0; k < K; k++) red[k * NWV + w] = v[k];
     }
+    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < K; k++) v[k] = red[k * 256];
+    for (int k = 0; k < K; k++) {
+        double t = red[k * NWV];
+        for (int q = 1; q < NWV; q++) t += red[k * NWV + q];
+        v[k] = t;
+    }
     __syncthreads();
 }
+
+// Threads per block of the per-PSF kernels (one block per PSF: a few hundred
+// blocks, so each needs many threads to keep its loads in flight); the
+// likelihood and window-reduction kernels hold more per thread and take 512
+// (256 registers).
+constexpr int kPsfThreads = 1024, kCropThreads = 512;
 
 struct CropArgs {
     int R, S;
@@ -283,16 +294,17 @@ struct CropArgs {
 // 4: one block per PSF.  Forward: psf = |B| / sum |B|.  Otherwise w = dL/dpsf
 // (given, or dNLL/dpsf of the calibration model) and dL/dB = (w / S -
 // sum(w |B|) / S^2) B / |B| on the crop.
-__global__ __launch_bounds__(256) void k_crop256(CropArgs A) {
-    __shared__ double red[9 * 256];
+__global__ __launch_bounds__(kCropThreads) void k_crop256(CropArgs A) {
+    constexpr int NT = kCropThreads;
+    __shared__ double red[9 * (NT / 64)];
     const int b = blockIdx.x, R = A.R, P = R * R;
     const double2* Bc = A.Bc + (size_t)b * P;
     double s1[1] = {0.0};
-    for (int p = threadIdx.x; p < P; p += 256) s1[0] += sqrt(Bc[p].x * Bc[p].x + Bc[p].y * Bc[p].y);
-    block_sum256<1>(s1, red);
+    for (int p = threadIdx.x; p < P; p += NT) s1[0] += sqrt(Bc[p].x * Bc[p].x + Bc[p].y * Bc[p].y);
+    block_sum_nt<NT, 1>(s1, red);
     const double S = s1[0];
     if (A.out) {
-        for (int p = threadIdx.x; p < P; p += 256)
+        for (int p = threadIdx.x; p < P; p += NT)
             A.out[(size_t)b * P + p] = (float)(sqrt(Bc[p].x * Bc[p].x + Bc[p].y * Bc[p].y) / S);
         return;
     }
@@ -308,7 +320,7 @@ __global__ __launch_bounds__(256) void k_crop256(CropArgs A) {
         }
     }
     const double av = A.pixels ? A.amp[b] : 0.0;
-    for (int p = threadIdx.x; p < P; p += 256) {
+    for (int p = threadIdx.x; p < P; p += NT) {
         const double m = sqrt(Bc[p].x * Bc[p].x + Bc[p].y * Bc[p].y);
         double wv;
         if (A.pixels) {
@@ -335,7 +347,7 @@ __global__ __launch_bounds__(256) void k_crop256(CropArgs A) {
         }
         acc[0] += wv * m;
     }
-    block_sum256<9>(acc, red);
+    block_sum_nt<NT, 9>(acc, red);
     if (threadIdx.x == 0 && A.pixels) {
         A.nll[b] = acc[1];
         A.damp[b] = acc[2];
@@ -344,7 +356,7 @@ __global__ __launch_bounds__(256) void k_crop256(CropArgs A) {
             for (int c = 0; c < 6; c++) A.dpoly[(size_t)b * 6 + c] = acc[3 + c];
     }
     const double wm = acc[0], iS = 1.0 / S, c2 = wm * iS * iS;
-    for (int p = threadIdx.x; p < P; p += 256) {
+    for (int p = threadIdx.x; p < P; p += NT) {
         const double2 bv = Bc[p];
         const double m = sqrt(bv.x * bv.x + bv.y * bv.y);
         double wv;
@@ -367,25 +379,27 @@ __global__ __launch_bounds__(256) void k_crop256(CropArgs A) {
 }
 
 // 5c: dNLL/dsigma = Re sum_k conj(FdB(k)) dG/dsigma(k) FI(k), one block per PSF.
-__global__ __launch_bounds__(256) void k_dsig256(int s2, const double2* FdB, const double2* FI, const double* gks,
-                                                 double* dsig) {
-    __shared__ double red[256];
+__global__ __launch_bounds__(kPsfThreads) void k_dsig256(int s2, const double2* FdB, const double2* FI,
+                                                         const double* gks, double* dsig) {
+    constexpr int NT = kPsfThreads;
+    __shared__ double red[NT / 64];
     const int b = blockIdx.x, P = s2 * s2;
     double a[1] = {0.0};
-    for (int p = threadIdx.x; p < P; p += 256) {
+    for (int p = threadIdx.x; p < P; p += NT) {
         const double2 u = FdB[(size_t)b * P + p], v = FI[(size_t)b * P + p];
         a[0] += gks[p] * (u.x * v.x + u.y * v.y);
     }
-    block_sum256<1>(a, red);
+    block_sum_nt<NT, 1>(a, red);
     if (threadIdx.x == 0) dsig[b] = a[0];
 }
 
 // 7a: dL/dx, dL/dy, dL/dz, dL/dzoil of each PSF from dL/dq at the window.
-__global__ __launch_bounds__(256) void k_dparams256(Geo g, const double* qbar, float* dparams) {
-    __shared__ double red[4 * 256];
+__global__ __launch_bounds__(kCropThreads) void k_dparams256(Geo g, const double* qbar, float* dparams) {
+    constexpr int NT = kCropThreads;
+    __shared__ double red[4 * (NT / 64)];
     const int b = blockIdx.x;
     double a[4] = {0, 0, 0, 0};
-    for (int w = threadIdx.x; w < kWin; w += 256) {
+    for (int w = threadIdx.x; w < kWin; w += NT) {
         const double gq = qbar[(size_t)b * kWin + w];
         if (gq == 0.0) continue;
         const int nu = w / kW4 - kW4 / 2, nv = w % kW4 - kW4 / 2;
@@ -395,7 +409,7 @@ __global__ __launch_bounds__(256) void k_dparams256(Geo g, const double* qbar, f
         a[2] += t.z * gq;
         a[3] -= t.w * gq;
     }
-    block_sum256<4>(a, red);
+    block_sum_nt<NT, 4>(a, red);
     if (threadIdx.x < 4) dparams[(size_t)b * 4 + threadIdx.x] = (float)a[threadIdx.x];
 }
 
@@ -605,7 +619,7 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
     A.damp = e.damp;
     A.dbg = e.dbg;
     A.dpoly = e.dpoly;
-    hipLaunchKernelGGL(k_crop256, dim3(B), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_crop256, dim3(B), dim3(kCropThreads), 0, st, A);
     HIP_TRY(c, hipGetLastError());
     if (!adjoint) return ZOLA_OK;
 
@@ -613,7 +627,7 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
     HIP_TRY(c, (lines_s2<false>(s2, IoCropRowsIn{s2, R, dBc, T}, B * R, -1.0, tw, st)));
     HIP_TRY(c, (lines_s2<true>(s2, IoColsCropIn{s2, R, T}, B * s2, -1.0, tw, st)));
     if (e.dsig) {
-        hipLaunchKernelGGL(k_dsig256, dim3(B), dim3(256), 0, st, s2, T, FI, gks, e.dsig);
+        hipLaunchKernelGGL(k_dsig256, dim3(B), dim3(kPsfThreads), 0, st, s2, T, FI, gks, e.dsig);
         HIP_TRY(c, hipGetLastError());
     }
     HIP_TRY(c, (lines_s2<true>(s2, IoCols{s2, T, T, gk}, B * s2, +1.0, tw, st)));
@@ -622,7 +636,7 @@ int zola256_run(zola_ctx* c, const Eval256& e, hipStream_t st) {
     HIP_TRY(c, (lines<kN4, false>(IoAdjStage2{s2, T, E, S1}, B * s2, -1.0, tw, st)));
     HIP_TRY(c, (lines<kN4, true>(IoAdjStage1{g, e.params, S1, qbar}, B * kW4, -1.0, tw, st)));
     if (e.dparams) {
-        hipLaunchKernelGGL(k_dparams256, dim3(B), dim3(256), 0, st, g, qbar, e.dparams);
+        hipLaunchKernelGGL(k_dparams256, dim3(B), dim3(kCropThreads), 0, st, g, qbar, e.dparams);
         HIP_TRY(c, hipGetLastError());
     }
     if (e.da && c->J > 0) {
