@@ -61,3 +61,28 @@ def test_phase_update_is_bit_exact(size_fft, R, J):
     upd.close()
     fresh.close()
     np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("size_fft,R,J", [(64, 31, 45), (140, 64, 45)])
+def test_backward_dev_accumulates_da(torch_cuda, size_fft, R, J):
+    """zola_psf_backward_dev adds dL/da to da (the calibration loops use an
+    internal overwriting form): two calls on the same da give twice one call."""
+    torch = torch_cuda
+    print(f"case: dL/da accumulation, size_fft {size_fft}, R {R}", flush=True)
+    ctx = Z.Context(Z.optics(size_fft=size_fft), R, _coeffs(J, 7))
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(8)
+    B = 5
+    par = torch.from_numpy(np.stack([rng.uniform(-0.1, 0.1, B), rng.uniform(-0.1, 0.1, B),
+                                     rng.uniform(-0.8, 0.8, B), np.zeros(B)], 1).astype(np.float32)).cuda()
+    w = torch.from_numpy(rng.normal(size=(B, R, R)).astype(np.float32)).cuda()
+    dp = torch.zeros(B, 4, device="cuda")
+    da1 = torch.zeros(J, dtype=torch.float64, device="cuda")
+    ctx.psf_backward_dev(par, w, dp, da1)
+    da2 = torch.full((J,), 0.5, dtype=torch.float64, device="cuda")
+    ctx.psf_backward_dev(par, w, dp, da2)
+    ctx.psf_backward_dev(par, w, dp, da2)
+    torch.cuda.synchronize()
+    ctx.close()
+    one, two = da1.cpu().numpy(), da2.cpu().numpy()
+    np.testing.assert_allclose(two, 0.5 + 2.0 * one, rtol=1e-12, atol=1e-12 * np.abs(one).max())
