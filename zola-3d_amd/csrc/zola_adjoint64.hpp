@@ -16,7 +16,7 @@
 // A calibration evaluation has a few hundred PSFs, so one wave per PSF would
 // leave most of the chip idle and take one wave's latency; the work of a PSF
 // is split over four waves:
-//  * forward: stage 1 on wave 0; stage 2, wave wr the output columns
+//  * forward: stage 1 by output-row parity (rows 4m + wr); stage 2, wave wr the output columns
 //    b = 4k + wr of every grid row (lane), kept in registers for the backward
 //    pass; row blur, wave wr a quarter of the crop columns; column blur and
 //    likelihood, wave wr a quarter of the ROI rows (lane = ROI column);
@@ -172,6 +172,44 @@ __device__ __forceinline__ void stage2_adj_split(float (&dr)[64 / NW], float (&d
     });
 }
 
+// Stage 1 of E for the grid rows a = 4m + H (H = 0..3, wave-uniform):
+//   S1(a, v) = sum_nu P(nu, v) exp(2 pi i nu a / 64), nu in [-16, 16).
+// With a = 4m + H the sum is a 16-point IDFT over m of the input twiddled by
+// exp(2 pi i nu H / 64) (folded into the phase) and folded mod 16.  Lane
+// (v, s) evaluates the 16 frequencies of half s (s = 0: 0..15, s = 1:
+// -16..-1); the two lanes of a column add each other's half (permlane32_swap),
+// so both hold the folded input, and each stores half of the 16 outputs.
+__device__ __forceinline__ void stage1_quarter64(const BlockTables64& T, WaveScratch64& S, int lane, int H, float X,
+                                                 float Y, float z, float zoil) {
+    const int v = lane & 31, s = lane >> 5;
+    const bool up = s != 0;
+    const float nv = (float)(v - 16);
+    const float Xh = X + (float)H * (float)(2.0 * kPi / 64.0);
+    const float cY = Y * (nv + 0.5f) + 0.5f * X;
+    float re[16], im[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int nu = k - 16 * s;
+        const int wi = (nu + 16) * kW + v;
+        const float q = T.phi[wi] + Xh * (float)nu + cY + T.kz[wi] * z - T.kzo[wi] * zoil;
+        float sn, cs;
+        sincos_reduced(q, sn, cs);
+        const float a = T.amp[wi];
+        const float pr = a * cs, pi = a * sn;
+        re[k] = pr + other_half(pr, up);
+        im[k] = pi + other_half(pi, up);
+    }
+    fft_dif<16, +1>(re, im);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        if ((m & 1) == s) {
+            const int a = 4 * m + H;
+            S.re[a * kS1 + v] = re[fidx<16>(m)];
+            S.im[a * kS1 + v] = im[fidx<16>(m)];
+        }
+    }
+}
+
 // Calls f(integral_constant<int, W>) for the wave index wr (wave-uniform).
 template <int NW, class F>
 __device__ __forceinline__ void per_wave(int wr, F&& f) {
@@ -219,8 +257,8 @@ __global__ __launch_bounds__(64 * NW) void k_psf_adjoint64(Model64 M, AdjArgs A)
     const float X = M.kappa * x, Y = M.kappa * y;
     const float h1reg = T.h3[lane];
 
-    // ---- forward: stage 1 (wave 0)
-    if (wr == 0) stage1_64(T, lds.S, lane, 0, X, Y, z, zoil, M.kappa);
+    // ---- forward: stage 1, waves 0..3 the grid rows of parity wr mod 4
+    if (wr < 4) stage1_quarter64(T, lds.S, lane, wr, X, Y, z, zoil);
     __syncthreads();
     // ---- stage 2: E(a, NW k + wr) of row a = lane, kept for the backward pass
     float Er[CPW], Ei[CPW];
