@@ -4,5 +4,4 @@
 namespace zola {
 template hipError_t launch_psf_forward128_r<20>(const Model128&, int, const float*, float*, hipStream_t);
 template hipError_t launch_fit128_r<20>(const Model128&, int, const FitArgs&, hipStream_t);
-template hipError_t launch_fit128o_r<20>(const Model128&, const FitArgs&, hipStream_t);
 }  // namespace zola
